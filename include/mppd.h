@@ -1,0 +1,125 @@
+/*
+ * mppd.h -- thin C ABI over micropp_amd's gfx950 HIP kernels.
+ *
+ * This is the layer the host micropp<3> class (and the test harness, through
+ * ctypes) calls for each step of the RVE hot path.  Each entry point names
+ * the reference function it replaces (paths relative to the Micropp tree).
+ *
+ * Conventions
+ *  - Every function returns an int status: MPPD_OK (0) or a negative code;
+ *    mppd_last_error() gives the message of the calling thread's last failure.
+ *  - Arrays named *_d are device pointers (allocate with mppd_malloc); arrays
+ *    named *_h or fixed-size arrays (eps[6], sig[6]) are host memory.
+ *  - Vectors indexed by DOF have nrow = 3 * nx * ny * nz entries in the
+ *    reference numbering (DOF = node * 3 + d, node = (k * ny + j) * nx + i).
+ *  - Internal variables have nvars = nelem * 8 * 7 entries, index
+ *    (e * 8 + gp) * 7 + v (reference include/params.hpp:41).
+ *  - Each solver owns one HIP stream; calls on one solver are serialised,
+ *    calls on different solvers may run concurrently.
+ */
+#ifndef MICROPP_AMD_MPPD_H
+#define MICROPP_AMD_MPPD_H
+
+#include <stddef.h>
+
+#include "material_base.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MPPD_OK = 0,
+  MPPD_ERR_ARG = -1,     /* invalid argument */
+  MPPD_ERR_HIP = -2,     /* HIP runtime failure */
+  MPPD_ERR_NOMEM = -3,   /* device allocation failed */
+  MPPD_ERR_NODEV = -4,   /* no usable gfx950 device */
+  MPPD_ERR_NOTCONV = -5  /* iteration limit reached (informational) */
+};
+
+typedef struct mppd_rve mppd_rve;
+typedef struct mppd_solver mppd_solver;
+
+const char *mppd_last_error(void);
+int mppd_device_count(int *n);
+
+/* ---- plain device memory (plumbing for the callers of this ABI) ---- */
+int mppd_malloc(void **ptr_d, size_t bytes);
+int mppd_free(void *ptr_d);
+int mppd_memcpy_h2d(void *dst_d, const void *src_h, size_t bytes);
+int mppd_memcpy_d2h(void *dst_h, const void *src_d, size_t bytes);
+int mppd_memcpy_d2d(void *dst_d, const void *src_d, size_t bytes);
+int mppd_memset(void *ptr_d, int value, size_t bytes);
+int mppd_synchronize(void);
+
+/* ---- RVE geometry ----
+ * Replaces the geometric part of the micropp<3> constructor: calc_bmat
+ * (src/micro3D.cpp:80-120), get_elem_type for every element
+ * (src/micropp.cpp:117-124, 338-544), calc_volume_fractions (:609-627) and
+ * make_material (src/material.cpp:26-46).  size[] counts nodes per side, as
+ * micropp_params_t::size does.  Fails with MPPD_ERR_ARG for an unknown
+ * micro_type or material type. */
+int mppd_rve_create(mppd_rve **out, const int size[3], int micro_type, const double geo[4],
+                    const struct material_base mats[3]);
+int mppd_rve_destroy(mppd_rve *r);
+/* nrow = 3 nx ny nz, nelem = (nx-1)(ny-1)(nz-1), nvars = 56 nelem */
+int mppd_rve_info(const mppd_rve *r, int *nrow, int *nelem, int *nvars);
+int mppd_rve_bmat(const mppd_rve *r, double *bmat_h /* [8][6][24] */);
+int mppd_rve_elem_types(const mppd_rve *r, int *et_h /* [nelem] */);
+int mppd_rve_volume_fractions(const mppd_rve *r, double vf_h[2] /* Vm, Vf */);
+
+/* ---- constitutive laws on n points ----
+ * material_t::get_stress / get_ctan / evolute (src/material.cpp:76-302).
+ * hist_d may be NULL (no history); sig_d, ctan_d (36 per point, ctan[j*6+i]
+ * = dsig_j/deps_i), hist_new_d (7 per point) and nl_d may each be NULL to
+ * skip that output. */
+int mppd_material_eval(const struct material_base *m, int n, const double *eps_d, const double *hist_d,
+                       double *sig_d, double *ctan_d, double *hist_new_d, int *nl_d);
+
+/* ---- one solver: ELL matrix + Jacobi-PCG workspace for one RVE ----
+ * ell_init (src/ell-common.cpp:34-139): 27-point stencil, 81 slots per row,
+ * out-of-grid neighbours point at column fj with a zero value. */
+int mppd_solver_create(mppd_solver **out, mppd_rve *r);
+int mppd_solver_destroy(mppd_solver *s);
+/* The matrix in the reference layout: cols_h/vals_h are [nrow][81]
+ * row-major (either may be NULL). */
+int mppd_ell_download(mppd_solver *s, int *cols_h, double *vals_h);
+int mppd_ell_upload_vals(mppd_solver *s, const double *vals_h);
+
+/* set_displ_bc (src/micro3D.cpp:26-78): u = eps_t x on the boundary nodes */
+int mppd_set_displ_bc(mppd_solver *s, const double eps[6], double *u_d);
+/* assembly_rhs (src/assembly.cpp:27-103): b = -(sum_e B^T sigma wg), zero on
+ * Dirichlet DOFs; *norm = ||b||_2 (may be NULL) */
+int mppd_assembly_rhs(mppd_solver *s, const double *u_d, const double *vars_d, double *b_d, double *norm);
+/* assembly_mat (src/assembly.cpp:105-121) with ell_add_3D / ell_set_bc_3D
+ * (src/ell-common.cpp:166-297): Jacobian into the solver's own matrix */
+int mppd_assembly_mat(mppd_solver *s, const double *u_d, const double *vars_d);
+/* ell_mvp (src/ell.cpp:35-44): y = A x */
+int mppd_ell_mvp(mppd_solver *s, const double *x_d, double *y_d);
+/* ell_solve_cgpd (src/ell.cpp:66-122): Jacobi-PCG from x = 0 on the solver's
+ * matrix (or on A0's matrix when A0 != NULL).  *its = iterations, *err = r.z
+ * at exit, as in the reference.  Convergence is decided on the device. */
+int mppd_cg(mppd_solver *s, const mppd_solver *A0, const double *b_d, double *x_d, double min_err, double rel_err,
+            int max_its, int *its, double *err);
+/* calc_ave_stress (src/average.cpp:57-82) */
+int mppd_ave_stress(mppd_solver *s, const double *u_d, const double *vars_d, double sig[6]);
+/* calc_vars_new (src/update.cpp:32-56); *nonlinear = OR of the evolute flags */
+int mppd_vars_new(mppd_solver *s, const double *u_d, const double *vars_old_d, double *vars_new_d, int *nonlinear);
+
+/* newton_raphson (src/solve.cpp:28-82) */
+typedef struct {
+  int nr_max_its;
+  double nr_max_tol, nr_rel_tol;
+  int cg_max_its;
+  double cg_abs_tol, cg_rel_tol;
+  int its_with_A0; /* used when A0 != NULL */
+} mppd_newton_params;
+/* out = {its, solver_its, converged}.  u_d is in/out (warm start kept on the
+ * interior, boundary overwritten by eps). */
+int mppd_newton(mppd_solver *s, const mppd_solver *A0, const double eps[6], const double *vars_old_d, double *u_d,
+                const mppd_newton_params *p, int out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,473 @@
+"""micropp_amd -- MI355X (gfx950) implementation of Micropp's RVE homogenization hot path.
+
+Python mirror of the C interface in include/micropp_c.h (the reference's
+``micropp3_*`` API, src/micropp_c.cpp) and of the kernel-level ABI in
+include/mppd.h, loaded with ctypes from the in-tree ``libmicropp_amd.so``.
+
+The library is the product; this module only binds it.  If the shared
+library is missing, importing the bindings raises: there is no fallback.
+Build it with ``python -m micropp_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmicropp_amd.so")
+
+# reference include/types.hpp:85-99 (+ the seeded sphere list), :115
+(MIC_HOMOGENEOUS, MIC_SPHERE, MIC_LAYER_Y, MIC_CILI_FIB_X, MIC_CILI_FIB_Z, MIC_CILI_FIB_XZ, MIC_QUAD_FIB_XYZ,
+ MIC_QUAD_FIB_XZ, MIC_QUAD_FIB_XZ_BROKEN_X, MIC3D_SPHERES, MIC3D_8, MIC3D_FIBS_20_ORDER, MIC3D_FIBS_20_DISORDER,
+ MIC3D_RANDOM_SPHERES) = range(14)
+FE_LINEAR, FE_ONE_WAY, FE_FULL, MIX_RULE_CHAMIS = range(4)
+MATERIAL_ELASTIC, MATERIAL_PLASTIC, MATERIAL_DAMAGE = range(3)
+NUM_VAR_GP = 7
+
+
+class MaterialBase(C.Structure):
+    """``struct material_base`` (include/material_base.h)."""
+
+    _fields_ = [(n, C.c_double) for n in ("E", "nu", "Ka", "Sy", "k", "mu", "lambda_", "Xt")] + [
+        ("type", C.c_int)
+    ]
+
+
+class Micropp3Params(C.Structure):
+    """``struct micropp3_params`` (include/micropp_c.h)."""
+
+    _fields_ = [
+        ("ngp", C.c_int),
+        ("size", C.c_int * 3),
+        ("type", C.c_int),
+        ("geo_params", C.c_double * 4),
+        ("materials", MaterialBase * 3),
+        ("coupling", C.POINTER(C.c_int)),
+        ("subiterations", C.c_int),
+        ("nsubiterations", C.c_int),
+        ("mpi_rank", C.c_int),
+        ("nr_max_its", C.c_int),
+        ("nr_max_tol", C.c_double),
+        ("nr_rel_tol", C.c_double),
+        ("cg_max_its", C.c_int),
+        ("cg_abs_tol", C.c_double),
+        ("cg_rel_tol", C.c_double),
+        ("calc_ctan_lin", C.c_int),
+        ("use_A0", C.c_int),
+        ("its_with_A0", C.c_int),
+        ("lin_stress", C.c_int),
+        ("write_log", C.c_int),
+    ]
+
+
+class Micropp3Handle(C.Structure):
+    _fields_ = [("ptr", C.c_void_p)]
+
+
+class NewtonParams(C.Structure):
+    """``mppd_newton_params`` (include/mppd.h)."""
+
+    _fields_ = [("nr_max_its", C.c_int), ("nr_max_tol", C.c_double), ("nr_rel_tol", C.c_double),
+                ("cg_max_its", C.c_int), ("cg_abs_tol", C.c_double), ("cg_rel_tol", C.c_double),
+                ("its_with_A0", C.c_int)]
+
+
+# every symbol include/*.h declares, with (restype, argtypes)
+_VP, _I, _D, _PD, _PI = C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)
+_H = C.POINTER(Micropp3Handle)
+SIGNATURES = {
+    # material_base.h
+    "material_set": (None, [C.POINTER(MaterialBase), _I, _D, _D, _D, _D, _D]),
+    "material_print": (None, [C.POINTER(MaterialBase)]),
+    # micropp_c.h
+    "micropp3_new": (None, [_H, _I, _PI, _I, _PD, C.POINTER(MaterialBase), _PI, _I, _I]),
+    "micropp3_free": (None, [_H]),
+    "micropp3_set_strain": (None, [_H, _I, _PD]),
+    "micropp3_get_stress": (None, [_H, _I, _PD]),
+    "micropp3_get_ctan": (None, [_H, _I, _PD]),
+    "micropp3_homogenize": (None, [_H]),
+    "micropp3_homogenize_linear": (None, [_H]),
+    "micropp3_update_vars": (None, [_H]),
+    "micropp3_is_non_linear": (C.c_bool, [_H, _I]),
+    "micropp3_get_cost": (_I, [_H, _I]),
+    "micropp3_has_converged": (C.c_bool, [_H, _I]),
+    "micropp3_has_subiterated": (C.c_bool, [_H, _I]),
+    "micropp3_output": (None, [_H, _I, C.c_char_p]),
+    "micropp3_print_info": (None, [_H]),
+    "micropp3_get_non_linear_gps": (_I, [_H]),
+    "micropp3_write_restart": (None, [_H, _I]),
+    "micropp3_read_restart": (None, [_H, _I]),
+    "micropp3_output2": (None, [_H, _I, _I, _I]),
+    "micropp3_params_default": (None, [C.POINTER(Micropp3Params)]),
+    "micropp3_new_ext": (None, [_H, C.POINTER(Micropp3Params)]),
+    "micropp3_get_state": (_I, [_H, _I, _I, _PD]),
+    "micropp3_get_ctan_lin": (None, [_H, _PD]),
+    "micropp3_get_volume_fractions": (None, [_H, _PD]),
+    # mppd.h
+    "mppd_last_error": (C.c_char_p, []),
+    "mppd_device_count": (_I, [_PI]),
+    "mppd_malloc": (_I, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "mppd_free": (_I, [_VP]),
+    "mppd_memcpy_h2d": (_I, [_VP, _VP, C.c_size_t]),
+    "mppd_memcpy_d2h": (_I, [_VP, _VP, C.c_size_t]),
+    "mppd_memcpy_d2d": (_I, [_VP, _VP, C.c_size_t]),
+    "mppd_memset": (_I, [_VP, _I, C.c_size_t]),
+    "mppd_synchronize": (_I, []),
+    "mppd_rve_create": (_I, [C.POINTER(C.c_void_p), _PI, _I, _PD, C.POINTER(MaterialBase)]),
+    "mppd_rve_destroy": (_I, [_VP]),
+    "mppd_rve_info": (_I, [_VP, _PI, _PI, _PI]),
+    "mppd_rve_bmat": (_I, [_VP, _PD]),
+    "mppd_rve_elem_types": (_I, [_VP, _PI]),
+    "mppd_rve_volume_fractions": (_I, [_VP, _PD]),
+    "mppd_material_eval": (_I, [C.POINTER(MaterialBase), _I, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "mppd_solver_create": (_I, [C.POINTER(C.c_void_p), _VP]),
+    "mppd_solver_destroy": (_I, [_VP]),
+    "mppd_ell_download": (_I, [_VP, _PI, _PD]),
+    "mppd_ell_upload_vals": (_I, [_VP, _PD]),
+    "mppd_set_displ_bc": (_I, [_VP, _PD, _VP]),
+    "mppd_assembly_rhs": (_I, [_VP, _VP, _VP, _VP, _PD]),
+    "mppd_assembly_mat": (_I, [_VP, _VP, _VP]),
+    "mppd_ell_mvp": (_I, [_VP, _VP, _VP]),
+    "mppd_cg": (_I, [_VP, _VP, _VP, _VP, _D, _D, _I, _PI, _PD]),
+    "mppd_ave_stress": (_I, [_VP, _VP, _VP, _PD]),
+    "mppd_vars_new": (_I, [_VP, _VP, _VP, _VP, _PI]),
+    "mppd_newton": (_I, [_VP, _VP, _PD, _VP, _VP, C.POINTER(NewtonParams), _PI]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load libmicropp_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `python -m micropp_amd.build`")
+        lib = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class MppdError(RuntimeError):
+    pass
+
+
+def _chk(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().mppd_last_error().decode(errors="replace")
+        raise MppdError(f"{what} failed ({rc}): {msg}")
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_PD)
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(_PI)
+
+
+def gpu_available() -> bool:
+    n = C.c_int(0)
+    try:
+        return load().mppd_device_count(C.byref(n)) == 0 and n.value > 0
+    except (RuntimeError, OSError):
+        return False
+
+
+def material(type_, E, nu, Ka=0.0, Sy=0.0, Xt=0.0) -> MaterialBase:
+    """material_set through the library (reference src/material.c:26-38)."""
+    m = MaterialBase()
+    load().material_set(C.byref(m), int(type_), float(E), float(nu), float(Ka), float(Sy), float(Xt))
+    return m
+
+
+# ---------------------------------------------------------------------------
+# micropp3 (include/micropp_c.h)
+# ---------------------------------------------------------------------------
+class Micropp3:
+    """One ``micropp<3>`` instance through the C interface.
+
+    Keyword arguments are the fields of ``micropp_params_t``; the defaults are
+    the reference's (include/types.hpp:43-83).
+    """
+
+    def __init__(self, size, micro_type=MIC_HOMOGENEOUS, geo_params=(0.1, 0.1, 0.1, 0.1), materials=None, ngp=1,
+                 coupling=None, **kw):
+        lib = load()
+        p = Micropp3Params()
+        lib.micropp3_params_default(C.byref(p))
+        p.ngp = ngp
+        p.size[:] = list((size,) * 3 if np.isscalar(size) else size)
+        p.type = micro_type
+        p.geo_params[:] = list(geo_params)
+        mats = materials or [material(0, 1e6, 0.3)] * 3
+        for i in range(3):
+            p.materials[i] = mats[i]
+        self._coupling = None
+        if coupling is not None:
+            self._coupling = (C.c_int * ngp)(*coupling)
+            p.coupling = C.cast(self._coupling, _PI)
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise TypeError(f"unknown parameter {k}")
+            setattr(p, k, v)
+        self.params = p
+        self.h = Micropp3Handle()
+        lib.micropp3_new_ext(C.byref(self.h), C.byref(p))
+        if not self.h.ptr:
+            raise MppdError("micropp3_new_ext failed (see stderr)")
+        self.ngp = ngp
+        nx, ny, nz = p.size
+        self.nndim = 3 * nx * ny * nz
+        self.nvars = (nx - 1) * (ny - 1) * (nz - 1) * 8 * NUM_VAR_GP
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.ptr:
+            load().micropp3_free(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_strain(self, gp, eps):
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        load().micropp3_set_strain(C.byref(self.h), gp, _dp(e))
+
+    def get_stress(self, gp):
+        s = np.zeros(6)
+        load().micropp3_get_stress(C.byref(self.h), gp, _dp(s))
+        return s
+
+    def get_ctan(self, gp):
+        c = np.zeros(36)
+        load().micropp3_get_ctan(C.byref(self.h), gp, _dp(c))
+        return c.reshape(6, 6)
+
+    def homogenize(self):
+        load().micropp3_homogenize(C.byref(self.h))
+
+    def homogenize_linear(self):
+        load().micropp3_homogenize_linear(C.byref(self.h))
+
+    def update_vars(self):
+        load().micropp3_update_vars(C.byref(self.h))
+
+    def is_non_linear(self, gp):
+        return int(load().micropp3_is_non_linear(C.byref(self.h), gp))
+
+    def get_non_linear_gps(self):
+        return load().micropp3_get_non_linear_gps(C.byref(self.h))
+
+    def get_cost(self, gp):
+        return load().micropp3_get_cost(C.byref(self.h), gp)
+
+    def has_converged(self, gp):
+        return bool(load().micropp3_has_converged(C.byref(self.h), gp))
+
+    def has_subiterated(self, gp):
+        return bool(load().micropp3_has_subiterated(C.byref(self.h), gp))
+
+    def write_restart(self, restart_id):
+        load().micropp3_write_restart(C.byref(self.h), restart_id)
+
+    def read_restart(self, restart_id):
+        load().micropp3_read_restart(C.byref(self.h), restart_id)
+
+    def print_info(self):
+        load().micropp3_print_info(C.byref(self.h))
+
+    def state(self, gp, which):
+        n = self.nndim if which < 2 else self.nvars
+        out = np.zeros(n)
+        ok = load().micropp3_get_state(C.byref(self.h), gp, which, _dp(out))
+        return out if ok else None
+
+    def ctan_lin(self):
+        c = np.zeros(36)
+        load().micropp3_get_ctan_lin(C.byref(self.h), _dp(c))
+        return c.reshape(6, 6)
+
+    def volume_fractions(self):
+        v = np.zeros(2)
+        load().micropp3_get_volume_fractions(C.byref(self.h), _dp(v))
+        return v
+
+
+# ---------------------------------------------------------------------------
+# kernel-level ABI (include/mppd.h)
+# ---------------------------------------------------------------------------
+class DeviceArray:
+    """A device buffer (mppd_malloc) with numpy round trips."""
+
+    def __init__(self, n, dtype=np.float64, host=None):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        _chk(load().mppd_malloc(C.byref(p), max(self.nbytes, 1)), "mppd_malloc")
+        self.ptr = p
+        if host is not None:
+            self.upload(host)
+        else:
+            _chk(load().mppd_memset(self.ptr, 0, max(self.nbytes, 1)), "mppd_memset")
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        return cls(a.size, a.dtype, a)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        _chk(load().mppd_memcpy_h2d(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes), "h2d")
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        _chk(load().mppd_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().mppd_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _ptr(a):
+    return None if a is None else a.ptr
+
+
+class Rve:
+    """mppd_rve: geometry, phase map and shape functions of one RVE on the GPU."""
+
+    def __init__(self, size, micro_type, geo, materials):
+        s = np.array(list((size,) * 3 if np.isscalar(size) else size), dtype=np.int32)
+        g = np.ascontiguousarray(geo, dtype=np.float64)
+        mats = (MaterialBase * 3)(*materials)
+        h = C.c_void_p()
+        _chk(load().mppd_rve_create(C.byref(h), _ip(s), int(micro_type), _dp(g), mats), "mppd_rve_create")
+        self.h = h
+        nrow, nelem, nvars = C.c_int(), C.c_int(), C.c_int()
+        _chk(load().mppd_rve_info(h, C.byref(nrow), C.byref(nelem), C.byref(nvars)), "mppd_rve_info")
+        self.nrow, self.nelem, self.nvars = nrow.value, nelem.value, nvars.value
+        self.size = tuple(int(v) for v in s)
+
+    def bmat(self):
+        b = np.zeros(8 * 6 * 24)
+        _chk(load().mppd_rve_bmat(self.h, _dp(b)), "mppd_rve_bmat")
+        return b.reshape(8, 6, 24)
+
+    def elem_types(self):
+        et = np.zeros(self.nelem, dtype=np.int32)
+        _chk(load().mppd_rve_elem_types(self.h, _ip(et)), "mppd_rve_elem_types")
+        return et
+
+    def volume_fractions(self):
+        v = np.zeros(2)
+        _chk(load().mppd_rve_volume_fractions(self.h, _dp(v)), "mppd_rve_volume_fractions")
+        return v
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().mppd_rve_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Solver:
+    """mppd_solver: the ELL matrix and CG workspace of one RVE."""
+
+    def __init__(self, rve: Rve):
+        h = C.c_void_p()
+        _chk(load().mppd_solver_create(C.byref(h), rve.h), "mppd_solver_create")
+        self.h = h
+        self.rve = rve
+        self.nrow = rve.nrow
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().mppd_solver_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def ell(self):
+        cols = np.zeros(self.nrow * 81, dtype=np.int32)
+        vals = np.zeros(self.nrow * 81)
+        _chk(load().mppd_ell_download(self.h, _ip(cols), _dp(vals)), "mppd_ell_download")
+        return cols.reshape(-1, 81), vals.reshape(-1, 81)
+
+    def upload_vals(self, vals):
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        _chk(load().mppd_ell_upload_vals(self.h, _dp(v)), "mppd_ell_upload_vals")
+
+    def set_displ_bc(self, eps, u: DeviceArray):
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        _chk(load().mppd_set_displ_bc(self.h, _dp(e), u.ptr), "mppd_set_displ_bc")
+
+    def assembly_rhs(self, u: DeviceArray, vars_: DeviceArray | None, b: DeviceArray):
+        n = C.c_double()
+        _chk(load().mppd_assembly_rhs(self.h, u.ptr, _ptr(vars_), b.ptr, C.byref(n)), "mppd_assembly_rhs")
+        return n.value
+
+    def assembly_mat(self, u: DeviceArray | None, vars_: DeviceArray | None = None):
+        _chk(load().mppd_assembly_mat(self.h, _ptr(u), _ptr(vars_)), "mppd_assembly_mat")
+
+    def mvp(self, x: DeviceArray, y: DeviceArray):
+        _chk(load().mppd_ell_mvp(self.h, x.ptr, y.ptr), "mppd_ell_mvp")
+
+    def cg(self, b: DeviceArray, x: DeviceArray, min_err=1e-50, rel_err=1e-5, max_its=1000, A0=None):
+        its, err = C.c_int(), C.c_double()
+        _chk(load().mppd_cg(self.h, None if A0 is None else A0.h, b.ptr, x.ptr, min_err, rel_err, max_its,
+                            C.byref(its), C.byref(err)), "mppd_cg")
+        return its.value, err.value
+
+    def ave_stress(self, u: DeviceArray, vars_: DeviceArray | None = None):
+        s = np.zeros(6)
+        _chk(load().mppd_ave_stress(self.h, u.ptr, _ptr(vars_), _dp(s)), "mppd_ave_stress")
+        return s
+
+    def vars_new(self, u: DeviceArray, vars_old: DeviceArray | None, vars_new: DeviceArray):
+        nl = C.c_int()
+        _chk(load().mppd_vars_new(self.h, u.ptr, _ptr(vars_old), vars_new.ptr, C.byref(nl)), "mppd_vars_new")
+        return nl.value
+
+    def newton(self, eps, u: DeviceArray, vars_old: DeviceArray | None = None, A0=None, nr_max_its=4,
+               nr_max_tol=1e-10, nr_rel_tol=1e-3, cg_max_its=1000, cg_abs_tol=1e-50, cg_rel_tol=1e-5,
+               its_with_A0=1):
+        p = NewtonParams(nr_max_its, nr_max_tol, nr_rel_tol, cg_max_its, cg_abs_tol, cg_rel_tol, its_with_A0)
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        out = np.zeros(3, dtype=np.int32)
+        _chk(load().mppd_newton(self.h, None if A0 is None else A0.h, _dp(e), _ptr(vars_old), u.ptr, C.byref(p),
+                                _ip(out)), "mppd_newton")
+        return tuple(int(v) for v in out)
+
+
+def material_eval(m: MaterialBase, eps: np.ndarray, hist: np.ndarray | None = None):
+    """Stress, tangent and history update of ``m`` at n points, on the GPU."""
+    eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1, 6)
+    n = eps.shape[0]
+    de = DeviceArray.from_numpy(eps)
+    dh = DeviceArray.from_numpy(np.ascontiguousarray(hist, dtype=np.float64)) if hist is not None else None
+    ds, dc, dhn = DeviceArray(n * 6), DeviceArray(n * 36), DeviceArray(n * NUM_VAR_GP)
+    dnl = DeviceArray(n, np.int32)
+    _chk(load().mppd_material_eval(C.byref(m), n, de.ptr, _ptr(dh), ds.ptr, dc.ptr, dhn.ptr, dnl.ptr),
+         "mppd_material_eval")
+    return (ds.numpy().reshape(n, 6), dc.numpy().reshape(n, 6, 6), dhn.numpy().reshape(n, NUM_VAR_GP),
+            dnl.numpy())

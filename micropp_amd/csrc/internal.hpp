@@ -1,0 +1,105 @@
+// internal.hpp -- device-side data layout of micropp_amd and the launchers
+// shared by the .hip translation units.
+//
+// ELL on the device (SELL-64): rows are grouped into slices of 64 (one
+// wavefront); inside a slice the 81 slots are slot-major, so slot s of the 64
+// rows of slice q sits at [(q * 81 + s) * 64 + lane].  One wavefront streams
+// a slice with 81 fully coalesced 512-byte loads of values and 256-byte loads
+// of columns.  Row and slot numbering are the reference's (row = node*3 + fi,
+// slot = nb*3 + fj, nb = (dz+1)*9 + (dy+1)*3 + (dx+1)); rows are padded to a
+// multiple of 256 with zero values and column 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "material_base.h"
+#include "mppd.h"
+
+namespace mpp {
+
+constexpr int kSlots = 81;
+constexpr int kBlock = 256;     // threads per block of the row-space kernels
+constexpr int kRowPad = 256;    // row padding (a block of rows)
+constexpr int kMaxSpheres = 64; // MIC3D_RANDOM_SPHERES list length
+constexpr int kNumMicroTypes = 14;     // MIC_HOMOGENEOUS .. MIC3D_RANDOM_SPHERES
+constexpr int kMicRandomSpheres = 13;  // MIC3D_RANDOM_SPHERES
+
+// Geometry passed by value to kernels.
+struct Grid {
+  int nx, ny, nz;
+  int nex, ney, nez;
+  int nn, nrow, nrow_pad, nelem;
+  double dx, dy, dz, wg, vol_tot, evol;
+};
+
+// Jacobi-PCG scalars of one system (device resident).
+struct CgState {
+  double rz, pn0, pn, alpha, beta, min_err, rel_err, pAp;
+  int its, max_its, done, pad;
+  unsigned cnt;
+  unsigned pad2[3];
+};
+
+// Status of an averaging / norm reduction.
+struct RedState {
+  double v[8];
+  unsigned cnt;
+  int flag;
+  unsigned pad[2];
+};
+
+size_t round_up(size_t v, size_t m);
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define MPP_HIP(call)                                  \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return mpp::hip_fail(_e, #call); \
+  } while (0)
+
+// ---------------- launchers (geometry.hip) ----------------
+hipError_t launch_bmat(const Grid &g, double *bmat, hipStream_t st);
+hipError_t launch_random_spheres(const double *geo, double *spheres, hipStream_t st);
+hipError_t launch_phase_map(const Grid &g, int type, const double *geo, const double *spheres, int nsph, int *et,
+                            int *bad, hipStream_t st);
+hipError_t launch_volume_fractions(const Grid &g, const int *et, double *vf, hipStream_t st);
+hipError_t launch_ell_cols(const Grid &g, int *cols, hipStream_t st);
+hipError_t launch_sell_to_rows(const Grid &g, const int *cols, const double *vals, int *cols_rm, double *vals_rm,
+                               hipStream_t st);
+hipError_t launch_rows_to_sell(const Grid &g, const double *vals_rm, double *vals, hipStream_t st);
+
+// ---------------- launchers (assembly.hip) ----------------
+hipError_t launch_material_eval(const material_base &m, int n, const double *eps, const double *hist, double *sig,
+                                double *ctan, double *hist_new, int *nl, hipStream_t st);
+hipError_t launch_set_displ_bc(const Grid &g, const double *eps6, double *u, hipStream_t st);
+// Element stiffness: per phase (phase_mode: one 24x24 block per material,
+// u/vars ignored) or per element.
+hipError_t launch_elem_ke(const Grid &g, const double *bmat, const material_base *mats, const int *et, const double *u,
+                          const double *vars, int phase_mode, double *ke, hipStream_t st);
+hipError_t launch_gather_vals(const Grid &g, const int *et, const double *ke, int phase_mode, double *vals,
+                              hipStream_t st);
+hipError_t launch_elem_be(const Grid &g, const double *bmat, const material_base *mats, const int *et, const double *u,
+                          const double *vars, double *be, hipStream_t st);
+hipError_t launch_gather_rhs(const Grid &g, const double *be, double *b, double *partials, RedState *red,
+                             hipStream_t st);
+hipError_t launch_ave_stress(const Grid &g, const double *bmat, const material_base *mats, const int *et,
+                             const double *u, const double *vars, double *partials, RedState *red, hipStream_t st);
+hipError_t launch_vars_new(const Grid &g, const double *bmat, const material_base *mats, const int *et, const double *u,
+                           const double *vars_old, double *vars_new, RedState *red, hipStream_t st);
+
+// ---------------- launchers (solver.hip) ----------------
+hipError_t launch_spmv(const Grid &g, const int *cols, const double *vals, const double *x, double *y, hipStream_t st);
+hipError_t launch_cg_init(const Grid &g, const int *cols, const double *vals, const double *b, double *x, double *r,
+                          double *z, double *p, double *kinv, double *partials, CgState *cg, hipStream_t st);
+hipError_t launch_cg_iter(const Grid &g, const int *cols, const double *vals, double *x, double *r, double *z,
+                          double *p, double *Ap, const double *kinv, double *partials, CgState *cg, hipStream_t st);
+hipError_t launch_axpy(int n, double a, const double *x, double *y, hipStream_t st);
+hipError_t launch_copy(int n, const double *x, double *y, hipStream_t st);
+
+}  // namespace mpp

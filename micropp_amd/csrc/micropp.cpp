@@ -1,0 +1,489 @@
+// micropp.cpp -- micropp<3> on the GPU: the reference's public API
+// (include/micropp.hpp:176-217) driving the mppd_* kernels.
+//
+// The per-GP state (u_n, u_k, vars_n, vars_k) lives in device memory; every
+// finite-element operation (boundary values, assembly, CG, averages,
+// history) runs in the gfx950 kernels.  Host code keeps the reference's
+// control flow: the Newton loop, subiterations, the FE_FULL perturbation loop,
+// cost bookkeeping, and the 6x6 products of homogenize_linear.
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+
+#include "micropp.hpp"
+#include "mppd.h"
+
+namespace {
+
+void check(int rc, const char *what) {
+  if (rc != MPPD_OK) throw std::runtime_error(std::string("micropp_amd: ") + what + ": " + mppd_last_error());
+}
+
+double *dzeros(size_t n) {
+  void *p = nullptr;
+  check(mppd_malloc(&p, n * sizeof(double)), "device allocation");
+  check(mppd_memset(p, 0, n * sizeof(double)), "device memset");
+  return static_cast<double *>(p);
+}
+
+struct gp_state {
+  double strain_old[6] = {0, 0, 0, 0, 0, 0};
+  double strain[6] = {0, 0, 0, 0, 0, 0};
+  double stress[6] = {0, 0, 0, 0, 0, 0};
+  double ctan[36] = {0};
+  bool allocated = false;
+  double *vars_n = nullptr, *vars_k = nullptr;  // device, nvars
+  double *u_n = nullptr, *u_k = nullptr;        // device, nndim
+  long cost = 0;
+  bool converged = true;
+  bool subiterated = false;
+  int coupling = FE_ONE_WAY;
+};
+
+}  // namespace
+
+template <int tdim>
+struct micropp<tdim>::impl {
+  int ngp, nx, ny, nz, nn, nndim, nelem, nvars;
+  int micro_type;
+  double geo[4];
+  material_base mats[MAX_MATERIALS];
+  int nsubiterations, mpi_rank;
+  bool subiterations;
+  mppd_newton_params np;
+  bool calc_ctan_lin_flag, use_A0, lin_stress, write_log_flag;
+  int log_id = 0;
+  ofstream log;
+  double Vm = 0, Vf = 0;
+  double ctan_lin_fe[36] = {0};
+  std::vector<gp_state> gp;
+  mppd_rve *rve = nullptr;
+  mppd_solver *sol = nullptr;
+  mppd_solver *A0 = nullptr;
+  double *u = nullptr;        // working displacement (device)
+  double *vars_aux = nullptr;  // vars_new when the GP has no history yet (device)
+
+  ~impl() {
+    for (auto &q : gp)
+      for (double *p : {q.vars_n, q.vars_k, q.u_n, q.u_k}) mppd_free(p);
+    mppd_free(u);
+    mppd_free(vars_aux);
+    mppd_solver_destroy(A0);
+    mppd_solver_destroy(sol);
+    mppd_rve_destroy(rve);
+  }
+
+  newton_t newton(const double eps[6], const double *vars_old) {
+    int out[3] = {0, 0, 0};
+    check(mppd_newton(sol, use_A0 ? A0 : nullptr, eps, vars_old, u, &np, out), "newton_raphson");
+    newton_t n;
+    n.its = out[0];
+    n.solver_its = out[1];
+    n.converged = out[2] != 0;
+    return n;
+  }
+
+  void ave_stress(const double *ud, const double *vars, double sig[6]) {
+    check(mppd_ave_stress(sol, ud, vars, sig), "calc_ave_stress");
+  }
+
+  void copy(double *dst, const double *src) {
+    check(mppd_memcpy_d2d(dst, src, sizeof(double) * nndim), "device copy");
+  }
+
+  // calc_ctan_lin_fe_models (src/micropp.cpp:255-284)
+  void ctan_lin_fe_models() {
+    for (int i = 0; i < 6; ++i) {
+      check(mppd_memset(u, 0, sizeof(double) * nndim), "device memset");
+      double eps[6] = {0, 0, 0, 0, 0, 0}, sig[6];
+      eps[i] += D_EPS_CTAN_AVE;
+      newton(eps, nullptr);
+      ave_stress(u, nullptr, sig);
+      for (int v = 0; v < 6; ++v) ctan_lin_fe[v * 6 + i] = sig[v] / D_EPS_CTAN_AVE;
+    }
+  }
+
+  // homogenize_linear(gp) (src/homogenize.cpp:101-109)
+  static void linear(gp_state &q) {
+    for (int i = 0; i < 6; ++i) q.stress[i] = 0.0;
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) q.stress[i] += q.ctan[i * 6 + j] * q.strain[j];
+  }
+
+  // homogenize_fe_one_way / homogenize_fe_full (src/homogenize.cpp:111-282)
+  void fe(gp_state &q, bool full) {
+    check(mppd_memset(vars_aux, 0, sizeof(double) * nvars), "device memset");
+    double *vars_new = q.allocated ? q.vars_k : vars_aux;
+    q.cost = 0;
+    q.subiterated = false;
+    copy(u, q.u_n);
+    newton_t n = newton(q.strain, q.vars_n);
+    copy(q.u_k, u);
+    q.cost += n.solver_its;
+    q.converged = n.converged;
+    if (!q.converged && subiterations) {
+      double eps_sub[6], deps[6];
+      copy(u, q.u_n);
+      for (int i = 0; i < 6; ++i) eps_sub[i] = q.strain_old[i];
+      q.subiterated = true;
+      for (int i = 0; i < 6; ++i) deps[i] = (q.strain[i] - q.strain_old[i]) / nsubiterations;
+      for (int s = 0; s < nsubiterations; ++s) {
+        for (int j = 0; j < 6; ++j) eps_sub[j] += deps[j];
+        n = newton(eps_sub, q.vars_n);
+        q.cost += n.solver_its;
+      }
+      q.converged = n.converged;
+      copy(q.u_k, u);
+    }
+    if (lin_stress)
+      linear(q);
+    else
+      ave_stress(q.u_k, q.vars_n, q.stress);
+    int nl = 0;
+    check(mppd_vars_new(sol, q.u_k, q.vars_n, vars_new, &nl), "calc_vars_new");
+    if (nl && !q.allocated) {
+      q.vars_n = dzeros(nvars);
+      q.vars_k = dzeros(nvars);
+      q.allocated = true;
+      check(mppd_memcpy_d2d(q.vars_k, vars_new, sizeof(double) * nvars), "device copy");
+    }
+    if (full && q.allocated) {
+      double eps1[6], sig0[6], sig1[6];
+      copy(u, q.u_k);
+      for (int v = 0; v < 6; ++v) sig0[v] = q.stress[v];
+      for (int i = 0; i < 6; ++i) {
+        for (int v = 0; v < 6; ++v) eps1[v] = q.strain[v];
+        eps1[i] += D_EPS_CTAN_AVE;
+        newton(eps1, q.vars_n);
+        q.cost += n.solver_its;  // the reference adds the sigma solve's count again
+        ave_stress(u, q.vars_n, sig1);
+        for (int v = 0; v < 6; ++v) q.ctan[v * 6 + i] = (sig1[v] - sig0[v]) / D_EPS_CTAN_AVE;
+      }
+    }
+  }
+
+  void chamis(double c[36]) const;
+};
+
+// calc_ctan_lin_mix_rule_Chamis (src/micropp.cpp:286-331) and invert_3x3
+// (include/util.hpp:145-163): a closed-form 6x6, no FE work.
+template <int tdim>
+void micropp<tdim>::impl::chamis(double c[36]) const {
+  const double Em = mats[0].E, nu_m = mats[0].nu, Gm = Em / (2 * (1 + nu_m));
+  const double Ef = mats[1].E, nu_f = mats[1].nu, Gf = Ef / (2 * (1 + nu_f));
+  const double E11 = Vf * Ef + Vm * Em;
+  const double E22 = Em / (1 - sqrt(Vf) * (1 - Em / Ef));
+  const double nu12 = Vf * nu_f + Vm * nu_m;
+  const double G12 = Gm / (1 - sqrt(Vf) * (1 - Gm / Gf));
+  const double nu23 = nu12;
+  const double m[3][3] = {{1 / E11, -nu12 / E11, -nu12 / E11},
+                          {-nu12 / E11, 1 / E22, -nu23 / E22},
+                          {-nu12 / E11, -nu23 / E22, 1 / E22}};
+  const double det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+                     m[0][1] * (m[1][0] * m[2][2] - m[2][0] * m[1][2]) +
+                     m[0][2] * (m[1][0] * m[2][1] - m[2][0] * m[1][1]);
+  double o[3][3];
+  o[0][0] = +(m[1][1] * m[2][2] - m[2][1] * m[1][2]) / det;
+  o[0][1] = -(m[1][0] * m[2][2] - m[2][0] * m[1][2]) / det;
+  o[0][2] = +(m[1][0] * m[2][1] - m[2][0] * m[1][1]) / det;
+  o[1][0] = -(m[0][1] * m[2][2] - m[2][1] * m[0][2]) / det;
+  o[1][1] = +(m[0][0] * m[2][2] - m[2][0] * m[0][2]) / det;
+  o[1][2] = -(m[0][0] * m[2][1] - m[2][0] * m[0][1]) / det;
+  o[2][0] = +(m[0][1] * m[1][2] - m[1][1] * m[0][2]) / det;
+  o[2][1] = -(m[0][0] * m[1][2] - m[1][0] * m[0][2]) / det;
+  o[2][2] = +(m[0][0] * m[1][1] - m[1][0] * m[0][1]) / det;
+  for (int i = 0; i < 36; ++i) c[i] = 0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) c[i * 6 + j] = o[i][j];
+  for (int i = 3; i < 6; ++i) c[i * 6 + i] = G12;
+}
+
+// Constructor (src/micropp.cpp:29-183)
+template <int tdim>
+micropp<tdim>::micropp(const micropp_params_t &params) : p_(new impl()) {
+  impl &s = *p_;
+  s.ngp = params.ngp;
+  s.nx = params.size[0];
+  s.ny = params.size[1];
+  s.nz = params.size[2];
+  s.nn = s.nx * s.ny * s.nz;
+  s.nndim = s.nn * 3;
+  s.nelem = (s.nx - 1) * (s.ny - 1) * (s.nz - 1);
+  s.nvars = s.nelem * 8 * NUM_VAR_GP;
+  s.micro_type = params.type;
+  for (int i = 0; i < 4; ++i) s.geo[i] = params.geo_params[i];
+  for (int i = 0; i < MAX_MATERIALS; ++i) s.mats[i] = params.materials[i];
+  s.subiterations = params.subiterations;
+  s.nsubiterations = params.nsubiterations;
+  s.mpi_rank = params.mpi_rank;
+  s.np.nr_max_its = params.nr_max_its;
+  s.np.nr_max_tol = params.nr_max_tol;
+  s.np.nr_rel_tol = params.nr_rel_tol;
+  s.np.cg_max_its = params.cg_max_its;
+  s.np.cg_abs_tol = params.cg_abs_tol;
+  s.np.cg_rel_tol = params.cg_rel_tol;
+  s.np.its_with_A0 = params.its_with_A0;
+  s.calc_ctan_lin_flag = params.calc_ctan_lin;
+  s.use_A0 = params.use_A0;
+  s.lin_stress = params.lin_stress;
+  s.write_log_flag = params.write_log;
+  try {
+    if (s.ngp < 0) throw std::invalid_argument("micropp_amd: ngp must be >= 0");
+    if (s.micro_type < 0 || s.micro_type > MIC3D_RANDOM_SPHERES)
+      throw std::invalid_argument("micropp_amd: invalid micro_type " + std::to_string(s.micro_type));
+    check(mppd_rve_create(&s.rve, params.size, s.micro_type, s.geo, s.mats), "RVE set-up");
+    check(mppd_solver_create(&s.sol, s.rve), "solver set-up");
+    double vf[2];
+    check(mppd_rve_volume_fractions(s.rve, vf), "volume fractions");
+    s.Vm = vf[0];
+    s.Vf = vf[1];
+    s.u = dzeros(s.nndim);
+    s.vars_aux = dzeros(s.nvars);
+    s.gp.resize(s.ngp);
+    int n_fe = 0;
+    for (int g = 0; g < s.ngp; ++g) {
+      gp_state &q = s.gp[g];
+      q.coupling = params.coupling ? params.coupling[g] : FE_ONE_WAY;
+      if (q.coupling == FE_LINEAR || q.coupling == FE_ONE_WAY || q.coupling == FE_FULL) n_fe++;
+      if (!params.coupling || q.coupling == FE_ONE_WAY || q.coupling == FE_FULL) {
+        q.u_n = dzeros(s.nndim);
+        q.u_k = dzeros(s.nndim);
+      }
+    }
+    if (s.use_A0) {
+      check(mppd_solver_create(&s.A0, s.rve), "A0 set-up");
+      check(mppd_assembly_mat(s.A0, s.u, nullptr), "A0 assembly");
+    }
+    if (s.calc_ctan_lin_flag && n_fe > 0) s.ctan_lin_fe_models();
+    for (auto &q : s.gp) {
+      if (q.coupling == MIX_RULE_CHAMIS)
+        s.chamis(q.ctan);
+      else
+        memcpy(q.ctan, s.ctan_lin_fe, sizeof(q.ctan));
+    }
+    if (s.write_log_flag) {
+      std::stringstream fn;
+      fn << "micropp-profiling-" << s.mpi_rank << ".log";
+      s.log.open(fn.str(), ios::out);
+      s.log << "#<gp_id>  <non-linear>  <cost>  <converged>" << endl;
+    }
+  } catch (...) {
+    delete p_;
+    p_ = nullptr;
+    throw;
+  }
+}
+
+template <int tdim>
+micropp<tdim>::~micropp() {
+  delete p_;
+}
+
+// set_strain / get_stress / get_ctan (src/homogenize.cpp:30-49)
+template <int tdim>
+void micropp<tdim>::set_strain(const int gp_id, const double *strain) {
+  assert(gp_id >= 0 && gp_id < p_->ngp);
+  memcpy(p_->gp[gp_id].strain, strain, 6 * sizeof(double));
+}
+
+template <int tdim>
+void micropp<tdim>::get_stress(const int gp_id, double *stress) const {
+  assert(gp_id >= 0 && gp_id < p_->ngp);
+  memcpy(stress, p_->gp[gp_id].stress, 6 * sizeof(double));
+}
+
+template <int tdim>
+void micropp<tdim>::get_ctan(const int gp_id, double *ctan) const {
+  assert(gp_id >= 0 && gp_id < p_->ngp);
+  memcpy(ctan, p_->gp[gp_id].ctan, 36 * sizeof(double));
+}
+
+// homogenize (src/homogenize.cpp:68-99).  The GPs are solved one after the
+// other, each solve filling the GPU.  A failed device call marks the GP as
+// not converged and is reported on stderr; the other GPs still run.
+template <int tdim>
+void micropp<tdim>::homogenize() {
+  impl &s = *p_;
+  for (int g = 0; g < s.ngp; ++g) {
+    gp_state &q = s.gp[g];
+    try {
+      if (q.coupling == FE_LINEAR || q.coupling == MIX_RULE_CHAMIS)
+        impl::linear(q);
+      else if (q.coupling == FE_ONE_WAY)
+        s.fe(q, false);
+      else if (q.coupling == FE_FULL)
+        s.fe(q, true);
+    } catch (const std::exception &ex) {
+      q.converged = false;
+      cerr << ex.what() << " (gp " << g << ")" << endl;
+    }
+  }
+  if (s.write_log_flag) {
+    // write_log (src/output.cpp:264-282)
+    s.log << "log_id : " << s.log_id << endl;
+    for (int g = 0; g < s.ngp; ++g)
+      s.log << "\t" << g << "\t" << s.gp[g].allocated << "\t" << s.gp[g].cost << "\t" << s.gp[g].converged << endl;
+    s.log_id++;
+  }
+}
+
+template <int tdim>
+void micropp<tdim>::homogenize_linear() {
+  for (auto &q : p_->gp) impl::linear(q);
+}
+
+template <int tdim>
+int micropp<tdim>::is_non_linear(const int gp_id) const {
+  assert(gp_id >= 0 && gp_id < p_->ngp);
+  return (int)p_->gp[gp_id].allocated;
+}
+
+template <int tdim>
+int micropp<tdim>::get_non_linear_gps(void) const {
+  int c = 0;
+  for (const auto &q : p_->gp) c += q.allocated ? 1 : 0;
+  return c;
+}
+
+template <int tdim>
+int micropp<tdim>::get_cost(int gp_id) const {
+  assert(gp_id >= 0 && gp_id < p_->ngp);
+  return (int)p_->gp[gp_id].cost;
+}
+
+template <int tdim>
+bool micropp<tdim>::has_converged(int gp_id) const {
+  assert(gp_id >= 0 && gp_id < p_->ngp);
+  return p_->gp[gp_id].converged;
+}
+
+template <int tdim>
+bool micropp<tdim>::has_subiterated(int gp_id) const {
+  assert(gp_id >= 0 && gp_id < p_->ngp);
+  return p_->gp[gp_id].subiterated;
+}
+
+// gp_t::update_vars (include/gp.hpp:95-105)
+template <int tdim>
+void micropp<tdim>::update_vars() {
+  for (auto &q : p_->gp) {
+    std::swap(q.vars_n, q.vars_k);
+    std::swap(q.u_n, q.u_k);
+    memcpy(q.strain_old, q.strain, sizeof(q.strain));
+  }
+}
+
+// write_restart / read_restart (src/output.cpp:216-262, include/gp.hpp:107-124):
+// per GP a bool, then vars_n and u_n if the GP has history.
+template <int tdim>
+void micropp<tdim>::write_restart(const int restart_id) const {
+  const impl &s = *p_;
+  std::stringstream fn;
+  fn << "micropp-restart-" << s.mpi_rank << "-" << restart_id << ".bin";
+  ofstream file(fn.str(), ios::out | ios::binary);
+  std::vector<double> buf;
+  for (const auto &q : s.gp) {
+    const bool a = q.allocated;
+    file.write((const char *)&a, sizeof(bool));
+    if (a) {
+      buf.resize(s.nvars);
+      check(mppd_memcpy_d2h(buf.data(), q.vars_n, sizeof(double) * s.nvars), "restart copy");
+      file.write((const char *)buf.data(), sizeof(double) * s.nvars);
+      buf.resize(s.nndim);
+      check(mppd_memcpy_d2h(buf.data(), q.u_n, sizeof(double) * s.nndim), "restart copy");
+      file.write((const char *)buf.data(), sizeof(double) * s.nndim);
+    }
+  }
+}
+
+template <int tdim>
+void micropp<tdim>::read_restart(const int restart_id) const {
+  impl &s = *p_;
+  std::stringstream fn;
+  fn << "micropp-restart-" << s.mpi_rank << "-" << restart_id << ".bin";
+  ifstream file(fn.str(), ios::in | ios::binary);
+  std::vector<double> buf;
+  for (auto &q : s.gp) {
+    bool a = false;
+    file.read((char *)&a, sizeof(bool));
+    q.allocated = a;
+    if (a) {
+      if (!q.vars_n) q.vars_n = dzeros(s.nvars);
+      if (!q.vars_k) q.vars_k = dzeros(s.nvars);
+      buf.resize(s.nvars);
+      file.read((char *)buf.data(), sizeof(double) * s.nvars);
+      check(mppd_memcpy_h2d(q.vars_n, buf.data(), sizeof(double) * s.nvars), "restart copy");
+      buf.resize(s.nndim);
+      file.read((char *)buf.data(), sizeof(double) * s.nndim);
+      if (q.u_n) check(mppd_memcpy_h2d(q.u_n, buf.data(), sizeof(double) * s.nndim), "restart copy");
+    }
+  }
+}
+
+template <int tdim>
+void micropp<tdim>::output(int gp_id, const char *filename) {
+  (void)gp_id;
+  cerr << "micropp_amd: VTU output is not available (" << filename << ")" << endl;
+}
+
+template <int tdim>
+void micropp<tdim>::output2(const int gp_id, const int elem_global, const int time_step) {
+  (void)gp_id;
+  cerr << "micropp_amd: VTU output is not available (micropp-" << elem_global << "-" << time_step << ")" << endl;
+}
+
+// print_info (src/micropp.cpp:546-597)
+template <int tdim>
+void micropp<tdim>::print_info() const {
+  const impl &s = *p_;
+  int cnt[4] = {0, 0, 0, 0};
+  for (const auto &q : s.gp)
+    if (q.coupling >= 0 && q.coupling < 4) cnt[q.coupling]++;
+  cout << "micropp3 (micropp_amd, gfx950)" << endl;
+  cout << "Micro-structure   : " << micro_names[s.micro_type] << endl;
+  cout << "MATRIX [%]        : " << s.Vm << endl;
+  cout << "FIBER  [%]        : " << s.Vf << endl;
+  cout << "FE_LINEAR         : " << cnt[FE_LINEAR] << " GPs" << endl;
+  cout << "FE_ONE_WAY        : " << cnt[FE_ONE_WAY] << " GPs" << endl;
+  cout << "FE_FULL           : " << cnt[FE_FULL] << " GPs" << endl;
+  cout << "MIX_RULE_CHAMIS   : " << cnt[MIX_RULE_CHAMIS] << " GPs" << endl;
+  cout << "USE A0            : " << s.use_A0 << endl;
+  cout << "NUM SUBITS        : " << s.nsubiterations << endl;
+  cout << "MPI RANK          : " << s.mpi_rank << endl;
+  cout << "ngp :" << s.ngp << " nx :" << s.nx << " ny :" << s.ny << " nz :" << s.nz << " nn :" << s.nn << endl;
+  cout << "geo_params:";
+  for (int i = 0; i < 4; ++i) cout << " " << s.geo[i];
+  cout << endl;
+  for (int i = 0; i < MAX_MATERIALS; ++i) material_print(&s.mats[i]);
+  cout << endl << "ctan_lin_fe = " << endl;
+  for (int i = 0; i < 6; ++i) {
+    for (int j = 0; j < 6; ++j) cout << s.ctan_lin_fe[i * 6 + j] << "\t";
+    cout << endl;
+  }
+  cout << endl;
+}
+
+template <int tdim>
+bool micropp<tdim>::get_state(int gp_id, int which, double *out) const {
+  const impl &s = *p_;
+  assert(gp_id >= 0 && gp_id < s.ngp);
+  const gp_state &q = s.gp[gp_id];
+  const double *src = which == 0 ? q.u_n : which == 1 ? q.u_k : which == 2 ? q.vars_n : q.vars_k;
+  if (!src) return false;
+  const size_t n = which < 2 ? (size_t)s.nndim : (size_t)s.nvars;
+  check(mppd_memcpy_d2h(out, src, sizeof(double) * n), "state copy");
+  return true;
+}
+
+template <int tdim>
+void micropp<tdim>::get_ctan_lin(double *ctan) const {
+  memcpy(ctan, p_->ctan_lin_fe, sizeof(p_->ctan_lin_fe));
+}
+
+template <int tdim>
+void micropp<tdim>::get_volume_fractions(double *vf) const {
+  vf[0] = p_->Vm;
+  vf[1] = p_->Vf;
+}
+
+template class micropp<3>;

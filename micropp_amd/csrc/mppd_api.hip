@@ -1,0 +1,534 @@
+// mppd_api.hip -- the thin C ABI (include/mppd.h) over the gfx950 kernels.
+//
+// Host code here only allocates, launches and steers: every floating-point
+// operation of the hot path runs in the kernels of geometry.hip,
+// assembly.hip and solver.hip.  The host reads back only the scalars that
+// steer control flow (residual norms, CG state, nonlinearity flags) and the
+// 6-component averages.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace mpp {
+
+static thread_local std::string g_err;
+
+size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+int hip_fail(hipError_t e, const char *what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? MPPD_ERR_NOMEM : MPPD_ERR_HIP;
+}
+
+template <typename T>
+static int dalloc(T **p, size_t n, bool zero = true) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  if (zero) {
+    e = hipMemset(*p, 0, n * sizeof(T));
+    // hipMemset runs on the null stream, which the solvers' non-blocking
+    // streams do not order against: finish it before the buffer is used
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "hipMemset");
+  }
+  return MPPD_OK;
+}
+
+static void dfree(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+// make_material (src/material.cpp:26-46) with the constructors of
+// include/material.hpp:66-135: derived constants rebuilt from E and nu, the
+// fields a law does not use set to -1.
+static int make_material(const material_base &in, material_base &m) {
+  if (in.type < MATERIAL_ELASTIC || in.type > MATERIAL_DAMAGE) return fail(MPPD_ERR_ARG, "invalid material type");
+  m = in;
+  m.k = in.E / (3. * (1. - 2. * in.nu));
+  m.mu = in.E / (2. * (1. + in.nu));
+  m.lambda = in.nu * in.E / ((1. + in.nu) * (1. - 2. * in.nu));
+  if (in.type != MATERIAL_PLASTIC) {
+    m.Ka = -1.0;
+    m.Sy = -1.0;
+  }
+  if (in.type != MATERIAL_DAMAGE) m.Xt = -1.0;
+  return MPPD_OK;
+}
+
+}  // namespace mpp
+
+using namespace mpp;
+
+struct mppd_rve {
+  Grid g;
+  int micro_type = 0;
+  double geo[4];
+  material_base mats[3];
+  double *bmat = nullptr;
+  int *et = nullptr;
+  double *spheres = nullptr;
+  int nsph = 0;
+  double vf[2] = {0, 0};
+  int all_elastic = 1;
+};
+
+struct mppd_solver {
+  mppd_rve *r = nullptr;
+  hipStream_t st = nullptr;
+  int *cols = nullptr;
+  double *vals = nullptr;
+  double *ke = nullptr;
+  int phase_mode = 0;
+  int ke_phase_ready = 0;
+  double *be = nullptr;
+  double *x = nullptr, *rr = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *kinv = nullptr;
+  double *b = nullptr, *du = nullptr;
+  double *partials = nullptr;
+  CgState *cg = nullptr, *cg_h = nullptr;
+  RedState *red = nullptr, *red_h = nullptr;
+};
+
+extern "C" {
+
+const char *mppd_last_error(void) { return g_err.c_str(); }
+
+int mppd_device_count(int *n) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    *n = 0;
+    return hip_fail(e, "hipGetDeviceCount");
+  }
+  *n = c;
+  return c > 0 ? MPPD_OK : fail(MPPD_ERR_NODEV, "no HIP device");
+}
+
+int mppd_malloc(void **ptr_d, size_t bytes) {
+  MPP_HIP(hipMalloc(ptr_d, bytes ? bytes : 1));
+  return MPPD_OK;
+}
+int mppd_free(void *ptr_d) {
+  if (ptr_d) MPP_HIP(hipFree(ptr_d));
+  return MPPD_OK;
+}
+// The plain-memory helpers complete before they return (the solvers' streams
+// are non-blocking, so null-stream work is not ordered against them).
+int mppd_memcpy_h2d(void *dst_d, const void *src_h, size_t bytes) {
+  MPP_HIP(hipMemcpy(dst_d, src_h, bytes, hipMemcpyHostToDevice));
+  MPP_HIP(hipDeviceSynchronize());
+  return MPPD_OK;
+}
+int mppd_memcpy_d2h(void *dst_h, const void *src_d, size_t bytes) {
+  MPP_HIP(hipMemcpy(dst_h, src_d, bytes, hipMemcpyDeviceToHost));
+  MPP_HIP(hipDeviceSynchronize());
+  return MPPD_OK;
+}
+int mppd_memcpy_d2d(void *dst_d, const void *src_d, size_t bytes) {
+  MPP_HIP(hipMemcpy(dst_d, src_d, bytes, hipMemcpyDeviceToDevice));
+  MPP_HIP(hipDeviceSynchronize());
+  return MPPD_OK;
+}
+int mppd_memset(void *ptr_d, int value, size_t bytes) {
+  MPP_HIP(hipMemset(ptr_d, value, bytes));
+  MPP_HIP(hipDeviceSynchronize());
+  return MPPD_OK;
+}
+int mppd_synchronize(void) {
+  MPP_HIP(hipDeviceSynchronize());
+  return MPPD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// RVE
+// ---------------------------------------------------------------------------
+int mppd_rve_create(mppd_rve **out, const int size[3], int micro_type, const double geo[4],
+                    const struct material_base mats[3]) {
+  *out = nullptr;
+  if (!size || !geo || !mats) return fail(MPPD_ERR_ARG, "null argument");
+  for (int d = 0; d < 3; ++d)
+    if (size[d] < 2) return fail(MPPD_ERR_ARG, "size must be >= 2 nodes per side");
+  if (micro_type < 0 || micro_type >= kNumMicroTypes) return fail(MPPD_ERR_ARG, "invalid micro_type");
+  const double nrow_d = 3.0 * size[0] * size[1] * size[2];
+  if (nrow_d * kSlots > 2147483647.0) return fail(MPPD_ERR_ARG, "RVE too large for 32-bit column indices");
+  mppd_rve *r = new mppd_rve();
+  for (int i = 0; i < 3; ++i) {
+    const int rc = make_material(mats[i], r->mats[i]);
+    if (rc) {
+      delete r;
+      return rc;
+    }
+    if (mats[i].type != MATERIAL_ELASTIC) r->all_elastic = 0;
+  }
+  Grid &g = r->g;
+  g.nx = size[0];
+  g.ny = size[1];
+  g.nz = size[2];
+  g.nex = g.nx - 1;
+  g.ney = g.ny - 1;
+  g.nez = g.nz - 1;
+  g.nn = g.nx * g.ny * g.nz;
+  g.nrow = 3 * g.nn;
+  g.nrow_pad = (int)round_up(g.nrow, kRowPad);
+  g.nelem = g.nex * g.ney * g.nez;
+  // micropp constructor (src/micropp.cpp:29-71): unit cube
+  const double lx = 1.0, ly = 1.0, lz = 1.0;
+  g.dx = lx / g.nex;
+  g.dy = ly / g.ney;
+  g.dz = lz / g.nez;
+  g.wg = g.dx * g.dy * g.dz / 8;
+  g.vol_tot = lx * ly * lz;
+  g.evol = g.dx * g.dy * g.dz;
+  r->micro_type = micro_type;
+  for (int i = 0; i < 4; ++i) r->geo[i] = geo[i];
+  int rc = MPPD_OK;
+  int *bad = nullptr;
+  double *vf_d = nullptr;
+  do {
+    if ((rc = dalloc(&r->bmat, 8 * 144))) break;
+    if ((rc = dalloc(&r->et, (size_t)g.nelem))) break;
+    if ((rc = dalloc(&r->spheres, (size_t)kMaxSpheres * 4))) break;
+    if ((rc = dalloc(&bad, 1))) break;
+    if ((rc = dalloc(&vf_d, 2))) break;
+    hipError_t e = launch_bmat(g, r->bmat, nullptr);
+    if (e == hipSuccess && micro_type == kMicRandomSpheres) {
+      int n = (int)geo[1];
+      r->nsph = n < 0 ? 0 : (n > kMaxSpheres ? kMaxSpheres : n);
+      e = launch_random_spheres(geo, r->spheres, nullptr);
+    }
+    if (e == hipSuccess) e = launch_phase_map(g, micro_type, geo, r->spheres, r->nsph, r->et, bad, nullptr);
+    if (e == hipSuccess) e = launch_volume_fractions(g, r->et, vf_d, nullptr);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "rve set-up kernels");
+      break;
+    }
+    int bad_h = 0;
+    e = hipMemcpy(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(r->vf, vf_d, 2 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "rve set-up readback");
+      break;
+    }
+    if (bad_h) rc = fail(MPPD_ERR_ARG, "invalid micro_type");
+  } while (0);
+  dfree(bad);
+  dfree(vf_d);
+  if (rc) {
+    mppd_rve_destroy(r);
+    return rc;
+  }
+  *out = r;
+  return MPPD_OK;
+}
+
+int mppd_rve_destroy(mppd_rve *r) {
+  if (!r) return MPPD_OK;
+  dfree(r->bmat);
+  dfree(r->et);
+  dfree(r->spheres);
+  delete r;
+  return MPPD_OK;
+}
+
+int mppd_rve_info(const mppd_rve *r, int *nrow, int *nelem, int *nvars) {
+  if (!r) return fail(MPPD_ERR_ARG, "null rve");
+  if (nrow) *nrow = r->g.nrow;
+  if (nelem) *nelem = r->g.nelem;
+  if (nvars) *nvars = r->g.nelem * 8 * 7;
+  return MPPD_OK;
+}
+
+int mppd_rve_bmat(const mppd_rve *r, double *bmat_h) {
+  MPP_HIP(hipMemcpy(bmat_h, r->bmat, 8 * 144 * sizeof(double), hipMemcpyDeviceToHost));
+  return MPPD_OK;
+}
+
+int mppd_rve_elem_types(const mppd_rve *r, int *et_h) {
+  MPP_HIP(hipMemcpy(et_h, r->et, (size_t)r->g.nelem * sizeof(int), hipMemcpyDeviceToHost));
+  return MPPD_OK;
+}
+
+int mppd_rve_volume_fractions(const mppd_rve *r, double vf_h[2]) {
+  vf_h[0] = r->vf[0];
+  vf_h[1] = r->vf[1];
+  return MPPD_OK;
+}
+
+int mppd_material_eval(const struct material_base *m, int n, const double *eps_d, const double *hist_d,
+                       double *sig_d, double *ctan_d, double *hist_new_d, int *nl_d) {
+  if (!m || n < 0 || (n > 0 && !eps_d)) return fail(MPPD_ERR_ARG, "bad material_eval arguments");
+  material_base mm;
+  int rc = make_material(*m, mm);
+  if (rc) return rc;
+  MPP_HIP(launch_material_eval(mm, n, eps_d, hist_d, sig_d, ctan_d, hist_new_d, nl_d, nullptr));
+  MPP_HIP(hipDeviceSynchronize());
+  return MPPD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// solver
+// ---------------------------------------------------------------------------
+int mppd_solver_create(mppd_solver **out, mppd_rve *r) {
+  *out = nullptr;
+  if (!r) return fail(MPPD_ERR_ARG, "null rve");
+  mppd_solver *s = new mppd_solver();
+  s->r = r;
+  const Grid &g = r->g;
+  const size_t np = (size_t)g.nrow_pad;
+  s->phase_mode = r->all_elastic;
+  int rc = MPPD_OK;
+  do {
+    hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "hipStreamCreate");
+      break;
+    }
+    if ((rc = dalloc(&s->cols, np * kSlots, false))) break;
+    if ((rc = dalloc(&s->vals, np * kSlots))) break;
+    if ((rc = dalloc(&s->ke, (s->phase_mode ? 3 : (size_t)g.nelem) * 576, false))) break;
+    if ((rc = dalloc(&s->be, (size_t)g.nelem * 24, false))) break;
+    double **vecs[] = {&s->x, &s->rr, &s->z, &s->p, &s->Ap, &s->kinv, &s->b, &s->du};
+    for (double **v : vecs)
+      if ((rc = dalloc(v, np))) break;
+    if (rc) break;
+    const size_t nb = std::max(round_up(np, 256) / 256, round_up((size_t)g.nelem, 256) / 256) + 1;
+    if ((rc = dalloc(&s->partials, nb * 8))) break;
+    if ((rc = dalloc(&s->cg, 1))) break;
+    if ((rc = dalloc(&s->red, 1))) break;
+    e = hipHostMalloc((void **)&s->cg_h, sizeof(CgState), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&s->red_h, sizeof(RedState), hipHostMallocDefault);
+    if (e == hipSuccess) e = launch_ell_cols(g, s->cols, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) rc = hip_fail(e, "solver set-up");
+  } while (0);
+  if (rc) {
+    mppd_solver_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return MPPD_OK;
+}
+
+int mppd_solver_destroy(mppd_solver *s) {
+  if (!s) return MPPD_OK;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  for (void *p : {(void *)s->cols, (void *)s->vals, (void *)s->ke, (void *)s->be, (void *)s->x, (void *)s->rr,
+                  (void *)s->z, (void *)s->p, (void *)s->Ap, (void *)s->kinv, (void *)s->b, (void *)s->du,
+                  (void *)s->partials, (void *)s->cg, (void *)s->red})
+    dfree(p);
+  if (s->cg_h) (void)hipHostFree(s->cg_h);
+  if (s->red_h) (void)hipHostFree(s->red_h);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+  return MPPD_OK;
+}
+
+int mppd_ell_download(mppd_solver *s, int *cols_h, double *vals_h) {
+  const Grid &g = s->r->g;
+  const size_t n = (size_t)g.nrow * kSlots;
+  int *c = nullptr;
+  double *v = nullptr;
+  int rc = MPPD_OK;
+  if (cols_h && (rc = dalloc(&c, n, false))) return rc;
+  if (vals_h && (rc = dalloc(&v, n, false))) {
+    dfree(c);
+    return rc;
+  }
+  hipError_t e = launch_sell_to_rows(g, s->cols, s->vals, c, v, s->st);
+  if (e == hipSuccess && c) e = hipMemcpyAsync(cols_h, c, n * sizeof(int), hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess && v) e = hipMemcpyAsync(vals_h, v, n * sizeof(double), hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  dfree(c);
+  dfree(v);
+  return e == hipSuccess ? MPPD_OK : hip_fail(e, "ell_download");
+}
+
+int mppd_ell_upload_vals(mppd_solver *s, const double *vals_h) {
+  const Grid &g = s->r->g;
+  const size_t n = (size_t)g.nrow * kSlots;
+  double *v = nullptr;
+  int rc = dalloc(&v, n, false);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(v, vals_h, n * sizeof(double), hipMemcpyHostToDevice, s->st);
+  if (e == hipSuccess) e = launch_rows_to_sell(g, v, s->vals, s->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  dfree(v);
+  return e == hipSuccess ? MPPD_OK : hip_fail(e, "ell_upload_vals");
+}
+
+int mppd_set_displ_bc(mppd_solver *s, const double eps[6], double *u_d) {
+  MPP_HIP(launch_set_displ_bc(s->r->g, eps, u_d, s->st));
+  MPP_HIP(hipStreamSynchronize(s->st));
+  return MPPD_OK;
+}
+
+static hipError_t rhs_async(mppd_solver *s, const double *u_d, const double *vars_d, double *b_d) {
+  const mppd_rve *r = s->r;
+  hipError_t e = launch_elem_be(r->g, r->bmat, r->mats, r->et, u_d, vars_d, s->be, s->st);
+  if (e == hipSuccess) e = launch_gather_rhs(r->g, s->be, b_d, s->partials, s->red, s->st);
+  return e;
+}
+
+static hipError_t read_red(mppd_solver *s) {
+  hipError_t e = hipMemcpyAsync(s->red_h, s->red, sizeof(RedState), hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  return e;
+}
+
+int mppd_assembly_rhs(mppd_solver *s, const double *u_d, const double *vars_d, double *b_d, double *norm) {
+  MPP_HIP(rhs_async(s, u_d, vars_d, b_d));
+  MPP_HIP(read_red(s));
+  if (norm) *norm = s->red_h->v[0];
+  return MPPD_OK;
+}
+
+static hipError_t mat_async(mppd_solver *s, const double *u_d, const double *vars_d) {
+  const mppd_rve *r = s->r;
+  hipError_t e = hipSuccess;
+  if (s->phase_mode) {
+    if (!s->ke_phase_ready) {
+      e = launch_elem_ke(r->g, r->bmat, r->mats, r->et, nullptr, nullptr, 1, s->ke, s->st);
+      if (e == hipSuccess) s->ke_phase_ready = 1;
+    }
+  } else {
+    e = launch_elem_ke(r->g, r->bmat, r->mats, r->et, u_d, vars_d, 0, s->ke, s->st);
+  }
+  if (e == hipSuccess) e = launch_gather_vals(r->g, r->et, s->ke, s->phase_mode, s->vals, s->st);
+  return e;
+}
+
+int mppd_assembly_mat(mppd_solver *s, const double *u_d, const double *vars_d) {
+  MPP_HIP(mat_async(s, u_d, vars_d));
+  MPP_HIP(hipStreamSynchronize(s->st));
+  return MPPD_OK;
+}
+
+int mppd_ell_mvp(mppd_solver *s, const double *x_d, double *y_d) {
+  MPP_HIP(launch_spmv(s->r->g, s->cols, s->vals, x_d, y_d, s->st));
+  MPP_HIP(hipStreamSynchronize(s->st));
+  return MPPD_OK;
+}
+
+// CG on the device; the host queues iterations in chunks sized from the
+// observed convergence rate and reads the 64-byte state between chunks.
+static int cg_run(mppd_solver *s, const mppd_solver *A, const double *b_d, double min_err, double rel_err,
+                  int max_its, int *its_out, double *err_out) {
+  const Grid &g = s->r->g;
+  // the pinned mirror is idle here: every earlier use ended with a stream sync
+  *s->cg_h = CgState{};
+  s->cg_h->min_err = min_err;
+  s->cg_h->rel_err = rel_err;
+  s->cg_h->max_its = max_its;
+  MPP_HIP(hipMemcpyAsync(s->cg, s->cg_h, sizeof(CgState), hipMemcpyHostToDevice, s->st));
+  MPP_HIP(launch_cg_init(g, A->cols, A->vals, b_d, s->x, s->rr, s->z, s->p, s->kinv, s->partials, s->cg, s->st));
+  MPP_HIP(hipMemcpyAsync(s->cg_h, s->cg, sizeof(CgState), hipMemcpyDeviceToHost, s->st));
+  MPP_HIP(hipStreamSynchronize(s->st));
+  int queued = 0;
+  double pn_prev = s->cg_h->pn;
+  int its_prev = 0;
+  int chunk = 8;
+  while (!s->cg_h->done) {
+    const int n = std::min(chunk, max_its - queued);
+    if (n <= 0) break;
+    for (int i = 0; i < n; ++i)
+      MPP_HIP(launch_cg_iter(g, A->cols, A->vals, s->x, s->rr, s->z, s->p, s->Ap, s->kinv, s->partials, s->cg,
+                             s->st));
+    queued += n;
+    MPP_HIP(hipMemcpyAsync(s->cg_h, s->cg, sizeof(CgState), hipMemcpyDeviceToHost, s->st));
+    MPP_HIP(hipStreamSynchronize(s->st));
+    const CgState &c = *s->cg_h;
+    if (c.done) break;
+    // next chunk: iterations still needed at the observed rate, within [4, 256]
+    const double target = std::max(c.min_err, c.pn0 * c.rel_err);
+    const int dit = c.its - its_prev;
+    int next = chunk * 2;
+    if (dit > 0 && c.pn > 0 && pn_prev > 0 && c.pn < pn_prev) {
+      const double rate = std::log(c.pn / pn_prev) / dit;
+      const double need = std::log(target / c.pn) / rate;
+      if (std::isfinite(need)) next = (int)std::ceil(need) + 2;
+    }
+    chunk = std::max(4, std::min(256, next));
+    pn_prev = c.pn;
+    its_prev = c.its;
+  }
+  if (its_out) *its_out = s->cg_h->its;
+  if (err_out) *err_out = s->cg_h->rz;
+  return MPPD_OK;
+}
+
+int mppd_cg(mppd_solver *s, const mppd_solver *A0, const double *b_d, double *x_d, double min_err, double rel_err,
+            int max_its, int *its, double *err) {
+  if (!s || !b_d || !x_d) return fail(MPPD_ERR_ARG, "null argument");
+  const mppd_solver *A = A0 ? A0 : s;
+  int rc = cg_run(s, A, b_d, min_err, rel_err, max_its, its, err);
+  if (rc) return rc;
+  MPP_HIP(launch_copy(s->r->g.nrow, s->x, x_d, s->st));
+  MPP_HIP(hipStreamSynchronize(s->st));
+  return MPPD_OK;
+}
+
+int mppd_ave_stress(mppd_solver *s, const double *u_d, const double *vars_d, double sig[6]) {
+  const mppd_rve *r = s->r;
+  MPP_HIP(launch_ave_stress(r->g, r->bmat, r->mats, r->et, u_d, vars_d, s->partials, s->red, s->st));
+  MPP_HIP(read_red(s));
+  for (int v = 0; v < 6; ++v) sig[v] = s->red_h->v[v];
+  return MPPD_OK;
+}
+
+int mppd_vars_new(mppd_solver *s, const double *u_d, const double *vars_old_d, double *vars_new_d, int *nonlinear) {
+  const mppd_rve *r = s->r;
+  MPP_HIP(hipMemsetAsync(&s->red->flag, 0, sizeof(int), s->st));
+  MPP_HIP(launch_vars_new(r->g, r->bmat, r->mats, r->et, u_d, vars_old_d, vars_new_d, s->red, s->st));
+  MPP_HIP(read_red(s));
+  if (nonlinear) *nonlinear = s->red_h->flag ? 1 : 0;
+  return MPPD_OK;
+}
+
+int mppd_newton(mppd_solver *s, const mppd_solver *A0, const double eps[6], const double *vars_old_d, double *u_d,
+                const mppd_newton_params *p, int out[3]) {
+  if (!s || !eps || !u_d || !p || !out) return fail(MPPD_ERR_ARG, "null argument");
+  const Grid &g = s->r->g;
+  int its = 0, solver_its = 0, conv = 0;
+  MPP_HIP(launch_set_displ_bc(g, eps, u_d, s->st));
+  MPP_HIP(rhs_async(s, u_d, vars_old_d, s->b));
+  MPP_HIP(read_red(s));
+  double norm = s->red_h->v[0];
+  const double norm0 = norm;
+  while (its < p->nr_max_its) {
+    if (norm < p->nr_max_tol || norm < norm0 * p->nr_rel_tol) {
+      conv = 1;
+      break;
+    }
+    const mppd_solver *A = s;
+    if (!A0 || its > p->its_with_A0 - 1) {
+      MPP_HIP(mat_async(s, u_d, vars_old_d));
+    } else {
+      A = A0;
+    }
+    int cg_its = 0;
+    double err = 0;
+    int rc = cg_run(s, A, s->b, p->cg_abs_tol, p->cg_rel_tol, p->cg_max_its, &cg_its, &err);
+    if (rc) return rc;
+    solver_its += cg_its;
+    MPP_HIP(launch_axpy(g.nrow, 1.0, s->x, u_d, s->st));
+    MPP_HIP(rhs_async(s, u_d, vars_old_d, s->b));
+    MPP_HIP(read_red(s));
+    norm = s->red_h->v[0];
+    its++;
+  }
+  out[0] = its;
+  out[1] = solver_its;
+  out[2] = conv;
+  return MPPD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,363 @@
+"""GPU parity: the HIP path against the oracle, through the C ABI.
+
+Bit-exact: phase maps, ELL columns, B matrices, volume fractions, material
+laws, assembled matrix values, residuals, boundary values, SpMV and the
+history update (same arithmetic, no contraction, same summation order).
+Within tolerance: everything that passes through a parallel reduction (CG
+iterates, norms, averaged stress, ctan): relative error <= 1e-6 at CG
+tolerances tight enough that the two solutions agree to that level.
+"""
+import numpy as np
+import pytest
+
+from oracle import bindings as ob
+
+pytestmark = pytest.mark.gpu
+
+GEO = {0: 0.1, 1: 0.2, 2: 0.3, 3: 0.2, 4: 0.2, 5: 0.1, 6: 0.1, 7: 0.1, 8: 0.1, 9: 0.1, 10: 0.1, 11: 0.1, 12: 0.1,
+       13: None}
+SPHERES_GEO = (12345.0, 8.0, 0.05, 0.15)
+
+
+@pytest.fixture(scope="module")
+def mp():
+    import micropp_amd
+
+    micropp_amd.load()
+    if not micropp_amd.gpu_available():
+        pytest.skip("no HIP device")
+    return micropp_amd
+
+
+def mats_pair(mp, kind):
+    if kind == "elastic":
+        spec = [(0, 1e6, 0.3), (0, 1e7, 0.25), (0, 1e7, 0.25)]
+    elif kind == "plastic":
+        spec = [(1, 1e6, 0.3, 1e5, 1e3), (0, 1e7, 0.25), (1, 2e6, 0.3, 1e4, 5e2)]
+    else:
+        spec = [(2, 1e6, 0.3, 0, 0, 1e3), (0, 1e7, 0.25), (2, 1e6, 0.3, 0, 0, 1e3)]
+    return [mp.material(*s) for s in spec], [ob.material(*s) for s in spec]
+
+
+def rand_state(rng, nvars, kind):
+    v = np.zeros(nvars)
+    h = v.reshape(-1, 7)
+    if kind == "plastic":
+        h[:, :6] = rng.normal(0, 2e-3, (h.shape[0], 6))
+        h[:, 6] = rng.uniform(0, 1e-2, h.shape[0])
+    elif kind == "damage":
+        h[:, 0] = rng.uniform(0, 3e2, h.shape[0])
+        h[:, 1] = rng.uniform(-0.5, 0.5, h.shape[0])
+    return v
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# ---------------------------------------------------------------------------
+# bit-exact contracts
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("mtype", list(range(14)))
+def test_phase_map_bmat_vf_bitwise(mp, oracle, mtype):
+    gm, om = mats_pair(mp, "elastic")
+    for size in [(4, 4, 4), (11, 11, 11), (12, 12, 12), (5, 8, 11), (51, 51, 51)]:
+        geo = SPHERES_GEO if mtype == 13 else (GEO[mtype],) * 4
+        print(f"phase map type {mtype} size {size}", flush=True)
+        r = mp.Rve(size, mtype, geo, gm)
+        et, rc = oracle.elem_types(mtype, geo, *size)
+        assert rc == 0
+        assert np.array_equal(r.elem_types(), et), (mtype, size)
+        assert np.array_equal(r.bmat(), oracle.bmat(*size))
+        p, k = ob.make_params(size, mtype, geo, om, calc_ctan_lin=0)
+        assert np.array_equal(r.volume_fractions(), oracle.micro(p, k).volume_fractions())
+
+
+def test_invalid_micro_type_rejected(mp):
+    gm, _ = mats_pair(mp, "elastic")
+    with pytest.raises(mp.MppdError):
+        mp.Rve(5, 99, (0.1,) * 4, gm)
+
+
+@pytest.mark.parametrize("size", [(2, 2, 2), (3, 4, 5), (11, 11, 11), (51, 51, 51)])
+def test_ell_cols_bitwise(mp, oracle, size):
+    gm, _ = mats_pair(mp, "elastic")
+    print(f"cols {size}", flush=True)
+    s = mp.Solver(mp.Rve(size, 1, (0.2,) * 4, gm))
+    cols, _ = s.ell()
+    assert np.array_equal(cols, oracle.ell_cols3d(*size))
+
+
+@pytest.mark.parametrize("kind", ["elastic", "plastic", "damage"])
+def test_material_laws_bitwise(mp, oracle, kind):
+    gm, om = mats_pair(mp, kind)
+    rng = np.random.default_rng(11)
+    n = 400
+    eps = rng.normal(0, 1, (n, 6)) * 10 ** rng.uniform(-5, -1, (n, 1))
+    hist = rand_state(rng, n * 7, kind).reshape(n, 7)
+    for h in (None, hist):
+        sig, ctan, hn, nl = mp.material_eval(gm[0], eps, h)
+        for p in range(n):
+            hp = None if h is None else np.ascontiguousarray(h[p])
+            assert np.array_equal(sig[p], oracle.mat_stress(om[0], eps[p], hp))
+            assert np.array_equal(ctan[p], oracle.mat_ctan(om[0], eps[p], hp))
+            ho = np.zeros(7)
+            assert nl[p] == oracle.mat_evolute(om[0], eps[p], hp, ho)
+            assert np.array_equal(hn[p], ho)
+
+
+@pytest.mark.parametrize("kind", ["elastic", "plastic", "damage"])
+@pytest.mark.parametrize("size", [(5, 4, 6), (9, 9, 9)])
+def test_assembly_bitwise(mp, oracle, kind, size):
+    gm, om = mats_pair(mp, kind)
+    print(f"assembly {kind} {size}", flush=True)
+    r = mp.Rve(size, 1, (0.3,) * 4, gm)
+    s = mp.Solver(r)
+    orve = oracle.rve(size, 1, (0.3,) * 4, om)
+    rng = np.random.default_rng(12)
+    u = rng.normal(0, 1e-3, r.nrow)
+    du = mp.DeviceArray.from_numpy(u)
+    vh = rand_state(rng, r.nvars, kind) if kind != "elastic" else None
+    dv = mp.DeviceArray.from_numpy(vh) if vh is not None else None
+    for v_h, v_d in ((None, None), (vh, dv)):
+        s.assembly_mat(du, v_d)
+        _, vals = s.ell()
+        assert np.array_equal(vals, orve.assembly_mat(u, v_h))
+        db = mp.DeviceArray(r.nrow)
+        norm = s.assembly_rhs(du, v_d, db)
+        bo, no = orve.assembly_rhs(u, v_h)
+        assert np.array_equal(db.numpy(), bo)
+        assert abs(norm - no) <= 1e-13 * no
+        sig = s.ave_stress(du, v_d)
+        assert rel(sig, orve.ave_stress(u, v_h)) < 1e-13
+        dvn = mp.DeviceArray(r.nvars)
+        nl = s.vars_new(du, v_d, dvn)
+        vo, nlo = orve.vars_new(u, v_h)
+        assert nl == nlo and np.array_equal(dvn.numpy(), vo)
+    eps = rng.normal(0, 1e-3, 6)
+    s.set_displ_bc(eps, du)
+    u2 = u.copy()
+    orve.set_displ_bc(eps, u2)
+    assert np.array_equal(du.numpy(), u2)
+
+
+@pytest.mark.parametrize("size", [(6, 6, 6), (21, 21, 21)])
+def test_spmv_bitwise(mp, oracle, size):
+    gm, om = mats_pair(mp, "elastic")
+    r = mp.Rve(size, 1, (0.3,) * 4, gm)
+    s = mp.Solver(r)
+    s.assembly_mat(None)
+    cols, vals = s.ell()
+    x = np.random.default_rng(13).normal(size=r.nrow)
+    dx, dy = mp.DeviceArray.from_numpy(x), mp.DeviceArray(r.nrow)
+    s.mvp(dx, dy)
+    assert np.array_equal(dy.numpy(), oracle.ell_mvp(cols, vals, x))
+    # an arbitrary uploaded matrix in the reference layout
+    v2 = np.random.default_rng(14).normal(size=vals.shape)
+    s.upload_vals(v2)
+    s.mvp(dx, dy)
+    assert np.array_equal(dy.numpy(), oracle.ell_mvp(cols, v2, x))
+
+
+# ---------------------------------------------------------------------------
+# reductions: tolerance
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("tol", [1e-5, 1e-8, 1e-12])
+def test_cg_parity(mp, oracle, tol):
+    gm, om = mats_pair(mp, "elastic")
+    size = (11, 11, 11)
+    r = mp.Rve(size, 1, (0.2,) * 4, gm)
+    s = mp.Solver(r)
+    u = mp.DeviceArray(r.nrow)
+    s.set_displ_bc([5e-3, 0, 0, 0, 0, 0], u)
+    s.assembly_mat(u)
+    b = mp.DeviceArray(r.nrow)
+    s.assembly_rhs(u, None, b)
+    x = mp.DeviceArray(r.nrow)
+    its, err = s.cg(b, x, 1e-50, tol, 2000)
+    cols, vals = s.ell()
+    xo, io, eo = oracle.ell_cg(r.nrow // 3, 3, 81, 13, cols, vals, b.numpy(), 1e-50, tol, 2000)
+    print(f"cg tol {tol}: its gpu {its} oracle {io}", flush=True)
+    assert abs(its - io) <= 1
+    assert rel(x.numpy(), xo) < 10 * tol
+    assert err > 0 and abs(np.log10(err / eo)) < 1
+
+
+def test_cg_max_its_and_zero_rhs(mp):
+    gm, _ = mats_pair(mp, "elastic")
+    r = mp.Rve((6, 6, 6), 1, (0.2,) * 4, gm)
+    s = mp.Solver(r)
+    u = mp.DeviceArray(r.nrow)
+    s.set_displ_bc([1e-3, 0, 0, 0, 0, 0], u)
+    s.assembly_mat(u)
+    b = mp.DeviceArray(r.nrow)
+    s.assembly_rhs(u, None, b)
+    x = mp.DeviceArray(r.nrow)
+    its, _ = s.cg(b, x, 1e-50, 1e-30, 3)
+    assert its == 3
+    its, _ = s.cg(mp.DeviceArray(r.nrow), x, 1e-50, 1e-8, 100)
+    assert its == 0 and not np.any(x.numpy())
+
+
+@pytest.mark.parametrize("kind", ["elastic", "plastic", "damage"])
+def test_newton_parity(mp, oracle, kind):
+    gm, om = mats_pair(mp, kind)
+    size = (8, 8, 8)
+    r = mp.Rve(size, 1, (0.3,) * 4, gm)
+    s = mp.Solver(r)
+    orve = oracle.rve(size, 1, (0.3,) * 4, om)
+    rng = np.random.default_rng(15)
+    vh = rand_state(rng, r.nvars, kind) if kind != "elastic" else None
+    dv = mp.DeviceArray.from_numpy(vh) if vh is not None else None
+    eps = np.array([4e-3, -1e-3, 1e-3, 2e-3, -3e-3, 1e-3])
+    u = mp.DeviceArray(r.nrow)
+    out = s.newton(eps, u, dv, cg_rel_tol=1e-12, cg_max_its=3000)
+    uo = np.zeros(r.nrow)
+    outo = orve.newton(eps, uo, vh, cg_rel_tol=1e-12, cg_max_its=3000)
+    print(f"newton {kind}: gpu {out} oracle {outo}", flush=True)
+    assert out[0] == outo[0] and out[2] == outo[2]
+    assert abs(out[1] - outo[1]) <= 2 * out[0]
+    assert rel(u.numpy(), uo) < 1e-9
+    assert rel(s.ave_stress(u, dv), orve.ave_stress(uo, vh)) < 1e-9
+
+
+def _drive(m, strains, steps):
+    rec = []
+    for t in range(steps):
+        for g in range(m.ngp):
+            m.set_strain(g, strains[g] * (t + 1))
+        m.homogenize()
+        rec.append([(m.get_stress(g), m.get_ctan(g), m.get_cost(g), m.has_converged(g), m.has_subiterated(g),
+                     m.is_non_linear(g)) for g in range(m.ngp)])
+        m.update_vars()
+    return rec
+
+
+@pytest.mark.parametrize("kind,lin_stress,subits", [("elastic", 0, 0), ("plastic", 0, 0), ("damage", 0, 1),
+                                                    ("plastic", 1, 1)])
+def test_homogenize_parity(mp, oracle, kind, lin_stress, subits):
+    """micropp<3> through the C interface vs the oracle instance, several steps."""
+    gm, om = mats_pair(mp, kind)
+    coupling = [ob.FE_FULL, ob.FE_ONE_WAY, ob.FE_LINEAR, ob.MIX_RULE_CHAMIS]
+    kw = dict(lin_stress=lin_stress, subiterations=subits, nsubiterations=3, nr_max_its=4, cg_rel_tol=1e-12,
+              cg_max_its=3000)
+    m = mp.Micropp3(6, 1, (0.3, 0, 0, 0), gm, ngp=4, coupling=coupling, **kw)
+    p, k = ob.make_params(6, 1, (0.3, 0, 0, 0), om, ngp=4, coupling=coupling, **kw)
+    o = oracle.micro(p, k)
+    assert rel(m.ctan_lin(), o.ctan_lin()) < 1e-9
+    assert np.array_equal(m.volume_fractions(), o.volume_fractions())
+    rng = np.random.default_rng(16)
+    strains = [rng.uniform(-2e-3, 2e-3, 6) for _ in range(4)]
+    a, b = _drive(m, strains, 4), _drive(o, strains, 4)
+    for t, (sa, sb) in enumerate(zip(a, b)):
+        for g, (ga, gb) in enumerate(zip(sa, sb)):
+            print(f"step {t} gp {g}: sigma rel {rel(ga[0], gb[0]):.2e} ctan rel {rel(ga[1], gb[1]):.2e}", flush=True)
+            assert rel(ga[0], gb[0]) < 1e-6
+            assert rel(ga[1], gb[1]) < 1e-6
+            assert ga[3:] == gb[3:]
+
+
+@pytest.mark.parametrize("name", ["benchmark_elastic", "benchmark_plastic", "benchmark_damage"])
+def test_reference_sigma_tables_gpu(mp, tables, name):
+    tab = tables[name]
+    mats = [mp.material(*tab["material"]), mp.material(0, 3e7, 0.25), mp.material(0, 3e7, 0.25)]
+    m = mp.Micropp3(tab["n"], mp.MIC_HOMOGENEOUS, materials=mats, lin_stress=0)
+    t, dt = 0.0, tab["t_final"] / tab["steps"]
+    eps = np.zeros(6)
+    for step in range(tab["steps"]):
+        eps[tab["dir"]] = tab["eps_max"] * t
+        m.set_strain(0, eps)
+        m.homogenize()
+        assert np.all(np.abs(m.get_stress(0) - np.array(tab["sig"][step])) < tab["tol"]), (name, step)
+        m.update_vars()
+        t += dt
+
+
+def test_reference_batch_invariant_gpu(mp, tables):
+    t = tables["test3d_5"]
+    mats = [mp.material(*row) for row in t["materials"]]
+    m = mp.Micropp3(t["n"], mp.MIC_SPHERE, (0.1,) * 4, mats, ngp=t["ngp"])
+    e1, e2 = np.zeros(6), np.zeros(6)
+    for _ in range(t["steps"]):
+        e1[t["dir"]] += t["d_eps_1"]
+        e2[t["dir"]] += t["d_eps_2"]
+        for g in range(t["ngp"]):
+            m.set_strain(g, e1 if g % 2 else e2)
+        m.homogenize()
+        s = [m.get_stress(g) for g in range(t["ngp"])]
+        for g in range(2, t["ngp"]):
+            assert np.all(np.abs(s[g][:3] - s[g % 2][:3]) < t["tol"])
+        m.update_vars()
+
+
+def test_config0_cpu_case(mp, oracle):
+    """10^3 elements, centred sphere r=0.2, CG rel 1e-8 / max 2000, sigma by
+    volume average, ctan from the six 1e-8 solves."""
+    gm, om = mats_pair(mp, "elastic")
+    kw = dict(lin_stress=0, cg_rel_tol=1e-8, cg_max_its=2000)
+    m = mp.Micropp3(11, mp.MIC_SPHERE, (0.2, 0, 0, 0), gm, coupling=[mp.FE_FULL], **kw)
+    p, k = ob.make_params(11, ob.MIC_SPHERE, (0.2, 0, 0, 0), om, coupling=[ob.FE_FULL], **kw)
+    o = oracle.micro(p, k)
+    eps = [5e-3, 0, 0, 0, 0, 0]
+    m.set_strain(0, eps)
+    o.set_strain(0, eps)
+    m.homogenize()
+    o.homogenize()
+    print("config0 sigma gpu", m.get_stress(0), "oracle", o.get_stress(0), flush=True)
+    assert rel(m.get_stress(0), o.get_stress(0)) < 1e-6
+    assert rel(m.get_ctan(0), o.get_ctan(0)) < 1e-6
+    assert abs(m.get_cost(0) - o.get_cost(0)) <= 2
+
+
+# ---------------------------------------------------------------------------
+# full size (50^3 elements): size-independent properties
+# ---------------------------------------------------------------------------
+def test_full_size_spmv_properties(mp):
+    gm, _ = mats_pair(mp, "elastic")
+    r = mp.Rve(51, mp.MIC3D_RANDOM_SPHERES, SPHERES_GEO, gm)
+    s = mp.Solver(r)
+    s.assembly_mat(None)
+    n = r.nrow
+    rng = np.random.default_rng(17)
+    bc = np.zeros(n, bool)
+    idx = np.arange(n // 3)
+    i, j, k = idx % 51, (idx // 51) % 51, idx // 2601
+    nb = (i == 0) | (j == 0) | (k == 0) | (i == 50) | (j == 50) | (k == 50)
+    bc[np.repeat(nb, 3)] = True
+    x, y = rng.normal(size=n), rng.normal(size=n)
+    x[bc] = 0
+    y[bc] = 0
+    dx, dy, ax, ay = mp.DeviceArray.from_numpy(x), mp.DeviceArray.from_numpy(y), mp.DeviceArray(n), mp.DeviceArray(n)
+    s.mvp(dx, ax)
+    s.mvp(dy, ay)
+    Ax, Ay = ax.numpy(), ay.numpy()
+    # symmetric interior block
+    assert abs(x @ Ay - y @ Ax) < 1e-10 * abs(x @ Ax)
+    # linearity
+    dz, az = mp.DeviceArray.from_numpy(x + 2 * y), mp.DeviceArray(n)
+    s.mvp(dz, az)
+    assert rel(az.numpy(), Ax + 2 * Ay) < 1e-12
+    # positive definite on the interior; identity on Dirichlet rows
+    assert x @ Ax > 0
+    e = np.zeros(n)
+    e[bc] = rng.normal(size=bc.sum())
+    de, ae = mp.DeviceArray.from_numpy(e), mp.DeviceArray(n)
+    s.mvp(de, ae)
+    assert np.array_equal(ae.numpy()[bc], e[bc])
+
+
+def test_full_size_patch_test(mp):
+    """Homogeneous 50^3 RVE: affine u is exact, so sigma = C eps and CG sees a
+    zero interior residual."""
+    gm = [mp.material(0, 1e6, 0.3)] * 3
+    m = mp.Micropp3(51, mp.MIC_HOMOGENEOUS, materials=gm, lin_stress=0, calc_ctan_lin=0, cg_rel_tol=1e-8)
+    eps = np.array([1e-3, -2e-3, 5e-4, 1e-3, -1e-3, 2e-3])
+    m.set_strain(0, eps)
+    m.homogenize()
+    lam, mu = 0.3 * 1e6 / (1.3 * 0.4), 1e6 / 2.6
+    C = np.zeros((6, 6))
+    C[:3, :3] = lam
+    C[range(3), range(3)] += 2 * mu
+    C[range(3, 6), range(3, 6)] = mu
+    assert rel(m.get_stress(0), C @ eps) < 1e-7
