@@ -119,6 +119,47 @@ typedef struct {
 int mppd_newton(mppd_solver *s, const mppd_solver *A0, const double eps[6], const double *vars_old_d, double *u_d,
                 const mppd_newton_params *p, int out[3]);
 
+/* ---- mixed-precision multi-right-hand-side solves (no reference
+ * counterpart; the reference runs these solves one by one: the six of
+ * calc_ctan_lin_fe_models, src/micropp.cpp:255-284, and one newton_raphson
+ * per Gauss point, src/homogenize.cpp:111-186) ----
+ * One msolver holds an FP32 copy of the stiffness of an RVE whose phases are
+ * all linear elastic, and K = 4, 8, 12 or 16 displacement fields.  A solve
+ * takes n <= K macro strains, sets u = eps x on the boundary and 0 inside,
+ * and iterates FP64 defect correction with FP32 Jacobi-PCG inner solves of
+ * all fields at once until, for every field, ||D^-1 (b - A u)|| (FP64)
+ * falls below max(abs_tol, rel_tol * its initial value) -- the reference
+ * CG's stopping metric, evaluated on the true FP64 residual.  Outputs the
+ * FP64 volume-averaged stress of each field (calc_ave_stress,
+ * src/average.cpp:57-82). */
+typedef struct mppd_msolver mppd_msolver;
+typedef struct {
+  double rel_tol;    /* outer relative tolerance (e.g. 1e-8) */
+  double abs_tol;    /* outer absolute tolerance (e.g. 1e-50) */
+  double inner_tol;  /* smallest inner FP32 reduction asked for (e.g. 1e-5) */
+  int max_inner_its; /* per inner solve */
+  int max_outer;     /* defect-correction steps */
+} mppd_mixed_params;
+int mppd_msolver_create(mppd_msolver **out, mppd_rve *r, int nrhs);
+int mppd_msolver_destroy(mppd_msolver *m);
+/* eps_h: n x 6; sig_h: n x 6; stats (may be NULL): {outer steps, inner
+ * iterations}; rel_h (may be NULL): n final residual ratios.  Returns
+ * MPPD_ERR_NOTCONV when a field did not reach the tolerance. */
+int mppd_msolver_solve(mppd_msolver *m, int n, const double *eps_h, const mppd_mixed_params *p, double *sig_h,
+                       int stats[2], double *rel_h);
+/* displacement field k (nrow doubles, reference DOF numbering) of the last solve */
+int mppd_msolver_get_u(mppd_msolver *m, int k, double *u_h);
+/* timing aid: average ms of the FP32 multi-RHS SpMV kernel and of a whole
+ * inner iteration, over iters launches, events on the solver's stream */
+int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *iter_ms);
+
+/* Timing aid (no reference counterpart; for the benchmark and rocprof
+ * cross-checks): runs `iters` CG iterations on the solver's matrix with b_d
+ * as right-hand side and the stopping test disabled, timing every SpMV kernel
+ * of the iteration (SpMV fused with p.Ap) and every whole iteration with HIP
+ * events on the solver's stream.  Averages in milliseconds. */
+int mppd_cg_profile(mppd_solver *s, const double *b_d, int iters, double *spmv_ms, double *iter_ms);
+
 #ifdef __cplusplus
 }
 #endif

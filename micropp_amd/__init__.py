@@ -134,7 +134,20 @@ SIGNATURES = {
     "mppd_ave_stress": (_I, [_VP, _VP, _VP, _PD]),
     "mppd_vars_new": (_I, [_VP, _VP, _VP, _VP, _PI]),
     "mppd_newton": (_I, [_VP, _VP, _PD, _VP, _VP, C.POINTER(NewtonParams), _PI]),
+    "mppd_cg_profile": (_I, [_VP, _VP, _I, _PD, _PD]),
+    "mppd_msolver_create": (_I, [C.POINTER(C.c_void_p), _VP, _I]),
+    "mppd_msolver_destroy": (_I, [_VP]),
+    "mppd_msolver_solve": (_I, [_VP, _I, _PD, C.c_void_p, _PD, _PI, _PD]),
+    "mppd_msolver_get_u": (_I, [_VP, _I, _PD]),
+    "mppd_msolver_profile": (_I, [_VP, _I, _PD, _PD]),
 }
+
+
+class MixedParams(C.Structure):
+    """``mppd_mixed_params`` (include/mppd.h)."""
+
+    _fields_ = [("rel_tol", C.c_double), ("abs_tol", C.c_double), ("inner_tol", C.c_double),
+                ("max_inner_its", C.c_int), ("max_outer", C.c_int)]
 
 _lib = None
 
@@ -457,6 +470,72 @@ class Solver:
         _chk(load().mppd_newton(self.h, None if A0 is None else A0.h, _dp(e), _ptr(vars_old), u.ptr, C.byref(p),
                                 _ip(out)), "mppd_newton")
         return tuple(int(v) for v in out)
+
+
+class MSolver:
+    """mppd_msolver: FP32-inner / FP64-refined solves of up to K strains on one elastic RVE."""
+
+    NOT_CONVERGED = -5
+
+    def __init__(self, rve: Rve, nrhs: int = 12):
+        h = C.c_void_p()
+        _chk(load().mppd_msolver_create(C.byref(h), rve.h, nrhs), "mppd_msolver_create")
+        self.h = h
+        self.rve = rve
+        self.K = nrhs
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().mppd_msolver_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def solve(self, strains, rel_tol=1e-8, abs_tol=1e-50, inner_tol=1e-5, max_inner_its=2000, max_outer=10):
+        """Averaged stress of each strain (n x 6), stats {outer, inner its}, final residual ratios."""
+        e = np.ascontiguousarray(strains, dtype=np.float64).reshape(-1, 6)
+        n = e.shape[0]
+        sig = np.zeros((n, 6))
+        stats = np.zeros(2, dtype=np.int32)
+        rel = np.zeros(n)
+        p = MixedParams(rel_tol, abs_tol, inner_tol, max_inner_its, max_outer)
+        rc = load().mppd_msolver_solve(self.h, n, _dp(e), C.cast(C.byref(p), C.c_void_p), _dp(sig), _ip(stats),
+                                       _dp(rel))
+        if rc not in (0, self.NOT_CONVERGED):
+            _chk(rc, "mppd_msolver_solve")
+        return sig, (int(stats[0]), int(stats[1])), rel, rc == 0
+
+    def u(self, k):
+        out = np.zeros(self.rve.nrow)
+        _chk(load().mppd_msolver_get_u(self.h, k, _dp(out)), "mppd_msolver_get_u")
+        return out
+
+    def profile(self, iters):
+        a, t = C.c_double(), C.c_double()
+        _chk(load().mppd_msolver_profile(self.h, iters, C.byref(a), C.byref(t)), "mppd_msolver_profile")
+        return a.value, t.value
+
+
+def homogenize_mixed(msolver: MSolver, strains, d_eps=1e-8, **kw):
+    """sigma of each strain and the linear-FE ctan from one multi-RHS solve.
+
+    The six unit-strain fields of calc_ctan_lin_fe_models (reference
+    src/micropp.cpp:255-284; eps = d_eps e_i from u = 0, ctan[:, i] =
+    sigma / d_eps) and the strains' own fields run together.
+    """
+    strains = np.asarray(strains, dtype=np.float64).reshape(-1, 6)
+    eps = np.vstack([np.eye(6) * d_eps, strains])
+    sig, stats, rel, ok = msolver.solve(eps, **kw)
+    ctan = sig[:6].T / d_eps
+    return sig[6:], ctan, stats, rel, ok
+
+
+def cg_profile(solver: "Solver", b: DeviceArray, iters: int):
+    """Average (SpMV kernel ms, whole CG iteration ms) over ``iters`` iterations."""
+    a, t = C.c_double(), C.c_double()
+    _chk(load().mppd_cg_profile(solver.h, b.ptr, iters, C.byref(a), C.byref(t)), "mppd_cg_profile")
+    return a.value, t.value
 
 
 def material_eval(m: MaterialBase, eps: np.ndarray, hist: np.ndarray | None = None):

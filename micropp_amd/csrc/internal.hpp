@@ -94,12 +94,41 @@ hipError_t launch_vars_new(const Grid &g, const double *bmat, const material_bas
                            const double *vars_old, double *vars_new, RedState *red, hipStream_t st);
 
 // ---------------- launchers (solver.hip) ----------------
+// CG state is a pair: iteration k reads state[k & 1] and writes
+// state[(k + 1) & 1] (parity = k).  part needs 3 * (nrow_pad / 256) doubles.
 hipError_t launch_spmv(const Grid &g, const int *cols, const double *vals, const double *x, double *y, hipStream_t st);
-hipError_t launch_cg_init(const Grid &g, const int *cols, const double *vals, const double *b, double *x, double *r,
-                          double *z, double *p, double *kinv, double *partials, CgState *cg, hipStream_t st);
+hipError_t launch_cg_init(const Grid &g, const double *vals, const double *b, double *x, double *r, double *z,
+                          double *p, double *kinv, double *part, double min_err, double rel_err, int max_its,
+                          CgState *state, hipStream_t st);
 hipError_t launch_cg_iter(const Grid &g, const int *cols, const double *vals, double *x, double *r, double *z,
-                          double *p, double *Ap, const double *kinv, double *partials, CgState *cg, hipStream_t st);
+                          double *p, double *Ap, const double *kinv, double *part, CgState *state, int parity,
+                          hipStream_t st);
+hipError_t launch_cg_step1(const Grid &g, const int *cols, const double *vals, const double *p, double *Ap,
+                           double *part, CgState *state, int parity, hipStream_t st);
+hipError_t launch_cg_step23(const Grid &g, double *x, double *r, double *z, double *p, const double *Ap,
+                            const double *kinv, double *part, CgState *state, int parity, hipStream_t st);
 hipError_t launch_axpy(int n, double a, const double *x, double *y, hipStream_t st);
 hipError_t launch_copy(int n, const double *x, double *y, hipStream_t st);
+
+// ---------------- launchers (mixed.hip): K right-hand sides, K in {4,8,12,16} ----------------
+size_t mixed_matrix_floats(const Grid &g);
+size_t mixed_vector_len(const Grid &g, int K);
+size_t mixed_partials(const Grid &g, int K);
+size_t mixed_state_bytes();
+hipError_t launch_m_asm32(const Grid &g, const int *et, const double *ke, int phase_mode, float *A, float *kinv,
+                          hipStream_t st);
+hipError_t launch_m_setbc(const Grid &g, int K, const double *eps, double *U, hipStream_t st);
+hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke, const double *U, const float *kinv,
+                          float *R, float *X, float *P, double *part, int first, double rel, double abs_tol,
+                          double tau_min, int max_inner, void *state, hipStream_t st);
+hipError_t launch_m_iter(const Grid &g, int K, const float *A, float *X, float *R, float *P, float *AP,
+                         const float *kinv, double *part, void *state, int parity, hipStream_t st);
+hipError_t launch_m_spmv_only(const Grid &g, int K, const float *A, const float *P, float *AP, double *part,
+                              const void *state, hipStream_t st);
+hipError_t launch_m_accum(const Grid &g, int K, const float *X, double *U, hipStream_t st);
+hipError_t launch_m_ave(const Grid &g, int K, const double *bmat, const material_base *mats_d, const int *et,
+                        const double *U, double *part, double *out, hipStream_t st);
+void mixed_state_view(const void *state_h, int parity, int *its, int *all_done);
+void mixed_outer_view(const void *state_h, int K, int *all_conv, double *pn, double *pn0);
 
 }  // namespace mpp

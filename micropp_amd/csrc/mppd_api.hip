@@ -302,7 +302,7 @@ int mppd_solver_create(mppd_solver **out, mppd_rve *r) {
     if (rc) break;
     const size_t nb = std::max(round_up(np, 256) / 256, round_up((size_t)g.nelem, 256) / 256) + 1;
     if ((rc = dalloc(&s->partials, nb * 8))) break;
-    if ((rc = dalloc(&s->cg, 1))) break;
+    if ((rc = dalloc(&s->cg, 2))) break;  // ping-pong pair
     if ((rc = dalloc(&s->red, 1))) break;
     e = hipHostMalloc((void **)&s->cg_h, sizeof(CgState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&s->red_h, sizeof(RedState), hipHostMallocDefault);
@@ -420,30 +420,26 @@ int mppd_ell_mvp(mppd_solver *s, const double *x_d, double *y_d) {
 
 // CG on the device; the host queues iterations in chunks sized from the
 // observed convergence rate and reads the 64-byte state between chunks.
+// After `queued` iterations the current state is state[queued & 1].
 static int cg_run(mppd_solver *s, const mppd_solver *A, const double *b_d, double min_err, double rel_err,
                   int max_its, int *its_out, double *err_out) {
   const Grid &g = s->r->g;
-  // the pinned mirror is idle here: every earlier use ended with a stream sync
-  *s->cg_h = CgState{};
-  s->cg_h->min_err = min_err;
-  s->cg_h->rel_err = rel_err;
-  s->cg_h->max_its = max_its;
-  MPP_HIP(hipMemcpyAsync(s->cg, s->cg_h, sizeof(CgState), hipMemcpyHostToDevice, s->st));
-  MPP_HIP(launch_cg_init(g, A->cols, A->vals, b_d, s->x, s->rr, s->z, s->p, s->kinv, s->partials, s->cg, s->st));
+  MPP_HIP(launch_cg_init(g, A->vals, b_d, s->x, s->rr, s->z, s->p, s->kinv, s->partials, min_err, rel_err, max_its,
+                         s->cg, s->st));
+  int queued = 0;
   MPP_HIP(hipMemcpyAsync(s->cg_h, s->cg, sizeof(CgState), hipMemcpyDeviceToHost, s->st));
   MPP_HIP(hipStreamSynchronize(s->st));
-  int queued = 0;
   double pn_prev = s->cg_h->pn;
   int its_prev = 0;
-  int chunk = 8;
+  int chunk = 32;
   while (!s->cg_h->done) {
     const int n = std::min(chunk, max_its - queued);
     if (n <= 0) break;
     for (int i = 0; i < n; ++i)
       MPP_HIP(launch_cg_iter(g, A->cols, A->vals, s->x, s->rr, s->z, s->p, s->Ap, s->kinv, s->partials, s->cg,
-                             s->st));
+                             queued + i, s->st));
     queued += n;
-    MPP_HIP(hipMemcpyAsync(s->cg_h, s->cg, sizeof(CgState), hipMemcpyDeviceToHost, s->st));
+    MPP_HIP(hipMemcpyAsync(s->cg_h, s->cg + (queued & 1), sizeof(CgState), hipMemcpyDeviceToHost, s->st));
     MPP_HIP(hipStreamSynchronize(s->st));
     const CgState &c = *s->cg_h;
     if (c.done) break;
@@ -528,6 +524,207 @@ int mppd_newton(mppd_solver *s, const mppd_solver *A0, const double eps[6], cons
   out[0] = its;
   out[1] = solver_its;
   out[2] = conv;
+  return MPPD_OK;
+}
+
+int mppd_cg_profile(mppd_solver *s, const double *b_d, int iters, double *spmv_ms, double *iter_ms) {
+  if (!s || !b_d || iters <= 0) return fail(MPPD_ERR_ARG, "bad cg_profile arguments");
+  const Grid &g = s->r->g;
+  // min_err = rel_err = 0: the stopping test never holds within iters + 1
+  MPP_HIP(launch_cg_init(g, s->vals, b_d, s->x, s->rr, s->z, s->p, s->kinv, s->partials, 0.0, 0.0, iters + 1, s->cg,
+                         s->st));
+  hipEvent_t ev[3];
+  for (auto &x : ev) MPP_HIP(hipEventCreate(&x));
+  // (1) the SpMV kernel back to back: it reads p and state[0], writes Ap and
+  // its partials, so repeating it leaves the CG state unchanged
+  hipError_t e = hipEventRecord(ev[0], s->st);
+  for (int i = 0; i < iters && e == hipSuccess; ++i)
+    e = launch_cg_step1(g, s->cols, s->vals, s->p, s->Ap, s->partials, s->cg, 0, s->st);
+  // (2) whole iterations
+  if (e == hipSuccess) e = hipEventRecord(ev[1], s->st);
+  for (int i = 0; i < iters && e == hipSuccess; ++i)
+    e = launch_cg_iter(g, s->cols, s->vals, s->x, s->rr, s->z, s->p, s->Ap, s->kinv, s->partials, s->cg, i, s->st);
+  if (e == hipSuccess) e = hipEventRecord(ev[2], s->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  float ms1 = 0.f, ms2 = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms1, ev[0], ev[1]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms2, ev[1], ev[2]);
+  for (auto &x : ev) (void)hipEventDestroy(x);
+  if (e != hipSuccess) return hip_fail(e, "cg_profile");
+  if (spmv_ms) *spmv_ms = (double)ms1 / iters;
+  if (iter_ms) *iter_ms = (double)ms2 / iters;
+  return MPPD_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// mixed-precision multi-RHS solver
+// ---------------------------------------------------------------------------
+struct mppd_msolver {
+  mppd_rve *r = nullptr;
+  int K = 0;
+  hipStream_t st = nullptr;
+  double *ke = nullptr;         // phase table 3 x 576 (FP64)
+  material_base *mats_d = nullptr;
+  float *A = nullptr, *kinv = nullptr;
+  float *X = nullptr, *R = nullptr, *P = nullptr, *AP = nullptr;
+  double *U = nullptr, *eps_d = nullptr, *part = nullptr, *sig_d = nullptr;
+  void *state = nullptr, *state_h = nullptr;
+  double *eps_h = nullptr, *sig_h = nullptr;  // pinned staging
+};
+
+extern "C" {
+
+int mppd_msolver_create(mppd_msolver **out, mppd_rve *r, int nrhs) {
+  *out = nullptr;
+  if (!r) return fail(MPPD_ERR_ARG, "null rve");
+  if (!(nrhs == 4 || nrhs == 8 || nrhs == 12 || nrhs == 16)) return fail(MPPD_ERR_ARG, "nrhs must be 4, 8, 12 or 16");
+  if (!r->all_elastic) return fail(MPPD_ERR_ARG, "the mixed-precision solver needs linear elastic phases");
+  mppd_msolver *m = new mppd_msolver();
+  m->r = r;
+  m->K = nrhs;
+  const Grid &g = r->g;
+  int rc = MPPD_OK;
+  do {
+    hipError_t e = hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "hipStreamCreate");
+      break;
+    }
+    const size_t nv = mixed_vector_len(g, nrhs);
+    if ((rc = dalloc(&m->ke, 3 * 576))) break;
+    if ((rc = dalloc(&m->mats_d, 3))) break;
+    if ((rc = dalloc(&m->A, mixed_matrix_floats(g), false))) break;
+    if ((rc = dalloc(&m->kinv, nv / nrhs))) break;
+    if ((rc = dalloc(&m->X, nv))) break;
+    if ((rc = dalloc(&m->R, nv))) break;
+    if ((rc = dalloc(&m->P, nv))) break;
+    if ((rc = dalloc(&m->AP, nv))) break;
+    if ((rc = dalloc(&m->U, nv))) break;
+    if ((rc = dalloc(&m->eps_d, 16 * 6))) break;
+    if ((rc = dalloc(&m->sig_d, 16 * 6))) break;
+    if ((rc = dalloc(&m->part, mixed_partials(g, nrhs)))) break;
+    if ((rc = dalloc((char **)&m->state, mixed_state_bytes()))) break;
+    e = hipHostMalloc(&m->state_h, mixed_state_bytes(), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&m->eps_h, 16 * 6 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&m->sig_h, 16 * 6 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemcpy(m->mats_d, r->mats, 3 * sizeof(material_base), hipMemcpyHostToDevice);
+    // element stiffness per phase (FP64, the plain path's kernel), then the FP32 node-block matrix
+    if (e == hipSuccess) e = launch_elem_ke(g, r->bmat, r->mats, r->et, nullptr, nullptr, 1, m->ke, m->st);
+    if (e == hipSuccess) e = launch_m_asm32(g, r->et, m->ke, 1, m->A, m->kinv, m->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+    if (e != hipSuccess) rc = hip_fail(e, "msolver set-up");
+  } while (0);
+  if (rc) {
+    mppd_msolver_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return MPPD_OK;
+}
+
+int mppd_msolver_destroy(mppd_msolver *m) {
+  if (!m) return MPPD_OK;
+  if (m->st) (void)hipStreamSynchronize(m->st);
+  for (void *p : {(void *)m->ke, (void *)m->mats_d, (void *)m->A, (void *)m->kinv, (void *)m->X, (void *)m->R,
+                  (void *)m->P, (void *)m->AP, (void *)m->U, (void *)m->eps_d, (void *)m->sig_d, (void *)m->part,
+                  m->state})
+    dfree(p);
+  for (void *p : {m->state_h, (void *)m->eps_h, (void *)m->sig_h})
+    if (p) (void)hipHostFree(p);
+  if (m->st) (void)hipStreamDestroy(m->st);
+  delete m;
+  return MPPD_OK;
+}
+
+int mppd_msolver_solve(mppd_msolver *m, int n, const double *eps_h, const mppd_mixed_params *p, double *sig_h,
+                       int stats[2], double *rel_h) {
+  if (!m || !eps_h || !p || !sig_h || n < 1 || n > m->K) return fail(MPPD_ERR_ARG, "bad msolver_solve arguments");
+  const Grid &g = m->r->g;
+  const int K = m->K;
+  for (int i = 0; i < K * 6; ++i) m->eps_h[i] = i < n * 6 ? eps_h[i] : 0.0;
+  MPP_HIP(hipMemcpyAsync(m->eps_d, m->eps_h, K * 6 * sizeof(double), hipMemcpyHostToDevice, m->st));
+  MPP_HIP(hipMemsetAsync(m->U, 0, mixed_vector_len(g, K) * sizeof(double), m->st));
+  MPP_HIP(launch_m_setbc(g, K, m->eps_d, m->U, m->st));
+  int outer = 0, inner_total = 0, all_conv = 0;
+  std::vector<double> pn(K), pn0(K);
+  for (;;) {
+    MPP_HIP(launch_m_resid(g, K, m->r->et, m->ke, m->U, m->kinv, m->R, m->X, m->P, m->part, outer == 0, p->rel_tol,
+                           p->abs_tol, p->inner_tol, p->max_inner_its, m->state, m->st));
+    MPP_HIP(hipMemcpyAsync(m->state_h, m->state, mixed_state_bytes(), hipMemcpyDeviceToHost, m->st));
+    MPP_HIP(hipStreamSynchronize(m->st));
+    mixed_outer_view(m->state_h, K, &all_conv, pn.data(), pn0.data());
+    if (all_conv || outer >= p->max_outer) break;
+    // inner FP32 CG, queued in chunks; after q iterations the state is state[q & 1]
+    int queued = 0, its = 0, done = 0;
+    int chunk = 32;
+    while (!done) {
+      const int nq = std::min(chunk, p->max_inner_its - queued);
+      if (nq <= 0) break;
+      for (int i = 0; i < nq; ++i)
+        MPP_HIP(launch_m_iter(g, K, m->A, m->X, m->R, m->P, m->AP, m->kinv, m->part, m->state, queued + i, m->st));
+      queued += nq;
+      MPP_HIP(hipMemcpyAsync(m->state_h, m->state, mixed_state_bytes(), hipMemcpyDeviceToHost, m->st));
+      MPP_HIP(hipStreamSynchronize(m->st));
+      mixed_state_view(m->state_h, queued, &its, &done);
+      chunk = std::min(128, chunk * 2);
+    }
+    inner_total += its;
+    MPP_HIP(launch_m_accum(g, K, m->X, m->U, m->st));
+    ++outer;
+  }
+  MPP_HIP(launch_m_ave(g, K, m->r->bmat, m->mats_d, m->r->et, m->U, m->part, m->sig_d, m->st));
+  MPP_HIP(hipMemcpyAsync(m->sig_h, m->sig_d, K * 6 * sizeof(double), hipMemcpyDeviceToHost, m->st));
+  MPP_HIP(hipStreamSynchronize(m->st));
+  for (int i = 0; i < n * 6; ++i) sig_h[i] = m->sig_h[i];
+  if (stats) {
+    stats[0] = outer;
+    stats[1] = inner_total;
+  }
+  if (rel_h)
+    for (int i = 0; i < n; ++i) rel_h[i] = pn0[i] > 0 ? pn[i] / pn0[i] : 0.0;
+  return all_conv ? MPPD_OK : fail(MPPD_ERR_NOTCONV, "mixed solve: tolerance not reached");
+}
+
+int mppd_msolver_get_u(mppd_msolver *m, int k, double *u_h) {
+  if (!m || !u_h || k < 0 || k >= m->K) return fail(MPPD_ERR_ARG, "bad msolver_get_u arguments");
+  const Grid &g = m->r->g;
+  std::vector<double> U(mixed_vector_len(g, m->K));
+  MPP_HIP(hipMemcpyAsync(U.data(), m->U, U.size() * sizeof(double), hipMemcpyDeviceToHost, m->st));
+  MPP_HIP(hipStreamSynchronize(m->st));
+  for (int i = 0; i < g.nrow; ++i) u_h[i] = U[(size_t)i * m->K + k];
+  return MPPD_OK;
+}
+
+int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *iter_ms) {
+  if (!m || iters <= 0) return fail(MPPD_ERR_ARG, "bad msolver_profile arguments");
+  const Grid &g = m->r->g;
+  const int K = m->K;
+  // a live inner state: unit strains on every field, no inner stopping
+  for (int i = 0; i < K * 6; ++i) m->eps_h[i] = (i % 7 == 0) ? 1e-3 : 0.0;
+  MPP_HIP(hipMemcpyAsync(m->eps_d, m->eps_h, K * 6 * sizeof(double), hipMemcpyHostToDevice, m->st));
+  MPP_HIP(hipMemsetAsync(m->U, 0, mixed_vector_len(g, K) * sizeof(double), m->st));
+  MPP_HIP(launch_m_setbc(g, K, m->eps_d, m->U, m->st));
+  MPP_HIP(launch_m_resid(g, K, m->r->et, m->ke, m->U, m->kinv, m->R, m->X, m->P, m->part, 1, 0.0, 0.0, 0.0,
+                         iters + 1, m->state, m->st));
+  hipEvent_t ev[3];
+  for (auto &x : ev) MPP_HIP(hipEventCreate(&x));
+  hipError_t e = hipEventRecord(ev[0], m->st);
+  for (int i = 0; i < iters && e == hipSuccess; ++i)
+    e = launch_m_spmv_only(g, K, m->A, m->P, m->AP, m->part, m->state, m->st);
+  if (e == hipSuccess) e = hipEventRecord(ev[1], m->st);
+  for (int i = 0; i < iters && e == hipSuccess; ++i)
+    e = launch_m_iter(g, K, m->A, m->X, m->R, m->P, m->AP, m->kinv, m->part, m->state, i, m->st);
+  if (e == hipSuccess) e = hipEventRecord(ev[2], m->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+  float ms1 = 0.f, ms2 = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms1, ev[0], ev[1]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms2, ev[1], ev[2]);
+  for (auto &x : ev) (void)hipEventDestroy(x);
+  if (e != hipSuccess) return hip_fail(e, "msolver_profile");
+  if (spmv_ms) *spmv_ms = (double)ms1 / iters;
+  if (iter_ms) *iter_ms = (double)ms2 / iters;
   return MPPD_OK;
 }
 

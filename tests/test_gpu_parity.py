@@ -35,7 +35,9 @@ def mats_pair(mp, kind):
     elif kind == "plastic":
         spec = [(1, 1e6, 0.3, 1e5, 1e3), (0, 1e7, 0.25), (1, 2e6, 0.3, 1e4, 5e2)]
     else:
-        spec = [(2, 1e6, 0.3, 0, 0, 1e3), (0, 1e7, 0.25), (2, 1e6, 0.3, 0, 0, 1e3)]
+        # Xt = 1e5 as in the reference's benchmark-damage: the threshold sits above
+        # the hard-coded hardening start, so the law stays finite
+        spec = [(2, 1e6, 0.3, 0, 0, 1e5), (0, 1e7, 0.25), (2, 1e6, 0.3, 0, 0, 1e5)]
     return [mp.material(*s) for s in spec], [ob.material(*s) for s in spec]
 
 
@@ -52,8 +54,13 @@ def rand_state(rng, nvars, kind):
 
 
 def rel(a, b):
-    a, b = np.asarray(a), np.asarray(b)
-    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+    """max |a - b| / max |b|; NaNs must sit at the same places in both."""
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    m = ~np.isnan(b)
+    if not m.any():
+        return 0.0
+    return np.abs(a[m] - b[m]).max() / max(np.abs(b[m]).max(), 1e-300)
 
 
 # ---------------------------------------------------------------------------
@@ -248,7 +255,8 @@ def test_homogenize_parity(mp, oracle, kind, lin_stress, subits):
     assert rel(m.ctan_lin(), o.ctan_lin()) < 1e-9
     assert np.array_equal(m.volume_fractions(), o.volume_fractions())
     rng = np.random.default_rng(16)
-    strains = [rng.uniform(-2e-3, 2e-3, 6) for _ in range(4)]
+    amp = 0.02 if kind == "damage" else 2e-3  # damage starts near r = 100, strain ~ 0.04 here
+    strains = [rng.uniform(-amp, amp, 6) for _ in range(4)]
     a, b = _drive(m, strains, 4), _drive(o, strains, 4)
     for t, (sa, sb) in enumerate(zip(a, b)):
         for g, (ga, gb) in enumerate(zip(sa, sb)):
