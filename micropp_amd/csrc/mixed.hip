@@ -9,20 +9,25 @@
 // comes from an FP32 Jacobi-preconditioned CG on all K systems at once: one
 // pass over the FP32 matrix serves K vectors.  The volume averages are FP64.
 //
-// FP32 matrix layout (node-blocked SELL-64): one thread per node; the node's
-// 27 neighbour blocks of 3x3 coefficients (slot s = nb*9 + fi*3 + fj, nb as
-// in the reference ELL) are padded to 244 floats and stored as 61 float4 per
-// node, quad-major inside a slice of 64 nodes: quad q of node n sits at
-// float4 index ((n / 64) * 61 + q) * 64 + n % 64.  Column indices are implicit
-// (structured 27-point stencil).  Rows and columns of Dirichlet nodes are
-// dropped (the CG vectors are zero there), which leaves the symmetric interior
-// block.  Vectors are [node][3][K] so a neighbour gather reads 3K consecutive
-// floats.
+// FP32 matrix layout (node-blocked SELL-64): the 27 neighbour blocks of 3x3
+// coefficients of a node (slot s = nb*9 + fi*3 + fj, nb as in the reference
+// ELL) are padded to 244 floats and stored as 61 float4 per node, quad-major
+// inside a slice of 64 nodes: quad q of node n sits at float4 index
+// ((n / 64) * 61 + q) * 64 + n % 64.  Column indices are implicit (structured
+// 27-point stencil).  Rows and columns of Dirichlet nodes are dropped (the CG
+// vectors are zero there), which leaves the symmetric interior block.
 //
-// Reductions are fence-free: blocks write FP64 partials and the next kernel
-// re-sums them in a fixed order.  Iteration k of the inner CG reads
-// state[k & 1] and writes state[(k+1) & 1].  Row-space kernels map blocks to
-// XCDs so that each XCD's L2 holds a contiguous slab of nodes.
+// Vectors are component-major: component c = d * K + k (DOF direction d of
+// field k) of node n sits at c * NP + n, NP = nodes rounded up to 256.  A
+// wave of the SpMV owns 64 consecutive nodes and 4 fields, so every matrix
+// load and every neighbour gather is a contiguous 256-byte line.  The FP64
+// displacements U use the same layout.
+//
+// Reductions are fence-free: blocks write FP64 partials in a [row][block]
+// table and a one-block kernel re-sums each row in a fixed order.  Iteration
+// k of the inner CG reads state[k & 1] and writes state[(k+1) & 1].  The
+// SpMV maps slices to XCDs so that each XCD's L2 holds a contiguous slab of
+// nodes.
 #include "internal.hpp"
 #include "physics.hpp"
 #include "reduce.hpp"
@@ -31,9 +36,10 @@ namespace mpp {
 
 constexpr int kMaxRhs = 16;
 constexpr int kNodeQuads = 61;  // 244 floats per node (243 coefficients + 1 pad)
+constexpr int kRedThreads = 1024;  // one-block reducers: one wave per row
 
 struct MState {
-  double rz[kMaxRhs], pn[kMaxRhs], pn0[kMaxRhs], tau[kMaxRhs];
+  double rz[kMaxRhs], pn[kMaxRhs], pn0[kMaxRhs], tau[kMaxRhs], alpha[kMaxRhs], beta[kMaxRhs];
   int done[kMaxRhs];
   int its, max_its, all_done, pad;
 };
@@ -43,6 +49,8 @@ struct MOuter {
   int conv[kMaxRhs];
   int all_conv, pad[3];
 };
+
+typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bool node_on_boundary(const Grid &g, int i, int j, int k) {
   return i == 0 || j == 0 || k == 0 || i == g.nx - 1 || j == g.ny - 1 || k == g.nz - 1;
@@ -55,27 +63,35 @@ __device__ __forceinline__ int xcd_block(int per) {
   return (b & 7) * per + (b >> 3);
 }
 
+// Sum of one row of a partial table, by one wave, lanes striding the row in a
+// fixed order.
+__device__ __forceinline__ double wave_row_sum(const double *row, int n) {
+  double s = 0.0;
+  for (int b = threadIdx.x & 63; b < n; b += 64) s += row[b];
+  return wave_sum(s);
+}
+
 // ---------------------------------------------------------------------------
 // FP32 matrix from the element blocks (phase table or per element), and the
-// FP32 Jacobi weights.  Per slot, the element contributions are summed in
-// FP64 in the reference element order, then rounded once.
+// FP32 Jacobi weights kinv[d * NP + n].  Per slot, the element contributions
+// are summed in FP64 in the reference element order, then rounded once.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_m_asm32(Grid g, int nn_pad, const int *et, const double *ke, int phase_mode,
+__global__ void __launch_bounds__(256) k_m_asm32(Grid g, int NP, const int *et, const double *ke, int phase_mode,
                                                  float *A, float *kinv) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn_pad) return;
+  if (n >= NP) return;
   float *base = A + ((size_t)(n >> 6) * kNodeQuads * 64 + (n & 63)) * 4;
   auto put = [&](int s, float v) { base[(size_t)(s >> 2) * 256 + (s & 3)] = v; };
   for (int s = 0; s < kNodeQuads * 4; ++s) put(s, 0.0f);
   if (n >= g.nn) {
-    for (int d = 0; d < 3; ++d) kinv[(size_t)n * 3 + d] = 0.0f;
+    for (int d = 0; d < 3; ++d) kinv[(size_t)d * NP + n] = 0.0f;
     return;
   }
   const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
   if (node_on_boundary(g, i, j, k)) {
     for (int d = 0; d < 3; ++d) {
       put(13 * 9 + d * 3 + d, 1.0f);
-      kinv[(size_t)n * 3 + d] = 0.0f;  // keeps z, p and Ap zero on Dirichlet DOFs
+      kinv[(size_t)d * NP + n] = 0.0f;  // keeps z, p and Ap zero on Dirichlet DOFs
     }
     return;
   }
@@ -110,7 +126,7 @@ __global__ void __launch_bounds__(256) k_m_asm32(Grid g, int nn_pad, const int *
           }
         }
         put(nb * 9 + fi * 3 + fj, (float)acc);
-        if (nb == 13 && fi == fj) kinv[(size_t)n * 3 + fi] = (float)(1.0 / acc);
+        if (nb == 13 && fi == fj) kinv[(size_t)fi * NP + n] = (float)(1.0 / acc);
       }
   }
 }
@@ -119,7 +135,7 @@ __global__ void __launch_bounds__(256) k_m_asm32(Grid g, int nn_pad, const int *
 // boundary values of K fields (set_displ_bc, src/micro3D.cpp:26-78), FP64
 // ---------------------------------------------------------------------------
 template <int K>
-__global__ void __launch_bounds__(256) k_m_setbc(Grid g, const double *eps, double *U) {
+__global__ void __launch_bounds__(256) k_m_setbc(Grid g, int NP, const double *eps, double *U) {
 #pragma clang fp contract(off)
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= g.nn) return;
@@ -146,7 +162,7 @@ __global__ void __launch_bounds__(256) k_m_setbc(Grid g, const double *eps, doub
     for (int d = 0; d < 3; ++d) {
       double s = 0.0;
       for (int c = 0; c < 3; ++c) s += et[d][c] * x[c];
-      U[((size_t)n * 3 + d) * K + r] = s;
+      U[(size_t)(d * K + r) * NP + n] = s;
     }
   }
 }
@@ -154,10 +170,11 @@ __global__ void __launch_bounds__(256) k_m_setbc(Grid g, const double *eps, doub
 // ---------------------------------------------------------------------------
 // FP64 residual r = -(K u) on interior DOFs (elastic: element blocks of the
 // phase table), rounded into the FP32 CG start: R = r, X = 0, P = kinv R.
-// Partials per block: [ (kinv r)^2 summed in FP64 | R.P ] for each field.
+// Partials: part[q * gridDim.x + block], q < K: (kinv r)^2 in FP64 of field
+// q; q >= K: R.P of field q - K.
 // ---------------------------------------------------------------------------
 template <int K>
-__global__ void __launch_bounds__(256) k_m_resid(Grid g, int nn_pad, int per, const int *et, const double *ke,
+__global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const int *et, const double *ke,
                                                  const double *U, const float *kinv, float *R, float *X, float *P,
                                                  double *part) {
   __shared__ double ke_s[3 * 576];
@@ -169,7 +186,7 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int nn_pad, int per, co
   double zz[K], rz[K];
 #pragma unroll
   for (int r = 0; r < K; ++r) zz[r] = rz[r] = 0.0;
-  if (n < nn_pad) {
+  if (n < NP) {
     const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
     const bool interior = n < g.nn && !node_on_boundary(g, i, j, k);
 #pragma unroll
@@ -188,18 +205,16 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int nn_pad, int per, co
               const double *kb = ke_s + et[e] * 576 + local_node(1 - a, 1 - b, 1 - c) * 3 * 24;
               for (int m = 0; m < 8; ++m) {
                 const int node = ((ez + kHexPos[m][2]) * g.ny + (ey + kHexPos[m][1])) * g.nx + ex + kHexPos[m][0];
-                const double *u = U + (size_t)node * 3 * K + kg;
 #pragma unroll
                 for (int fj = 0; fj < 3; ++fj) {
-                  const double2 u01 = *reinterpret_cast<const double2 *>(u + fj * K);
-                  const double2 u23 = *reinterpret_cast<const double2 *>(u + fj * K + 2);
+                  double u[4];
+#pragma unroll
+                  for (int q = 0; q < 4; ++q) u[q] = U[(size_t)(fj * K + kg + q) * NP + node];
 #pragma unroll
                   for (int d = 0; d < 3; ++d) {
                     const double cf = kb[d * 24 + m * 3 + fj];
-                    acc[d][0] += cf * u01.x;
-                    acc[d][1] += cf * u01.y;
-                    acc[d][2] += cf * u23.x;
-                    acc[d][3] += cf * u23.y;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[d][q] += cf * u[q];
                   }
                 }
               }
@@ -207,13 +222,13 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int nn_pad, int per, co
       }
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const float kv = kinv[(size_t)n * 3 + d];
+        const float kv = kinv[(size_t)d * NP + n];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const double r64 = -acc[d][q];
           const float r32 = (float)r64;
           const float p32 = kv * r32;
-          const size_t idx = ((size_t)n * 3 + d) * K + kg + q;
+          const size_t idx = (size_t)(d * K + kg + q) * NP + n;
           R[idx] = r32;
           X[idx] = 0.0f;
           P[idx] = p32;
@@ -233,29 +248,21 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int nn_pad, int per, co
   block_sum<2 * K, 256>(v, lds);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < 2 * K; ++q) part[(size_t)blockIdx.x * 2 * K + q] = v[q];
-}
-
-template <int NV>
-__device__ __forceinline__ void sum_parts(const double *part, int nparts, int stride, double (&out)[NV], double *lds) {
-  double acc[NV];
-#pragma unroll
-  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-  for (int b = threadIdx.x; b < nparts; b += 256)
-#pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] += part[(size_t)b * stride + q];
-  block_sum<NV, 256>(acc, lds);
-#pragma unroll
-  for (int q = 0; q < NV; ++q) out[q] = acc[q];
+    for (int q = 0; q < 2 * K; ++q) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
 }
 
 // one block: outer convergence test and the inner CG start state
 template <int K>
-__global__ void __launch_bounds__(256) k_m_outer(int nparts, const double *part, int first, double rel, double abs_tol,
-                                                 double tau_min, int max_inner, MOuter *o, MState *s0) {
-  __shared__ double lds[4 * 2 * K];
-  double t[2 * K];
-  sum_parts<2 * K>(part, nparts, 2 * K, t, lds);
+__global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const double *part, int first, double rel,
+                                                         double abs_tol, double tau_min, int max_inner, MOuter *o,
+                                                         MState *s0) {
+  __shared__ double t[2 * kMaxRhs];
+  const int w = threadIdx.x >> 6;
+  for (int q = w; q < 2 * K; q += kRedThreads / 64) {
+    const double s = wave_row_sum(part + (size_t)q * nparts, nparts);
+    if ((threadIdx.x & 63) == 0) t[q] = s;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
   int all = 1;
 #pragma unroll
@@ -274,6 +281,8 @@ __global__ void __launch_bounds__(256) k_m_outer(int nparts, const double *part,
     s0->pn0[r] = pn;
     // aim a little below what the outer test needs, but not below what FP32 can resolve
     s0->tau[r] = (pn > 0.0) ? fmax(tau_min, 0.3 * o->target[r] / pn) : 1.0;
+    s0->alpha[r] = 0.0;
+    s0->beta[r] = 0.0;
     s0->done[r] = conv;
   }
   o->all_conv = all;
@@ -283,214 +292,204 @@ __global__ void __launch_bounds__(256) k_m_outer(int nparts, const double *part,
 }
 
 // ---------------------------------------------------------------------------
-// inner CG, iteration part 1: AP = A P and partials of P.AP (FP64)
+// inner CG, step 1: AP = A P and the partials of P.AP (FP64).  One block per
+// slice of 64 nodes; wave w handles fields 4w .. 4w+3, so a block has K/4
+// waves.  Partial table part1[field * nsl + slice].
 // ---------------------------------------------------------------------------
 template <int K>
-__global__ void __launch_bounds__(256) k_m_spmv(Grid g, int nn_pad, int per, const float4 *A, const float *P, float *AP,
-                                                double *part, const MState *cur) {
-  __shared__ double lds[4 * K];
+__global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per, const float4 *A, const float *P,
+                                                float *AP, double *part1, const MState *cur) {
   if (cur->all_done) return;
-  const int lb = xcd_block(per);
-  const int n = lb * 256 + threadIdx.x;
-  double pap[K];
+  const int slice = xcd_block(per);
+  if (slice >= nsl) return;
+  const int r0 = (threadIdx.x >> 6) * 4, lane = threadIdx.x & 63;
+  if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole wave
+  const int n = slice * 64 + lane;
+  float acc[3][4], pc[3][4];
 #pragma unroll
-  for (int r = 0; r < K; ++r) pap[r] = 0.0;
-  if (n < nn_pad) {
-    float acc[3][K];
+  for (int d = 0; d < 3; ++d)
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int r = 0; r < K; ++r) acc[d][r] = 0.0f;
-    if (n < g.nn) {
-      const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
-      const int nxny = g.nx * g.ny;
-      typedef float f4v __attribute__((ext_vector_type(4)));
-      const f4v *arow = reinterpret_cast<const f4v *>(A) + (size_t)(n >> 6) * kNodeQuads * 64 + (n & 63);
-      // 7 chunks of up to 4 neighbours = 9 quads each (the last: 3 neighbours,
-      // 7 quads); the chunk loop stays rolled to bound the live registers
+    for (int r = 0; r < 4; ++r) acc[d][r] = pc[d][r] = 0.0f;
+  if (n < g.nn) {
+    const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
+    const int nxny = g.nx * g.ny;
+    const f4v *arow = reinterpret_cast<const f4v *>(A) + (size_t)slice * kNodeQuads * 64 + lane;
+    const float *pf = P + (size_t)r0 * NP;
+    // 7 chunks of up to 4 neighbours = 9 quads each (the last: 3 neighbours,
+    // 7 quads); the chunk loop stays rolled to bound the live registers
 #pragma unroll 1
-      for (int ch = 0; ch < 7; ++ch) {
-        constexpr int kq = 9;
-        float a[36];
+    for (int ch = 0; ch < 7; ++ch) {
+      float a[36];
 #pragma unroll
-        for (int q = 0; q < kq; ++q) {
-          if (ch * 9 + q < kNodeQuads) {
-            const f4v v = __builtin_nontemporal_load(&arow[(size_t)(ch * 9 + q) * 64]);
-            a[q * 4 + 0] = v.x;
-            a[q * 4 + 1] = v.y;
-            a[q * 4 + 2] = v.z;
-            a[q * 4 + 3] = v.w;
-          }
+      for (int q = 0; q < 9; ++q)
+        if (ch * 9 + q < kNodeQuads) {
+          const f4v v = arow[(size_t)(ch * 9 + q) * 64];
+          a[q * 4 + 0] = v.x;
+          a[q * 4 + 1] = v.y;
+          a[q * 4 + 2] = v.z;
+          a[q * 4 + 3] = v.w;
         }
+      float pv[4][3][4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int nb = ch * 4 + t;
-          if (nb < 27) {
-            const int dz = nb / 9 - 1, dy = (nb / 3) % 3 - 1, dx = nb % 3 - 1;
-            const bool in = (i + dx >= 0 && i + dx < g.nx) && (j + dy >= 0 && j + dy < g.ny) &&
-                            (k + dz >= 0 && k + dz < g.nz);
-            const int m = in ? n + dz * nxny + dy * g.nx + dx : n;  // outside: zero block, any valid node
-            const float *pm = P + (size_t)m * 3 * K;
+      for (int t = 0; t < 4; ++t) {
+        const int nb = ch * 4 + t;
+        const int dz = nb / 9 - 1, dy = (nb / 3) % 3 - 1, dx = nb % 3 - 1;
+        const bool in = nb < 27 && (i + dx >= 0 && i + dx < g.nx) && (j + dy >= 0 && j + dy < g.ny) &&
+                        (k + dz >= 0 && k + dz < g.nz);
+        const int m = in ? n + dz * nxny + dy * g.nx + dx : n;  // outside: zero block, any valid node
 #pragma unroll
-            for (int fj = 0; fj < 3; ++fj) {
-              float pv[K];
+        for (int fj = 0; fj < 3; ++fj)
 #pragma unroll
-              for (int r = 0; r < K; r += 4) {
-                const float4 v = *reinterpret_cast<const float4 *>(pm + fj * K + r);
-                pv[r] = v.x;
-                pv[r + 1] = v.y;
-                pv[r + 2] = v.z;
-                pv[r + 3] = v.w;
-              }
+          for (int r = 0; r < 4; ++r) pv[t][fj][r] = pf[(size_t)(fj * K + r) * NP + m];
+      }
+      if (ch == 3)  // neighbour 13 = ch 3, t 1 is the node itself: keep p for p.Ap
 #pragma unroll
-              for (int fi = 0; fi < 3; ++fi) {
-                const float c = a[t * 9 + fi * 3 + fj];
+        for (int fj = 0; fj < 3; ++fj)
 #pragma unroll
-                for (int r = 0; r < K; ++r) acc[fi][r] += c * pv[r];
-              }
+          for (int r = 0; r < 4; ++r) pc[fj][r] = pv[1][fj][r];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (ch * 4 + t < 27)
+#pragma unroll
+          for (int fj = 0; fj < 3; ++fj)
+#pragma unroll
+            for (int fi = 0; fi < 3; ++fi) {
+              const float c = a[t * 9 + fi * 3 + fj];
+#pragma unroll
+              for (int r = 0; r < 4; ++r) acc[fi][r] += c * pv[t][fj][r];
             }
-          }
-        }
-      }
     }
-    float *ap = AP + (size_t)n * 3 * K;
-    const float *pn = P + (size_t)n * 3 * K;
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int r = 0; r < K; r += 4) {
-        *reinterpret_cast<float4 *>(ap + d * K + r) = make_float4(acc[d][r], acc[d][r + 1], acc[d][r + 2], acc[d][r + 3]);
-        const float4 pv = *reinterpret_cast<const float4 *>(pn + d * K + r);
-        pap[r] += (double)pv.x * acc[d][r];
-        pap[r + 1] += (double)pv.y * acc[d][r + 1];
-        pap[r + 2] += (double)pv.z * acc[d][r + 2];
-        pap[r + 3] += (double)pv.w * acc[d][r + 3];
-      }
   }
-  block_sum<K, 256>(pap, lds);
-  if (threadIdx.x == 0)
+  float *apf = AP + (size_t)r0 * NP + n;
 #pragma unroll
-    for (int q = 0; q < K; ++q) part[(size_t)blockIdx.x * K + q] = pap[q];
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) apf[(size_t)(d * K + r) * NP] = acc[d][r];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s += (double)pc[d][r] * (double)acc[d][r];
+    s = wave_sum(s);
+    if (lane == 0) part1[(size_t)(r0 + r) * nsl + slice] = s;
+  }
 }
 
-// part 2: alpha; X += alpha P; R -= alpha AP; partials of z.z and R.z (z = kinv R)
+// step 2 (one block): alpha = rz / p.Ap of every live field
 template <int K>
-__global__ void __launch_bounds__(256) k_m_upd(int nn_pad, int per, int nparts, const double *part1, const float *P,
-                                               const float *AP, const float *kinv, float *X, float *R, double *part2,
-                                               const MState *cur) {
-  __shared__ double lds[4 * 2 * K];
-  __shared__ float alpha_s[K];
+__global__ void __launch_bounds__(kRedThreads) k_m_alpha(int nsl, const double *part1, MState *cur) {
   if (cur->all_done) return;
-  double pap[K];
-  sum_parts<K>(part1, nparts, K, pap, lds);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int r = 0; r < K; ++r) alpha_s[r] = cur->done[r] ? 0.0f : (float)(cur->rz[r] / pap[r]);
-  __syncthreads();
-  const int lb = xcd_block(per);
-  const int n = lb * 256 + threadIdx.x;
-  double zz[K], rz[K];
-#pragma unroll
-  for (int r = 0; r < K; ++r) zz[r] = rz[r] = 0.0;
-  if (n < nn_pad) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const float kv = kinv[(size_t)n * 3 + d];
-      const size_t o = ((size_t)n * 3 + d) * K;
-#pragma unroll
-      for (int r = 0; r < K; r += 4) {
-        const float4 p = *reinterpret_cast<const float4 *>(P + o + r);
-        const float4 ap = *reinterpret_cast<const float4 *>(AP + o + r);
-        float4 x = *reinterpret_cast<const float4 *>(X + o + r);
-        float4 rr = *reinterpret_cast<const float4 *>(R + o + r);
-        x.x += alpha_s[r] * p.x;
-        x.y += alpha_s[r + 1] * p.y;
-        x.z += alpha_s[r + 2] * p.z;
-        x.w += alpha_s[r + 3] * p.w;
-        rr.x -= alpha_s[r] * ap.x;
-        rr.y -= alpha_s[r + 1] * ap.y;
-        rr.z -= alpha_s[r + 2] * ap.z;
-        rr.w -= alpha_s[r + 3] * ap.w;
-        *reinterpret_cast<float4 *>(X + o + r) = x;
-        *reinterpret_cast<float4 *>(R + o + r) = rr;
-        const float z0 = kv * rr.x, z1 = kv * rr.y, z2 = kv * rr.z, z3 = kv * rr.w;
-        zz[r] += (double)z0 * z0;
-        zz[r + 1] += (double)z1 * z1;
-        zz[r + 2] += (double)z2 * z2;
-        zz[r + 3] += (double)z3 * z3;
-        rz[r] += (double)rr.x * z0;
-        rz[r + 1] += (double)rr.y * z1;
-        rz[r + 2] += (double)rr.z * z2;
-        rz[r + 3] += (double)rr.w * z3;
-      }
-    }
-  }
-  double v[2 * K];
-#pragma unroll
-  for (int r = 0; r < K; ++r) {
-    v[r] = zz[r];
-    v[K + r] = rz[r];
-  }
-  block_sum<2 * K, 256>(v, lds);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int q = 0; q < 2 * K; ++q) part2[(size_t)blockIdx.x * 2 * K + q] = v[q];
+  const int w = threadIdx.x >> 6;
+  if (w >= K || cur->done[w]) return;  // whole wave
+  const double pap = wave_row_sum(part1 + (size_t)w * nsl, nsl);
+  if ((threadIdx.x & 63) == 0) cur->alpha[w] = cur->rz[w] / pap;
 }
 
-// part 3: beta; P = kinv R + beta P; block 0 writes the next state
+// step 3: X += alpha P; R -= alpha AP; partials of z.z and R.z (z = kinv R).
+// Grid (float4 chunks of a component, component c = d * K + k); partial
+// tables pz / pr [c * gridDim.x + block].
 template <int K>
-__global__ void __launch_bounds__(256) k_m_pupd(int nn_pad, int per, int nparts, const double *part2, const float *R,
-                                                const float *kinv, float *P, const MState *cur, MState *nxt) {
-  __shared__ double lds[4 * 2 * K];
-  __shared__ float beta_s[K];
+__global__ void __launch_bounds__(256) k_m_upd(int NP, const float4 *P, const float4 *AP, const float4 *kinv, float4 *X,
+                                               float4 *R, double *pz, double *pr, const MState *cur) {
+  __shared__ double lds[4 * 2];
+  const int c = blockIdx.y, f = c % K, d = c / K;
+  if (cur->all_done || cur->done[f]) return;  // whole block
+  const float al = (float)cur->alpha[f];
+  const int q = blockIdx.x * 256 + threadIdx.x, nq = NP / 4;
+  double v[2] = {0.0, 0.0};
+  if (q < nq) {
+    const size_t o = (size_t)c * nq + q;
+    const float4 p = P[o], ap = AP[o], kv = kinv[(size_t)d * nq + q];
+    float4 x = X[o], r = R[o];
+    x.x += al * p.x;
+    x.y += al * p.y;
+    x.z += al * p.z;
+    x.w += al * p.w;
+    r.x -= al * ap.x;
+    r.y -= al * ap.y;
+    r.z -= al * ap.z;
+    r.w -= al * ap.w;
+    X[o] = x;
+    R[o] = r;
+    const float z0 = kv.x * r.x, z1 = kv.y * r.y, z2 = kv.z * r.z, z3 = kv.w * r.w;
+    v[0] = (double)z0 * z0 + (double)z1 * z1 + (double)z2 * z2 + (double)z3 * z3;
+    v[1] = (double)r.x * z0 + (double)r.y * z1 + (double)r.z * z2 + (double)r.w * z3;
+  }
+  block_sum<2, 256>(v, lds);
+  if (threadIdx.x == 0) {
+    pz[(size_t)c * gridDim.x + blockIdx.x] = v[0];
+    pr[(size_t)c * gridDim.x + blockIdx.x] = v[1];
+  }
+}
+
+// step 4 (one block): beta, stopping test, the next state
+template <int K>
+__global__ void __launch_bounds__(kRedThreads) k_m_beta(int nbx, const double *pz, const double *pr,
+                                                        const MState *cur, MState *nxt) {
+  __shared__ double zz_s[kMaxRhs], rz_s[kMaxRhs];
   if (cur->all_done) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = *cur;
+    const int *s = reinterpret_cast<const int *>(cur);
+    int *t = reinterpret_cast<int *>(nxt);
+    for (int q = threadIdx.x; q < (int)(sizeof(MState) / sizeof(int)); q += kRedThreads) t[q] = s[q];
     return;
   }
-  double t[2 * K];
-  sum_parts<2 * K>(part2, nparts, 2 * K, t, lds);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int r = 0; r < K; ++r) beta_s[r] = cur->done[r] ? 0.0f : (float)(t[K + r] / cur->rz[r]);
-  __syncthreads();
-  const int lb = xcd_block(per);
-  const int n = lb * 256 + threadIdx.x;
-  if (n < nn_pad) {
-#pragma unroll
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (w < K && !cur->done[w]) {
+    double sz = 0.0, sr = 0.0;
     for (int d = 0; d < 3; ++d) {
-      const float kv = kinv[(size_t)n * 3 + d];
-      const size_t o = ((size_t)n * 3 + d) * K;
-#pragma unroll
-      for (int r = 0; r < K; r += 4) {
-        const float4 rr = *reinterpret_cast<const float4 *>(R + o + r);
-        float4 p = *reinterpret_cast<const float4 *>(P + o + r);
-        p.x = kv * rr.x + beta_s[r] * p.x;
-        p.y = kv * rr.y + beta_s[r + 1] * p.y;
-        p.z = kv * rr.z + beta_s[r + 2] * p.z;
-        p.w = kv * rr.w + beta_s[r + 3] * p.w;
-        *reinterpret_cast<float4 *>(P + o + r) = p;
+      const size_t row = (size_t)(d * K + w) * nbx;
+      for (int b = lane; b < nbx; b += 64) {
+        sz += pz[row + b];
+        sr += pr[row + b];
       }
     }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    int all = 1;
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-      const int was = cur->done[r];
-      const double pn = was ? cur->pn[r] : sqrt(t[r]);
-      const int dn = was || pn < cur->tau[r] * cur->pn0[r] || !(pn == pn);
-      nxt->rz[r] = was ? cur->rz[r] : t[K + r];
-      nxt->pn[r] = pn;
-      nxt->pn0[r] = cur->pn0[r];
-      nxt->tau[r] = cur->tau[r];
-      nxt->done[r] = dn;
-      all &= dn;
+    sz = wave_sum(sz);
+    sr = wave_sum(sr);
+    if (lane == 0) {
+      zz_s[w] = sz;
+      rz_s[w] = sr;
     }
-    const int its = cur->its + 1;
-    nxt->its = its;
-    nxt->max_its = cur->max_its;
-    nxt->all_done = all || its >= cur->max_its;
   }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  int all = 1;
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    const int was = cur->done[r];
+    const double pn = was ? cur->pn[r] : sqrt(zz_s[r]);
+    const int dn = was || pn < cur->tau[r] * cur->pn0[r] || !(pn == pn);
+    nxt->rz[r] = was ? cur->rz[r] : rz_s[r];
+    nxt->beta[r] = was ? 0.0 : rz_s[r] / cur->rz[r];
+    nxt->alpha[r] = cur->alpha[r];
+    nxt->pn[r] = pn;
+    nxt->pn0[r] = cur->pn0[r];
+    nxt->tau[r] = cur->tau[r];
+    nxt->done[r] = dn;
+    all &= dn;
+  }
+  const int its = cur->its + 1;
+  nxt->its = its;
+  nxt->max_its = cur->max_its;
+  nxt->all_done = all || its >= cur->max_its;
+}
+
+// step 5: P = kinv R + beta P for the fields still running
+template <int K>
+__global__ void __launch_bounds__(256) k_m_pupd(int NP, const float4 *R, const float4 *kinv, float4 *P,
+                                                const MState *nxt) {
+  const int c = blockIdx.y, f = c % K, d = c / K;
+  if (nxt->all_done || nxt->done[f]) return;
+  const float be = (float)nxt->beta[f];
+  const int q = blockIdx.x * 256 + threadIdx.x, nq = NP / 4;
+  if (q >= nq) return;
+  const size_t o = (size_t)c * nq + q;
+  const float4 r = R[o], kv = kinv[(size_t)d * nq + q];
+  float4 p = P[o];
+  p.x = kv.x * r.x + be * p.x;
+  p.y = kv.y * r.y + be * p.y;
+  p.z = kv.z * r.z + be * p.z;
+  p.w = kv.w * r.w + be * p.w;
+  P[o] = p;
 }
 
 // U += X (FP64 update of the Newton / defect-correction step)
@@ -504,8 +503,8 @@ __global__ void __launch_bounds__(256) k_m_accum(size_t n, const float *X, doubl
 // partials [K][6], then a one-block sum.
 // ---------------------------------------------------------------------------
 template <int K>
-__global__ void __launch_bounds__(256) k_m_ave(Grid g, const double *bmat_g, const material_base *mats, const int *et,
-                                               const double *U, double *part) {
+__global__ void __launch_bounds__(256) k_m_ave(Grid g, int NP, const double *bmat_g, const material_base *mats,
+                                               const int *et, const double *U, double *part) {
   __shared__ double bm[8 * 144];
   __shared__ double lds[4 * 6];
   for (int t = threadIdx.x; t < 8 * 144; t += blockDim.x) bm[t] = bmat_g[t];
@@ -525,7 +524,7 @@ __global__ void __launch_bounds__(256) k_m_ave(Grid g, const double *bmat_g, con
       double ue[24];
       for (int m = 0; m < 8; ++m) {
         const int node = ((ez + kHexPos[m][2]) * g.ny + (ey + kHexPos[m][1])) * g.nx + ex + kHexPos[m][0];
-        for (int d = 0; d < 3; ++d) ue[m * 3 + d] = U[((size_t)node * 3 + d) * K + r];
+        for (int d = 0; d < 3; ++d) ue[m * 3 + d] = U[(size_t)(d * K + r) * NP + node];
       }
       const material_base m = mats[ph];
       for (int gp = 0; gp < 8; ++gp) {
@@ -560,32 +559,38 @@ __global__ void __launch_bounds__(256) k_m_ave_finish(int nparts, int nv, const 
 static unsigned nblk(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 struct MLaunch {
-  int nn_pad, per, nb_launch;
+  int NP;         // nodes rounded up to 256
+  int per, nb;    // node blocks of 256 per XCD, blocks launched (8 * per)
+  int nsl, pers;  // slices of 64 nodes, slices per XCD (8 * pers launched)
+  int nbx;        // blocks of 256 float4 per component
 };
 
 static MLaunch mshape(const Grid &g) {
   MLaunch s;
-  s.nn_pad = (int)round_up((size_t)g.nn, 256);
-  const int nb = s.nn_pad / 256;
-  s.per = (nb + 7) / 8;
-  s.nb_launch = s.per * 8;
+  s.NP = (int)round_up((size_t)g.nn, 256);
+  s.per = (s.NP / 256 + 7) / 8;
+  s.nb = s.per * 8;
+  s.nsl = s.NP / 64;
+  s.pers = (s.nsl + 7) / 8;
+  s.nbx = (int)nblk((size_t)s.NP / 4, 256);
   return s;
 }
 
-size_t mixed_matrix_floats(const Grid &g) { return (size_t)mshape(g).nn_pad * kNodeQuads * 4; }
-size_t mixed_vector_len(const Grid &g, int K) { return (size_t)mshape(g).nn_pad * 3 * K; }
+size_t mixed_matrix_floats(const Grid &g) { return (size_t)mshape(g).NP * kNodeQuads * 4; }
+size_t mixed_vector_len(const Grid &g, int K) { return (size_t)mshape(g).NP * 3 * K; }
 size_t mixed_partials(const Grid &g, int K) {
   const MLaunch s = mshape(g);
-  const size_t a = (size_t)s.nb_launch * 2 * K;
-  const size_t b = (size_t)nblk(g.nelem, 256) * 6 * K;
-  return std::max(a, b) + (size_t)s.nb_launch * K;
+  const size_t resid = (size_t)s.nb * 2 * K;
+  const size_t ave = (size_t)nblk(g.nelem, 256) * 6 * K;
+  const size_t iter = (size_t)K * s.nsl + 2 * (size_t)3 * K * s.nbx;
+  return std::max(std::max(resid, ave), iter);
 }
 size_t mixed_state_bytes() { return 2 * sizeof(MState) + sizeof(MOuter); }
 
 hipError_t launch_m_asm32(const Grid &g, const int *et, const double *ke, int phase_mode, float *A, float *kinv,
                           hipStream_t st) {
   const MLaunch s = mshape(g);
-  hipLaunchKernelGGL(k_m_asm32, dim3(s.nn_pad / 256), dim3(256), 0, st, g, s.nn_pad, et, ke, phase_mode, A, kinv);
+  hipLaunchKernelGGL(k_m_asm32, dim3(s.NP / 256), dim3(256), 0, st, g, s.NP, et, ke, phase_mode, A, kinv);
   return hipGetLastError();
 }
 
@@ -612,7 +617,8 @@ hipError_t launch_m_asm32(const Grid &g, const int *et, const double *ke, int ph
   }
 
 hipError_t launch_m_setbc(const Grid &g, int K, const double *eps, double *U, hipStream_t st) {
-  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_setbc<KK>, dim3(nblk(g.nn, 256)), dim3(256), 0, st, g, eps, U));
+  const MLaunch s = mshape(g);
+  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_setbc<KK>, dim3(nblk(g.nn, 256)), dim3(256), 0, st, g, s.NP, eps, U));
   return hipGetLastError();
 }
 
@@ -624,9 +630,8 @@ hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke,
   MState *ms = static_cast<MState *>(state);
   MOuter *mo = reinterpret_cast<MOuter *>(ms + 2);
   MPP_K_DISPATCH(K, {
-    hipLaunchKernelGGL(k_m_resid<KK>, dim3(s.nb_launch), dim3(256), 0, st, g, s.nn_pad, s.per, et, ke, U, kinv, R, X,
-                       P, part);
-    hipLaunchKernelGGL(k_m_outer<KK>, dim3(1), dim3(256), 0, st, s.nb_launch, part, first, rel, abs_tol, tau_min,
+    hipLaunchKernelGGL(k_m_resid<KK>, dim3(s.nb), dim3(256), 0, st, g, s.NP, s.per, et, ke, U, kinv, R, X, P, part);
+    hipLaunchKernelGGL(k_m_outer<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nb, part, first, rel, abs_tol, tau_min,
                        max_inner, mo, ms);
   });
   return hipGetLastError();
@@ -636,17 +641,22 @@ hipError_t launch_m_iter(const Grid &g, int K, const float *A, float *X, float *
                          const float *kinv, double *part, void *state, int parity, hipStream_t st) {
   const MLaunch s = mshape(g);
   MState *ms = static_cast<MState *>(state);
-  const MState *cur = ms + (parity & 1);
+  MState *cur = ms + (parity & 1);
   MState *nxt = ms + ((parity + 1) & 1);
-  double *part1 = part;                                // nb_launch * K
-  double *part2 = part + (size_t)s.nb_launch * K;      // nb_launch * 2K
+  double *part1 = part;                               // K x nsl
+  double *pz = part + (size_t)K * s.nsl;              // 3K x nbx
+  double *pr = pz + (size_t)3 * K * s.nbx;            // 3K x nbx
+  const dim3 vgrid(s.nbx, 3 * K);
   MPP_K_DISPATCH(K, {
-    hipLaunchKernelGGL(k_m_spmv<KK>, dim3(s.nb_launch), dim3(256), 0, st, g, s.nn_pad, s.per,
+    hipLaunchKernelGGL(k_m_spmv<KK>, dim3(8 * s.pers), dim3(16 * KK), 0, st, g, s.NP, s.nsl, s.pers,
                        reinterpret_cast<const float4 *>(A), P, AP, part1, cur);
-    hipLaunchKernelGGL(k_m_upd<KK>, dim3(s.nb_launch), dim3(256), 0, st, s.nn_pad, s.per, s.nb_launch, part1, P, AP,
-                       kinv, X, R, part2, cur);
-    hipLaunchKernelGGL(k_m_pupd<KK>, dim3(s.nb_launch), dim3(256), 0, st, s.nn_pad, s.per, s.nb_launch, part2, R,
-                       kinv, P, cur, nxt);
+    hipLaunchKernelGGL(k_m_alpha<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nsl, part1, cur);
+    hipLaunchKernelGGL(k_m_upd<KK>, vgrid, dim3(256), 0, st, s.NP, reinterpret_cast<const float4 *>(P),
+                       reinterpret_cast<const float4 *>(AP), reinterpret_cast<const float4 *>(kinv),
+                       reinterpret_cast<float4 *>(X), reinterpret_cast<float4 *>(R), pz, pr, cur);
+    hipLaunchKernelGGL(k_m_beta<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nbx, pz, pr, cur, nxt);
+    hipLaunchKernelGGL(k_m_pupd<KK>, vgrid, dim3(256), 0, st, s.NP, reinterpret_cast<const float4 *>(R),
+                       reinterpret_cast<const float4 *>(kinv), reinterpret_cast<float4 *>(P), nxt);
   });
   return hipGetLastError();
 }
@@ -655,7 +665,7 @@ hipError_t launch_m_spmv_only(const Grid &g, int K, const float *A, const float 
                               const void *state, hipStream_t st) {
   const MLaunch s = mshape(g);
   const MState *cur = static_cast<const MState *>(state);
-  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_spmv<KK>, dim3(s.nb_launch), dim3(256), 0, st, g, s.nn_pad, s.per,
+  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_spmv<KK>, dim3(8 * s.pers), dim3(16 * KK), 0, st, g, s.NP, s.nsl, s.pers,
                                        reinterpret_cast<const float4 *>(A), P, AP, part, cur));
   return hipGetLastError();
 }
@@ -668,8 +678,9 @@ hipError_t launch_m_accum(const Grid &g, int K, const float *X, double *U, hipSt
 
 hipError_t launch_m_ave(const Grid &g, int K, const double *bmat, const material_base *mats_d, const int *et,
                         const double *U, double *part, double *out, hipStream_t st) {
+  const MLaunch s = mshape(g);
   const unsigned nb = nblk(g.nelem, 256);
-  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_ave<KK>, dim3(nb), dim3(256), 0, st, g, bmat, mats_d, et, U, part));
+  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_ave<KK>, dim3(nb), dim3(256), 0, st, g, s.NP, bmat, mats_d, et, U, part));
   hipLaunchKernelGGL(k_m_ave_finish, dim3(1), dim3(128), 0, st, (int)nb, 6 * K, part, g.vol_tot, out);
   return hipGetLastError();
 }

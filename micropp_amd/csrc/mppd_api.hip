@@ -693,7 +693,10 @@ int mppd_msolver_get_u(mppd_msolver *m, int k, double *u_h) {
   std::vector<double> U(mixed_vector_len(g, m->K));
   MPP_HIP(hipMemcpyAsync(U.data(), m->U, U.size() * sizeof(double), hipMemcpyDeviceToHost, m->st));
   MPP_HIP(hipStreamSynchronize(m->st));
-  for (int i = 0; i < g.nrow; ++i) u_h[i] = U[(size_t)i * m->K + k];
+  // component-major: component d * K + k of node n at (d * K + k) * NP + n
+  const size_t np = U.size() / (3 * (size_t)m->K);
+  for (int n = 0; n < g.nn; ++n)
+    for (int d = 0; d < 3; ++d) u_h[(size_t)n * 3 + d] = U[((size_t)d * m->K + k) * np + n];
   return MPPD_OK;
 }
 
