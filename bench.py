@@ -118,6 +118,37 @@ def rel(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
 
 
+def pmc_traffic(summary):
+    """HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE runs whose summary (commands, copy-probe
+    scaling) is profiles/<summary>; None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", summary)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("| total traffic"):
+                mb = float(line.split("**")[1].split()[0])
+                return mb * 1e6, "profiles/" + summary
+    return None, None
+
+
+COPY_PROBE_BYTES = 1 << 30
+
+
+def copy_probe():
+    """Three copies of a 1 GiB buffer: each reads and writes COPY_PROBE_BYTES, the yardstick for the
+    FETCH_SIZE / WRITE_SIZE counters of the solver kernels."""
+    import torch
+
+    a = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,11 +157,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--profile-iters", type=int, default=100)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of COPY_PROBE_BYTES (scales HBM counter readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); RVEs are independent, so a "
               "batch would shard across GPUs with no exchange (see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import micropp_amd as mp
 
@@ -149,6 +185,7 @@ def main():
     nodes = SIZE ** 3
     spmv_bytes = BYTES_PER_NODE_SPMV32 * nodes
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    traffic, traffic_src = pmc_traffic(f"k_m_spmv{K}_50cube_traffic.md")
 
     line = {
         "metric": "RVE solves/s (sigma+ctan) at 50^3, 1 GPU",
@@ -169,8 +206,8 @@ def main():
                    "rve_solves_per_step": N_STRAINS, "rel_tol": CG_TOL,
                    "outer_steps": stats[0], "inner_iterations": stats[1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_m_spmv<12> (FP32 node-block SpMV, 12 RHS, + p.Ap)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": f"k_m_spmv<{K}> (FP32 node-block SpMV, {K} RHS, + p.Ap)",
                      "bytes_per_launch": spmv_bytes, "avg_ms": spmv_ms, "inner_iteration_ms": iter_ms},
     }
 
@@ -188,10 +225,12 @@ def main():
         p_spmv_ms, p_iter_ms = mp.cg_profile(solver, b, args.profile_iters)
         p_bytes = BYTES_PER_ROW_SPMV64 * 3 * nodes
         p_ach = p_bytes / (p_spmv_ms * 1e-3) / 1e9
+        p_traffic, p_src = pmc_traffic("k_cg_step1_50cube_traffic.md")
         line["plain_fp64"] = {"value": N_STRAINS / dt_plain, "ms_per_step": 1e3 * dt_plain,
                               "cg_iterations_per_step": cg_its,
                               "roofline": {"bound": "hbm", "achieved": p_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                           "frac": p_ach / HBM_PEAK_GBS, "kernel": "k_cg_step1 (FP64 ELL SpMV + p.Ap)",
+                                           "frac": p_ach / HBM_PEAK_GBS, "traffic": p_traffic,
+                                           "traffic_source": p_src, "kernel": "k_cg_step1 (FP64 ELL SpMV + p.Ap)",
                                            "bytes_per_launch": p_bytes, "avg_ms": p_spmv_ms,
                                            "cg_iteration_ms": p_iter_ms}}
         line["speedup_vs_plain_fp64"] = value / (N_STRAINS / dt_plain)

@@ -160,6 +160,51 @@ int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *it
  * events on the solver's stream.  Averages in milliseconds. */
 int mppd_cg_profile(mppd_solver *s, const double *b_d, int iters, double *spmv_ms, double *iter_ms);
 
+/* ---- batches of RVEs (replaces the per-GP loop of homogenize,
+ * src/homogenize.cpp:68-99 with homogenize_fe_one_way / homogenize_fe_full,
+ * :111-282, for many GPs at once; the reference runs one geometry per
+ * micropp<3> instance and its GPs one after the other) ----
+ * A batch holds nrve RVEs, each with its own size, geometry, materials,
+ * coupling, strain, displacements and history: one per macro Gauss point.
+ * Every step of every RVE runs in one launch per finite-element operation
+ * over the whole batch; each RVE follows the reference's per-GP algorithm
+ * (Newton with the exit quirk, subiterations, sigma from calc_ave_stress or
+ * ctan.eps, calc_vars_new with allocation on the first nonlinearity, the
+ * FE_FULL perturbation loop reusing u, the stale cost addition), so its
+ * results equal a micropp<3> instance of that one RVE. */
+typedef struct mppd_batch mppd_batch;
+enum { MPPD_FE_LINEAR = 0, MPPD_FE_ONE_WAY = 1, MPPD_FE_FULL = 2 }; /* = FE_LINEAR, FE_ONE_WAY, FE_FULL */
+typedef struct {
+  int size[3];         /* nodes per side (micropp_params_t::size) */
+  int micro_type;      /* MIC_* (include/micropp.hpp) */
+  double geo_params[4];
+  struct material_base materials[3];
+  int coupling;        /* MPPD_FE_LINEAR, MPPD_FE_ONE_WAY or MPPD_FE_FULL */
+} mppd_batch_rve;
+typedef struct {
+  mppd_newton_params newton; /* its_with_A0 unused */
+  int lin_stress;            /* sigma = ctan . eps (micropp_params_t::lin_stress) */
+  int calc_ctan_lin;         /* initial ctan from the six linear solves */
+  int subiterations;         /* subiterate a non-converged Newton */
+  int nsubiterations;
+  int graph_chunk;           /* CG iterations per captured graph; 0 = choose by timing */
+} mppd_batch_params;
+/* the micropp_params_t defaults (include/types.hpp:43-83) with graph_chunk 0 */
+void mppd_batch_params_default(mppd_batch_params *p);
+int mppd_batch_create(mppd_batch **out, int nrve, const mppd_batch_rve *rves, const mppd_batch_params *p);
+int mppd_batch_destroy(mppd_batch *b);
+/* eps: nrve x 6 (set_strain of every RVE) */
+int mppd_batch_set_strains(mppd_batch *b, const double *eps);
+int mppd_batch_homogenize(mppd_batch *b);
+/* sig: nrve x 6; ctan: nrve x 36 (row-major, ctan[v * 6 + i]) */
+int mppd_batch_get_stress(const mppd_batch *b, double *sig);
+int mppd_batch_get_ctan(const mppd_batch *b, double *ctan);
+/* info: nrve x 4 = {is_non_linear (history allocated), cost, converged, subiterated} */
+int mppd_batch_get_info(const mppd_batch *b, int *info);
+int mppd_batch_update_vars(mppd_batch *b);
+/* stats: {CG iterations queued, graph chunk length, graph launches, device ms of the last homogenize} */
+int mppd_batch_stats(const mppd_batch *b, double stats[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,7 +140,31 @@ SIGNATURES = {
     "mppd_msolver_solve": (_I, [_VP, _I, _PD, C.c_void_p, _PD, _PI, _PD]),
     "mppd_msolver_get_u": (_I, [_VP, _I, _PD]),
     "mppd_msolver_profile": (_I, [_VP, _I, _PD, _PD]),
+    "mppd_batch_params_default": (None, [_VP]),
+    "mppd_batch_create": (_I, [C.POINTER(C.c_void_p), _I, _VP, _VP]),
+    "mppd_batch_destroy": (_I, [_VP]),
+    "mppd_batch_set_strains": (_I, [_VP, _PD]),
+    "mppd_batch_homogenize": (_I, [_VP]),
+    "mppd_batch_get_stress": (_I, [_VP, _PD]),
+    "mppd_batch_get_ctan": (_I, [_VP, _PD]),
+    "mppd_batch_get_info": (_I, [_VP, _PI]),
+    "mppd_batch_update_vars": (_I, [_VP]),
+    "mppd_batch_stats": (_I, [_VP, _PD]),
 }
+
+
+class BatchRve(C.Structure):
+    """``mppd_batch_rve`` (include/mppd.h)."""
+
+    _fields_ = [("size", C.c_int * 3), ("micro_type", C.c_int), ("geo_params", C.c_double * 4),
+                ("materials", MaterialBase * 3), ("coupling", C.c_int)]
+
+
+class BatchParams(C.Structure):
+    """``mppd_batch_params`` (include/mppd.h)."""
+
+    _fields_ = [("newton", NewtonParams), ("lin_stress", C.c_int), ("calc_ctan_lin", C.c_int),
+                ("subiterations", C.c_int), ("nsubiterations", C.c_int), ("graph_chunk", C.c_int)]
 
 
 class MixedParams(C.Structure):
@@ -529,6 +553,81 @@ def homogenize_mixed(msolver: MSolver, strains, d_eps=1e-8, **kw):
     sig, stats, rel, ok = msolver.solve(eps, **kw)
     ctan = sig[:6].T / d_eps
     return sig[6:], ctan, stats, rel, ok
+
+
+class Batch:
+    """mppd_batch: many RVEs (own size, geometry, materials, coupling, strain and history), one launch per
+    finite-element operation for all of them; each RVE follows micropp<3>'s per-GP algorithm."""
+
+    def __init__(self, rves, **params):
+        """rves: list of dicts with size, micro_type, geo, materials (3), coupling (FE_* id).
+        params: newton fields (nr_max_its, nr_max_tol, nr_rel_tol, cg_max_its, cg_abs_tol, cg_rel_tol),
+        lin_stress, calc_ctan_lin, subiterations, nsubiterations, graph_chunk."""
+        p = BatchParams()
+        load().mppd_batch_params_default(C.byref(p))
+        for k, v in params.items():
+            if hasattr(p.newton, k) and k != "its_with_A0":
+                setattr(p.newton, k, v)
+            elif k in dict(BatchParams._fields_):
+                setattr(p, k, int(v))
+            else:
+                raise TypeError(f"unknown batch parameter {k}")
+        arr = (BatchRve * len(rves))()
+        for i, d in enumerate(rves):
+            s = d["size"]
+            s = (s,) * 3 if np.isscalar(s) else tuple(s)
+            arr[i].size[:] = [int(x) for x in s]
+            arr[i].micro_type = int(d["micro_type"])
+            arr[i].geo_params[:] = [float(x) for x in d["geo"]]
+            for k in range(3):
+                arr[i].materials[k] = d["materials"][k]
+            arr[i].coupling = int(d.get("coupling", FE_ONE_WAY))
+        h = C.c_void_p()
+        _chk(load().mppd_batch_create(C.byref(h), len(rves), C.cast(arr, C.c_void_p), C.cast(C.byref(p),
+                                                                                             C.c_void_p)),
+             "mppd_batch_create")
+        self.h = h
+        self.n = len(rves)
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().mppd_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def set_strains(self, eps):
+        e = np.ascontiguousarray(eps, dtype=np.float64).reshape(self.n, 6)
+        _chk(load().mppd_batch_set_strains(self.h, _dp(e)), "mppd_batch_set_strains")
+
+    def homogenize(self):
+        _chk(load().mppd_batch_homogenize(self.h), "mppd_batch_homogenize")
+
+    def stress(self):
+        out = np.zeros((self.n, 6))
+        _chk(load().mppd_batch_get_stress(self.h, _dp(out)), "mppd_batch_get_stress")
+        return out
+
+    def ctan(self):
+        out = np.zeros((self.n, 6, 6))
+        _chk(load().mppd_batch_get_ctan(self.h, _dp(out)), "mppd_batch_get_ctan")
+        return out
+
+    def info(self):
+        """n x 4: is_non_linear, cost, converged, subiterated."""
+        out = np.zeros((self.n, 4), dtype=np.int32)
+        _chk(load().mppd_batch_get_info(self.h, _ip(out)), "mppd_batch_get_info")
+        return out
+
+    def update_vars(self):
+        _chk(load().mppd_batch_update_vars(self.h), "mppd_batch_update_vars")
+
+    def stats(self):
+        """CG iterations queued, graph chunk length, graph launches, ms of the last homogenize."""
+        out = np.zeros(4)
+        _chk(load().mppd_batch_stats(self.h, _dp(out)), "mppd_batch_stats")
+        return out
 
 
 def cg_profile(solver: "Solver", b: DeviceArray, iters: int):

@@ -1,0 +1,237 @@
+// batch.hip -- many RVEs of uneven size, geometry and material state per
+// launch (gfx950).
+//
+// A batch holds B RVEs, one per macro Gauss point.  Every finite-element
+// operation of the Newton / CG path is ONE launch over the whole batch: the
+// launch's blocks are the concatenation of each RVE's blocks, and a
+// block -> RVE table (one int per block) tells a block which RVE it works on
+// and its index inside that RVE.  The per-RVE arithmetic is the single-RVE
+// kernels' (kernels.hpp bodies, same block shapes, same fixed reduction
+// order), so a batched RVE's results are bit-identical to solving it alone.
+// RVEs switched off in the per-RVE activity mask return at once; converged
+// CGs stop through their own device-side flag.
+//
+// Launch shapes (blocks per RVE):
+//   row   nrow_pad / 256       rows / nodes / CG vectors / SpMV
+//   elem  ceil(nelem / 256)    one thread per element
+//   ke    ceil(items / 4)      one wave per element stiffness (phase mode: 3)
+//   rve   1                    per-RVE finishing reductions
+#include "kernels.hpp"
+#include "batch.hpp"
+
+namespace mpp {
+
+#define MPP_BATCH_PROLOGUE(MAP)                 \
+  const int rid = MAP[blockIdx.x];              \
+  if (!act[rid]) return;                        \
+  const BRve &R = rv[rid];
+
+// ---------------- boundary values and vector moves ----------------
+__global__ void __launch_bounds__(256) kb_set_bc(const BRve *rv, const int *map, const int *act, const int *blk0,
+                                                 const double *eps) {
+  MPP_BATCH_PROLOGUE(map)
+  const int lb = blockIdx.x - blk0[rid];
+  set_bc_node(R.g, lb * 256 + threadIdx.x, eps + (size_t)rid * 6, R.uw);
+}
+
+// which: 0 uw <- u_n, 1 u_k <- uw, 2 uw <- u_k, 3 uw <- 0
+__global__ void __launch_bounds__(256) kb_move(const BRve *rv, const int *map, const int *act, const int *blk0,
+                                               int which) {
+  MPP_BATCH_PROLOGUE(map)
+  const int row = (blockIdx.x - blk0[rid]) * 256 + threadIdx.x;
+  if (row >= R.g.nrow) return;
+  switch (which) {
+    case 0: R.uw[row] = R.u_n[row]; break;
+    case 1: R.u_k[row] = R.uw[row]; break;
+    case 2: R.uw[row] = R.u_k[row]; break;
+    default: R.uw[row] = 0.0; break;
+  }
+}
+
+// history moves: what 0: vars_n <- 0 and vars_k <- vars_aux (first
+// nonlinearity); what 1: vars_aux <- 0
+__global__ void __launch_bounds__(256) kb_vars_move(const BRve *rv, const int *map, const int *act, const int *blk0,
+                                                    int what) {
+  MPP_BATCH_PROLOGUE(map)
+  const size_t n = (size_t)R.g.nelem * 8 * NVAR;
+  const size_t t0 = (size_t)(blockIdx.x - blk0[rid]) * 256 + threadIdx.x;
+  const size_t stride = (size_t)R.nb_elem * 256;
+  for (size_t t = t0; t < n; t += stride) {
+    if (what == 0) {
+      R.vars_n[t] = 0.0;
+      R.vars_k[t] = R.vars_aux[t];
+    } else {
+      R.vars_aux[t] = 0.0;
+    }
+  }
+}
+
+// ---------------- assembly ----------------
+__global__ void __launch_bounds__(256) kb_elem_ke(const BRve *rv, const int *map, const int *act, const int *blk0) {
+  MPP_BATCH_PROLOGUE(map)
+  elem_ke_body(R.g, blockIdx.x - blk0[rid], R.bmat, R.mats, R.et, R.uw, R.use_vars ? R.vars_n : nullptr,
+               R.phase_mode, R.ke);
+}
+
+__global__ void __launch_bounds__(256) kb_gather_vals(const BRve *rv, const int *map, const int *act,
+                                                      const int *blk0) {
+  MPP_BATCH_PROLOGUE(map)
+  gather_vals_row(R.g, (blockIdx.x - blk0[rid]) * 256 + threadIdx.x, R.et, R.ke, R.phase_mode, R.vals);
+}
+
+__global__ void __launch_bounds__(256) kb_elem_be(const BRve *rv, const int *map, const int *act, const int *blk0) {
+  MPP_BATCH_PROLOGUE(map)
+  elem_be_body(R.g, blockIdx.x - blk0[rid], R.bmat, R.mats, R.et, R.uw, R.use_vars ? R.vars_n : nullptr, R.be);
+}
+
+__global__ void __launch_bounds__(256) kb_gather_rhs(const BRve *rv, const int *map, const int *act,
+                                                     const int *blk0) {
+  MPP_BATCH_PROLOGUE(map)
+  gather_rhs_body(R.g, blockIdx.x - blk0[rid], R.be, R.b, R.part);
+}
+
+// one block per RVE: ||b||
+__global__ void __launch_bounds__(256) kb_norm_finish(const BRve *rv, const int *act, double *out) {
+  const int rid = blockIdx.x;
+  if (!act[rid]) return;
+  const BRve &R = rv[rid];
+  double t[1];
+  seg_sum<1>(R.part, R.nb_row, t);
+  if (threadIdx.x == 0) out[rid] = sqrt(t[0]);
+}
+
+// u = uw (which 0) or u_k (which 1); vars as the RVE's use_vars says
+__global__ void __launch_bounds__(256) kb_ave_stress(const BRve *rv, const int *map, const int *act, const int *blk0,
+                                                     int which) {
+  MPP_BATCH_PROLOGUE(map)
+  ave_stress_body(R.g, blockIdx.x - blk0[rid], R.bmat, R.mats, R.et, which ? R.u_k : R.uw,
+                  R.use_vars ? R.vars_n : nullptr, R.part);
+}
+
+__global__ void __launch_bounds__(256) kb_ave_finish(const BRve *rv, const int *act, double *out) {
+  const int rid = blockIdx.x;
+  if (!act[rid]) return;
+  const BRve &R = rv[rid];
+  double t[6];
+  seg_sum<6>(R.part, R.nb_elem, t);
+  if (threadIdx.x == 0)
+    for (int v = 0; v < 6; ++v) out[(size_t)rid * 6 + v] = t[v] / R.g.vol_tot;
+}
+
+// calc_vars_new on u_k: vars_old = vars_n if the RVE has history, else none;
+// vars_new = vars_k if it has history, else vars_aux.  flags[rid] |= nonlinear.
+__global__ void __launch_bounds__(256) kb_vars_new(const BRve *rv, const int *map, const int *act, const int *blk0,
+                                                   int *flags) {
+  MPP_BATCH_PROLOGUE(map)
+  vars_new_body(R.g, blockIdx.x - blk0[rid], R.bmat, R.mats, R.et, R.u_k, R.use_vars ? R.vars_n : nullptr,
+                R.use_vars ? R.vars_k : R.vars_aux, flags + rid);
+}
+
+__global__ void kb_zero_ints(int n, int *p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ---------------- CG ----------------
+__global__ void __launch_bounds__(256) kb_cg_init(const BRve *rv, const int *map, const int *act, const int *blk0) {
+  MPP_BATCH_PROLOGUE(map)
+  cg_init_body(R.g, blockIdx.x - blk0[rid], R.vals, R.b, R.x, R.r, R.z, R.p, R.kinv, R.part);
+}
+
+// one block per RVE; an inactive RVE's CG starts (and stays) done
+__global__ void __launch_bounds__(256) kb_cg_init_state(const BRve *rv, const int *act, double min_err,
+                                                        double rel_err, int max_its) {
+  const int rid = blockIdx.x;
+  const BRve &R = rv[rid];
+  if (!act[rid]) {
+    if (threadIdx.x == 0) {
+      CgState c{};
+      c.done = 1;
+      R.cg[0] = c;
+    }
+    return;
+  }
+  cg_init_state_body(R.part, R.nb_row, min_err, rel_err, max_its, R.cg);
+}
+
+__global__ void __launch_bounds__(256) kb_cg_step1(const BRve *rv, const int *map, const int *blk0, int parity) {
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  cg_step1_body(blockIdx.x - blk0[rid], R.cols, R.vals, R.p, R.Ap, R.part, R.cg + (parity & 1));
+}
+
+__global__ void __launch_bounds__(256) kb_cg_step2(const BRve *rv, const int *map, const int *blk0, int parity) {
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  cg_step2_body(blockIdx.x - blk0[rid], R.nb_row, R.x, R.r, R.z, R.p, R.Ap, R.kinv, R.part, R.part + R.nb_row,
+                R.cg + (parity & 1));
+}
+
+__global__ void __launch_bounds__(256) kb_cg_step3(const BRve *rv, const int *map, const int *blk0, int parity) {
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  cg_step3_body(blockIdx.x - blk0[rid], R.nb_row, R.z, R.p, R.part, R.part + R.nb_row, R.cg + (parity & 1),
+                R.cg + ((parity + 1) & 1));
+}
+
+// uw += x
+__global__ void __launch_bounds__(256) kb_axpy(const BRve *rv, const int *map, const int *act, const int *blk0) {
+#pragma clang fp contract(off)
+  MPP_BATCH_PROLOGUE(map)
+  const int row = (blockIdx.x - blk0[rid]) * 256 + threadIdx.x;
+  if (row < R.g.nrow) R.uw[row] += 1.0 * R.x[row];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_b_set_bc(const BLaunch &L, const double *eps, hipStream_t st) {
+  hipLaunchKernelGGL(kb_set_bc, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0, eps);
+  return hipGetLastError();
+}
+hipError_t launch_b_move(const BLaunch &L, int which, hipStream_t st) {
+  hipLaunchKernelGGL(kb_move, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0, which);
+  return hipGetLastError();
+}
+hipError_t launch_b_vars_move(const BLaunch &L, int what, hipStream_t st) {
+  hipLaunchKernelGGL(kb_vars_move, dim3(L.elem.nb), dim3(256), 0, st, L.rv, L.elem.map, L.act, L.elem.blk0, what);
+  return hipGetLastError();
+}
+hipError_t launch_b_mat(const BLaunch &L, const int *act_ke, const int *act_gather, hipStream_t st) {
+  hipLaunchKernelGGL(kb_elem_ke, dim3(L.ke.nb), dim3(64 * kKeWaves), 0, st, L.rv, L.ke.map, act_ke, L.ke.blk0);
+  hipLaunchKernelGGL(kb_gather_vals, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, act_gather, L.row.blk0);
+  return hipGetLastError();
+}
+hipError_t launch_b_rhs(const BLaunch &L, double *norms, hipStream_t st) {
+  hipLaunchKernelGGL(kb_elem_be, dim3(L.elem.nb), dim3(256), 0, st, L.rv, L.elem.map, L.act, L.elem.blk0);
+  hipLaunchKernelGGL(kb_gather_rhs, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0);
+  hipLaunchKernelGGL(kb_norm_finish, dim3(L.nrve), dim3(256), 0, st, L.rv, L.act, norms);
+  return hipGetLastError();
+}
+hipError_t launch_b_ave(const BLaunch &L, int which, double *sig, hipStream_t st) {
+  hipLaunchKernelGGL(kb_ave_stress, dim3(L.elem.nb), dim3(256), 0, st, L.rv, L.elem.map, L.act, L.elem.blk0, which);
+  hipLaunchKernelGGL(kb_ave_finish, dim3(L.nrve), dim3(256), 0, st, L.rv, L.act, sig);
+  return hipGetLastError();
+}
+hipError_t launch_b_vars_new(const BLaunch &L, int *flags, hipStream_t st) {
+  hipLaunchKernelGGL(kb_zero_ints, dim3((L.nrve + 255) / 256), dim3(256), 0, st, L.nrve, flags);
+  hipLaunchKernelGGL(kb_vars_new, dim3(L.elem.nb), dim3(256), 0, st, L.rv, L.elem.map, L.act, L.elem.blk0, flags);
+  return hipGetLastError();
+}
+hipError_t launch_b_cg_init(const BLaunch &L, double min_err, double rel_err, int max_its, hipStream_t st) {
+  hipLaunchKernelGGL(kb_cg_init, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0);
+  hipLaunchKernelGGL(kb_cg_init_state, dim3(L.nrve), dim3(256), 0, st, L.rv, L.act, min_err, rel_err, max_its);
+  return hipGetLastError();
+}
+hipError_t launch_b_cg_iter(const BLaunch &L, int parity, hipStream_t st) {
+  hipLaunchKernelGGL(kb_cg_step1, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
+  hipLaunchKernelGGL(kb_cg_step2, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
+  hipLaunchKernelGGL(kb_cg_step3, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
+  return hipGetLastError();
+}
+hipError_t launch_b_axpy(const BLaunch &L, hipStream_t st) {
+  hipLaunchKernelGGL(kb_axpy, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0);
+  return hipGetLastError();
+}
+
+}  // namespace mpp

@@ -1,0 +1,747 @@
+// batch_api.hip -- the mppd_batch_* C ABI: host control flow of a batch of
+// RVEs over the batched kernels of batch.hip.
+//
+// The host keeps, per RVE, the reference's per-GP bookkeeping (strain,
+// stress, ctan, cost, flags) and drives the Newton / CG control flow of all
+// RVEs together: per Newton step it reads one norm per RVE; per CG chunk it
+// reads the CG states.  Everything else stays on the device.  The CG
+// iterations of a chunk are replayed from a captured hipGraph.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "batch.hpp"
+#include "kernels.hpp"
+
+using namespace mpp;
+
+namespace {
+
+constexpr double kDEpsCtanAve = 1e-8;  // D_EPS_CTAN_AVE (include/params.hpp:32)
+
+struct BState {
+  double strain[6] = {0}, strain_old[6] = {0}, stress[6] = {0}, ctan[36] = {0};
+  int coupling = MPPD_FE_ONE_WAY;
+  int nonlinear_mat = 0;  // some phase is not elastic: history arrays exist
+  int allocated = 0;      // history in use (gp_t::allocated)
+  int converged = 1, subiterated = 0;
+  long cost = 0;
+  int vals_ready = 0;     // phase mode: vals assembled once
+};
+
+enum { M_MAIN = 0, M_KE = 1, M_GATHER = 2, M_NUM = 3 };
+
+}  // namespace
+
+struct mppd_batch {
+  int n = 0;
+  mppd_batch_params prm{};
+  hipStream_t st = nullptr;
+  std::vector<mppd_rve *> rves;
+  std::vector<BRve> h;
+  std::vector<BState> s;
+  BRve *rv = nullptr;
+  // launch shapes
+  int *map_d[3] = {nullptr, nullptr, nullptr}, *blk0_d[3] = {nullptr, nullptr, nullptr};
+  int nb[3] = {0, 0, 0};
+  // arenas
+  double *vec = nullptr, *vals = nullptr, *ke = nullptr, *be = nullptr, *vars = nullptr, *part = nullptr;
+  CgState *cg = nullptr;
+  std::map<std::tuple<int, int, int>, int *> cols;
+  // per-call device arrays and pinned staging
+  int *mask_d[M_NUM] = {nullptr, nullptr, nullptr};
+  int *mask_h[M_NUM] = {nullptr, nullptr, nullptr};
+  int *flags_d = nullptr, *flags_h = nullptr;
+  double *eps_d = nullptr, *eps_h = nullptr, *norms_d = nullptr, *norms_h = nullptr, *sig_d = nullptr,
+         *sig_h = nullptr;
+  CgState *cg_h = nullptr;
+  // graph of gchunk CG iterations
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int gchunk = 0;
+  double cg_queued = 0, graph_launches = 0, last_ms = 0;
+
+  BLaunch launch(int mask) const {
+    BLaunch L;
+    L.nrve = n;
+    L.rv = rv;
+    L.act = mask_d[mask];
+    L.row = {nb[0], map_d[0], blk0_d[0]};
+    L.elem = {nb[1], map_d[1], blk0_d[1]};
+    L.ke = {nb[2], map_d[2], blk0_d[2]};
+    return L;
+  }
+};
+
+namespace {
+
+int sync(mppd_batch *B) {
+  MPP_HIP(hipStreamSynchronize(B->st));
+  return MPPD_OK;
+}
+
+// mask upload: the pinned staging buffer is rewritten only after the stream
+// has drained (earlier copies from it have completed)
+int set_mask(mppd_batch *B, int k, const std::vector<int> &on) {
+  int rc = sync(B);
+  if (rc) return rc;
+  for (int r = 0; r < B->n; ++r) B->mask_h[k][r] = on[r];
+  MPP_HIP(hipMemcpyAsync(B->mask_d[k], B->mask_h[k], sizeof(int) * B->n, hipMemcpyHostToDevice, B->st));
+  return MPPD_OK;
+}
+
+int upload_table(mppd_batch *B) {
+  int rc = sync(B);
+  if (rc) return rc;
+  MPP_HIP(hipMemcpy(B->rv, B->h.data(), sizeof(BRve) * B->n, hipMemcpyHostToDevice));
+  return MPPD_OK;
+}
+
+// eps of every RVE: row r of e (6 per RVE)
+int upload_eps(mppd_batch *B, const std::vector<double> &e) {
+  int rc = sync(B);
+  if (rc) return rc;
+  std::memcpy(B->eps_h, e.data(), sizeof(double) * 6 * B->n);
+  MPP_HIP(hipMemcpyAsync(B->eps_d, B->eps_h, sizeof(double) * 6 * B->n, hipMemcpyHostToDevice, B->st));
+  return MPPD_OK;
+}
+
+int read_norms(mppd_batch *B) {
+  MPP_HIP(hipMemcpyAsync(B->norms_h, B->norms_d, sizeof(double) * B->n, hipMemcpyDeviceToHost, B->st));
+  return sync(B);
+}
+
+int read_sig(mppd_batch *B) {
+  MPP_HIP(hipMemcpyAsync(B->sig_h, B->sig_d, sizeof(double) * 6 * B->n, hipMemcpyDeviceToHost, B->st));
+  return sync(B);
+}
+
+int read_cg(mppd_batch *B) {
+  MPP_HIP(hipMemcpyAsync(B->cg_h, B->cg, sizeof(CgState) * 2 * B->n, hipMemcpyDeviceToHost, B->st));
+  return sync(B);
+}
+
+// g iterations queued on the stream (parities 0 .. g-1)
+hipError_t queue_iters(mppd_batch *B, int g) {
+  const BLaunch L = B->launch(M_MAIN);
+  for (int i = 0; i < g; ++i) {
+    hipError_t e = launch_b_cg_iter(L, i, B->st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+int capture_graph(mppd_batch *B, int g) {
+  if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->gexec = nullptr;
+  B->graph = nullptr;
+  int rc = sync(B);
+  if (rc) return rc;
+  MPP_HIP(hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal));
+  hipError_t e = queue_iters(B, g);
+  hipGraph_t graph = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(B->st, &graph);
+  if (e != hipSuccess) return hip_fail(e, "batch graph capture");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+  B->graph = graph;
+  MPP_HIP(hipGraphInstantiate(&B->gexec, graph, nullptr, nullptr, 0));
+  B->gchunk = g;
+  return MPPD_OK;
+}
+
+// ell_solve_cgpd (src/ell.cpp:66-122) of every RVE in `on`, queued in chunks
+// of whole graphs sized from the slowest RVE's observed convergence rate.
+int cg_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &its) {
+  const mppd_newton_params &p = B->prm.newton;
+  int rc = set_mask(B, M_MAIN, on);
+  if (rc) return rc;
+  MPP_HIP(launch_b_cg_init(B->launch(M_MAIN), p.cg_abs_tol, p.cg_rel_tol, p.cg_max_its, B->st));
+  if ((rc = read_cg(B))) return rc;
+  const int n = B->n, G = B->gchunk;
+  std::vector<double> pn_prev(n), target(n);
+  std::vector<int> its_prev(n, 0);
+  bool all = true;
+  for (int r = 0; r < n; ++r) {
+    const CgState &c = B->cg_h[2 * r];
+    pn_prev[r] = c.pn;
+    target[r] = std::max(c.min_err, c.pn0 * c.rel_err);
+    all &= c.done != 0;
+  }
+  int queued = 0, chunk = 2 * G;
+  while (!all && queued < p.cg_max_its) {
+    const int reps = std::max(1, (std::min(chunk, p.cg_max_its - queued) + G - 1) / G);
+    for (int i = 0; i < reps; ++i) MPP_HIP(hipGraphLaunch(B->gexec, B->st));
+    queued += reps * G;
+    B->cg_queued += reps * G;
+    B->graph_launches += reps;
+    if ((rc = read_cg(B))) return rc;
+    all = true;
+    int next = 0;
+    for (int r = 0; r < n; ++r) {
+      const CgState &c = B->cg_h[2 * r + (queued & 1)];
+      if (c.done) continue;
+      all = false;
+      int need = 2 * chunk;
+      const int dit = c.its - its_prev[r];
+      if (dit > 0 && c.pn > 0 && pn_prev[r] > 0 && c.pn < pn_prev[r]) {
+        const double rate = std::log(c.pn / pn_prev[r]) / dit;
+        const double est = std::log(target[r] / c.pn) / rate;
+        if (std::isfinite(est)) need = (int)std::ceil(est) + 2;
+      }
+      next = std::max(next, need);
+      pn_prev[r] = c.pn;
+      its_prev[r] = c.its;
+    }
+    chunk = std::max(G, std::min(256, next));
+  }
+  for (int r = 0; r < n; ++r) its[r] = on[r] ? B->cg_h[2 * r + (queued & 1)].its : 0;
+  return MPPD_OK;
+}
+
+// element stiffness + vals of the RVEs in `on` (phase-mode RVEs: once)
+int mat_batch(mppd_batch *B, const std::vector<int> &on) {
+  std::vector<int> ke(B->n, 0), ga(B->n, 0);
+  for (int r = 0; r < B->n; ++r) {
+    if (!on[r]) continue;
+    const bool phase = B->h[r].phase_mode != 0;
+    ke[r] = !phase;
+    ga[r] = !phase || !B->s[r].vals_ready;
+    if (phase) B->s[r].vals_ready = 1;
+  }
+  int rc = set_mask(B, M_KE, ke);
+  if (!rc) rc = set_mask(B, M_GATHER, ga);
+  if (rc) return rc;
+  MPP_HIP(launch_b_mat(B->launch(M_MAIN), B->mask_d[M_KE], B->mask_d[M_GATHER], B->st));
+  return MPPD_OK;
+}
+
+// newton_raphson (src/solve.cpp:28-82) of every RVE in `on`, on uw with the
+// strains in eps_d: its / solver_its / converged per RVE.
+int newton_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &its, std::vector<int> &sits,
+                 std::vector<int> &conv) {
+  const mppd_newton_params &p = B->prm.newton;
+  const int n = B->n;
+  int rc = set_mask(B, M_MAIN, on);
+  if (rc) return rc;
+  const BLaunch L = B->launch(M_MAIN);
+  MPP_HIP(launch_b_set_bc(L, B->eps_d, B->st));
+  MPP_HIP(launch_b_rhs(L, B->norms_d, B->st));
+  if ((rc = read_norms(B))) return rc;
+  std::vector<double> norm0(B->norms_h, B->norms_h + n);
+  std::vector<int> run(on), cg_its(n, 0);
+  for (int r = 0; r < n; ++r) its[r] = sits[r] = conv[r] = 0;
+  for (;;) {
+    bool any = false;
+    for (int r = 0; r < n; ++r) {
+      if (!run[r]) continue;
+      const double norm = B->norms_h[r];
+      if (norm < p.nr_max_tol || norm < norm0[r] * p.nr_rel_tol) {
+        conv[r] = 1;
+        run[r] = 0;
+      } else if (its[r] >= p.nr_max_its) {
+        run[r] = 0;
+      } else {
+        any = true;
+      }
+    }
+    if (!any) break;
+    if ((rc = mat_batch(B, run))) return rc;
+    if ((rc = cg_batch(B, run, cg_its))) return rc;  // leaves the main mask = run
+    for (int r = 0; r < n; ++r) sits[r] += cg_its[r];
+    MPP_HIP(launch_b_axpy(L, B->st));
+    MPP_HIP(launch_b_rhs(L, B->norms_d, B->st));
+    if ((rc = read_norms(B))) return rc;
+    for (int r = 0; r < n; ++r) its[r] += run[r];
+  }
+  return MPPD_OK;
+}
+
+int move(mppd_batch *B, const std::vector<int> &on, int which) {
+  int rc = set_mask(B, M_MAIN, on);
+  if (rc) return rc;
+  MPP_HIP(launch_b_move(B->launch(M_MAIN), which, B->st));
+  return MPPD_OK;
+}
+
+int ave_batch(mppd_batch *B, const std::vector<int> &on, int which) {
+  int rc = set_mask(B, M_MAIN, on);
+  if (rc) return rc;
+  MPP_HIP(launch_b_ave(B->launch(M_MAIN), which, B->sig_d, B->st));
+  return read_sig(B);
+}
+
+std::vector<double> strains_of(const mppd_batch *B) {
+  std::vector<double> e(6 * B->n);
+  for (int r = 0; r < B->n; ++r)
+    for (int v = 0; v < 6; ++v) e[r * 6 + v] = B->s[r].strain[v];
+  return e;
+}
+
+// calc_ctan_lin_fe_models (src/micropp.cpp:255-284) for every RVE
+int ctan_lin_batch(mppd_batch *B) {
+  const int n = B->n;
+  std::vector<int> all(n, 1), its(n), sits(n), conv(n);
+  for (int i = 0; i < 6; ++i) {
+    int rc = move(B, all, 3);
+    if (rc) return rc;
+    std::vector<double> e(6 * n, 0.0);
+    for (int r = 0; r < n; ++r) e[r * 6 + i] += kDEpsCtanAve;
+    if ((rc = upload_eps(B, e))) return rc;
+    if ((rc = newton_batch(B, all, its, sits, conv))) return rc;
+    if ((rc = ave_batch(B, all, 0))) return rc;
+    for (int r = 0; r < n; ++r)
+      for (int v = 0; v < 6; ++v) B->s[r].ctan[v * 6 + i] = B->sig_h[r * 6 + v] / kDEpsCtanAve;
+  }
+  return MPPD_OK;
+}
+
+// CG time per iteration for a graph of g iterations (a converged CG state,
+// so every kernel runs its early exit: this measures launch overhead), ms
+int time_chunk(mppd_batch *B, int g, double *ms_per_it) {
+  int rc = capture_graph(B, g);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  MPP_HIP(hipEventCreate(&e0));
+  MPP_HIP(hipEventCreate(&e1));
+  const int reps = std::max(1, 64 / g);
+  hipError_t e = hipGraphLaunch(B->gexec, B->st);  // warm
+  if (e == hipSuccess) e = hipEventRecord(e0, B->st);
+  for (int i = 0; i < reps && e == hipSuccess; ++i) e = hipGraphLaunch(B->gexec, B->st);
+  if (e == hipSuccess) e = hipEventRecord(e1, B->st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return hip_fail(e, "batch graph timing");
+  *ms_per_it = ms / (reps * g);
+  return MPPD_OK;
+}
+
+// Graph chunk length: the shortest whose per-iteration launch cost is within
+// 10% of the longest candidate's (short chunks waste fewer iterations past
+// convergence, long ones amortise the graph launch).
+int autotune(mppd_batch *B) {
+  const int cand[] = {2, 4, 8, 16, 32};
+  double t[5];
+  for (int i = 0; i < 5; ++i) {
+    int rc = time_chunk(B, cand[i], &t[i]);
+    if (rc) return rc;
+  }
+  int pick = 32;
+  for (int i = 0; i < 5; ++i)
+    if (t[i] <= 1.1 * t[4]) {
+      pick = cand[i];
+      break;
+    }
+  return capture_graph(B, pick);
+}
+
+void destroy(mppd_batch *B) {
+  if (B->st) (void)hipStreamSynchronize(B->st);
+  if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  for (int k = 0; k < 3; ++k) {
+    dfree(B->map_d[k]);
+    dfree(B->blk0_d[k]);
+  }
+  for (void *p : {(void *)B->rv, (void *)B->vec, (void *)B->vals, (void *)B->ke, (void *)B->be, (void *)B->vars,
+                  (void *)B->part, (void *)B->cg, (void *)B->flags_d, (void *)B->eps_d, (void *)B->norms_d,
+                  (void *)B->sig_d})
+    dfree(p);
+  for (auto &kv : B->cols) dfree(kv.second);
+  for (int k = 0; k < M_NUM; ++k) {
+    dfree(B->mask_d[k]);
+    if (B->mask_h[k]) (void)hipHostFree(B->mask_h[k]);
+  }
+  for (void *p : {(void *)B->flags_h, (void *)B->eps_h, (void *)B->norms_h, (void *)B->sig_h, (void *)B->cg_h})
+    if (p) (void)hipHostFree(p);
+  for (mppd_rve *r : B->rves) mppd_rve_destroy(r);
+  if (B->st) (void)hipStreamDestroy(B->st);
+  delete B;
+}
+
+template <typename T>
+int pinned(T **p, size_t n) {
+  MPP_HIP(hipHostMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocDefault));
+  std::memset(*p, 0, sizeof(T) * std::max<size_t>(n, 1));
+  return MPPD_OK;
+}
+
+int setup(mppd_batch *B, const mppd_batch_rve *in) {
+  const int n = B->n;
+  int rc = MPPD_OK;
+  MPP_HIP(hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking));
+  B->rves.assign(n, nullptr);
+  B->h.assign(n, BRve{});
+  B->s.assign(n, BState{});
+  for (int r = 0; r < n; ++r) {
+    const mppd_batch_rve &d = in[r];
+    if (d.coupling != MPPD_FE_LINEAR && d.coupling != MPPD_FE_ONE_WAY && d.coupling != MPPD_FE_FULL)
+      return fail(MPPD_ERR_ARG, "batch coupling must be FE_LINEAR, FE_ONE_WAY or FE_FULL");
+    if ((rc = mppd_rve_create(&B->rves[r], d.size, d.micro_type, d.geo_params, d.materials))) return rc;
+    B->s[r].coupling = d.coupling;
+    B->s[r].nonlinear_mat = !B->rves[r]->all_elastic;
+  }
+  // sizes, block tables and arena offsets
+  std::vector<int> map[3], blk0[3];
+  size_t rows = 0, slots = 0, kes = 0, bes = 0, nvars = 0, parts = 0;
+  for (int k = 0; k < 3; ++k) blk0[k].assign(n, 0);
+  for (int r = 0; r < n; ++r) {
+    const mppd_rve *v = B->rves[r];
+    const Grid &g = v->g;
+    BRve &R = B->h[r];
+    R.g = g;
+    R.phase_mode = v->all_elastic;
+    R.nb_row = g.nrow_pad / 256;
+    R.nb_elem = (g.nelem + 255) / 256;
+    const int nb_ke = ((R.phase_mode ? 3 : g.nelem) + kKeWaves - 1) / kKeWaves;
+    const int counts[3] = {R.nb_row, R.nb_elem, nb_ke};
+    for (int k = 0; k < 3; ++k) {
+      blk0[k][r] = (int)map[k].size();
+      map[k].insert(map[k].end(), counts[k], r);
+    }
+    rows += g.nrow_pad;
+    slots += (size_t)g.nrow_pad * kSlots;
+    kes += (size_t)(R.phase_mode ? 3 : g.nelem) * 576;
+    bes += (size_t)g.nelem * 24;
+    if (!v->all_elastic) nvars += (size_t)g.nelem * 8 * NVAR;
+    parts += std::max<size_t>(3 * (size_t)R.nb_row, 6 * (size_t)R.nb_elem);
+  }
+  for (int k = 0; k < 3; ++k) {
+    B->nb[k] = (int)map[k].size();
+    if ((rc = dalloc(&B->map_d[k], map[k].size(), false))) return rc;
+    if ((rc = dalloc(&B->blk0_d[k], n, false))) return rc;
+    MPP_HIP(hipMemcpy(B->map_d[k], map[k].data(), sizeof(int) * map[k].size(), hipMemcpyHostToDevice));
+    MPP_HIP(hipMemcpy(B->blk0_d[k], blk0[k].data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  }
+  constexpr int kRowVecs = 10;  // uw u_n u_k x r z p Ap kinv b
+  if ((rc = dalloc(&B->vec, rows * kRowVecs))) return rc;
+  if ((rc = dalloc(&B->vals, slots))) return rc;
+  if ((rc = dalloc(&B->ke, kes, false))) return rc;
+  if ((rc = dalloc(&B->be, bes, false))) return rc;
+  if ((rc = dalloc(&B->vars, nvars * 3))) return rc;
+  if ((rc = dalloc(&B->part, parts))) return rc;
+  if ((rc = dalloc(&B->cg, 2 * (size_t)n))) return rc;
+  if ((rc = dalloc(&B->rv, n))) return rc;
+  size_t o_row = 0, o_slot = 0, o_ke = 0, o_be = 0, o_var = 0, o_part = 0;
+  for (int r = 0; r < n; ++r) {
+    const mppd_rve *v = B->rves[r];
+    const Grid &g = v->g;
+    BRve &R = B->h[r];
+    R.bmat = v->bmat;
+    R.mats = v->mats_d;
+    R.et = v->et;
+    auto key = std::make_tuple(g.nx, g.ny, g.nz);
+    if (!B->cols.count(key)) {
+      int *c = nullptr;
+      if ((rc = dalloc(&c, (size_t)g.nrow_pad * kSlots, false))) return rc;
+      B->cols[key] = c;
+      MPP_HIP(launch_ell_cols(g, c, B->st));
+    }
+    R.cols = B->cols[key];
+    R.vals = B->vals + o_slot;
+    R.ke = B->ke + o_ke;
+    R.be = B->be + o_be;
+    double **vecs[kRowVecs] = {&R.uw, &R.u_n, &R.u_k, &R.x, &R.r, &R.z, &R.p, &R.Ap, &R.kinv, &R.b};
+    for (int q = 0; q < kRowVecs; ++q) *vecs[q] = B->vec + rows * q + o_row;
+    if (!v->all_elastic) {
+      const size_t nv = (size_t)g.nelem * 8 * NVAR;
+      R.vars_n = B->vars + o_var;
+      R.vars_k = B->vars + nvars + o_var;
+      R.vars_aux = B->vars + 2 * nvars + o_var;
+      o_var += nv;
+    }
+    R.part = B->part + o_part;
+    R.cg = B->cg + 2 * r;
+    R.use_vars = 0;
+    o_row += g.nrow_pad;
+    o_slot += (size_t)g.nrow_pad * kSlots;
+    o_ke += (size_t)(R.phase_mode ? 3 : g.nelem) * 576;
+    o_be += (size_t)g.nelem * 24;
+    o_part += std::max<size_t>(3 * (size_t)R.nb_row, 6 * (size_t)R.nb_elem);
+  }
+  for (int k = 0; k < M_NUM; ++k) {
+    if ((rc = dalloc(&B->mask_d[k], n))) return rc;
+    if ((rc = pinned(&B->mask_h[k], n))) return rc;
+  }
+  if ((rc = dalloc(&B->flags_d, n))) return rc;
+  if ((rc = dalloc(&B->eps_d, 6 * (size_t)n))) return rc;
+  if ((rc = dalloc(&B->norms_d, n))) return rc;
+  if ((rc = dalloc(&B->sig_d, 6 * (size_t)n))) return rc;
+  if ((rc = pinned(&B->flags_h, n))) return rc;
+  if ((rc = pinned(&B->eps_h, 6 * (size_t)n))) return rc;
+  if ((rc = pinned(&B->norms_h, n))) return rc;
+  if ((rc = pinned(&B->sig_h, 6 * (size_t)n))) return rc;
+  if ((rc = pinned(&B->cg_h, 2 * (size_t)n))) return rc;
+  if ((rc = upload_table(B))) return rc;
+  // element stiffness per phase of the all-elastic RVEs, once
+  std::vector<int> ph(n);
+  for (int r = 0; r < n; ++r) ph[r] = B->h[r].phase_mode;
+  if ((rc = set_mask(B, M_KE, ph))) return rc;
+  if ((rc = set_mask(B, M_GATHER, std::vector<int>(n, 0)))) return rc;
+  MPP_HIP(launch_b_mat(B->launch(M_MAIN), B->mask_d[M_KE], B->mask_d[M_GATHER], B->st));
+  // the CG graph (its kernels read the table and masks through fixed pointers)
+  if ((rc = set_mask(B, M_MAIN, std::vector<int>(n, 0)))) return rc;
+  MPP_HIP(launch_b_cg_init(B->launch(M_MAIN), 0.0, 0.0, 0, B->st));  // every CG done: timing runs exit early
+  if (B->prm.graph_chunk > 0)
+    rc = capture_graph(B, B->prm.graph_chunk);
+  else
+    rc = autotune(B);
+  if (rc) return rc;
+  if ((rc = sync(B))) return rc;
+  // micropp constructor: ctan of every GP from the linear solves
+  bool any_fe = false;
+  for (int r = 0; r < n; ++r) any_fe |= B->s[r].coupling != MPPD_FE_LINEAR;
+  if (B->prm.calc_ctan_lin && any_fe) rc = ctan_lin_batch(B);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mppd_batch_params_default(mppd_batch_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  // micropp_params_t defaults (include/types.hpp:43-83)
+  p->newton.nr_max_its = 4;
+  p->newton.nr_max_tol = 1.0e-10;
+  p->newton.nr_rel_tol = 1.0e-3;
+  p->newton.cg_max_its = 1000;
+  p->newton.cg_abs_tol = 1.0e-50;
+  p->newton.cg_rel_tol = 1.0e-5;
+  p->newton.its_with_A0 = 1;
+  p->lin_stress = 1;
+  p->calc_ctan_lin = 1;
+  p->subiterations = 0;
+  p->nsubiterations = 10;
+  p->graph_chunk = 0;
+}
+
+int mppd_batch_create(mppd_batch **out, int nrve, const mppd_batch_rve *rves, const mppd_batch_params *p) {
+  if (!out) return fail(MPPD_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (nrve < 1 || !rves || !p) return fail(MPPD_ERR_ARG, "bad batch arguments");
+  if (p->graph_chunk < 0 || p->graph_chunk > 256 || (p->graph_chunk & 1))
+    return fail(MPPD_ERR_ARG, "graph_chunk must be 0 or an even number <= 256");
+  if (p->subiterations && p->nsubiterations < 1) return fail(MPPD_ERR_ARG, "nsubiterations must be >= 1");
+  mppd_batch *B = new mppd_batch();
+  B->n = nrve;
+  B->prm = *p;
+  const int rc = setup(B, rves);
+  if (rc) {
+    destroy(B);
+    return rc;
+  }
+  *out = B;
+  return MPPD_OK;
+}
+
+int mppd_batch_destroy(mppd_batch *b) {
+  if (b) destroy(b);
+  return MPPD_OK;
+}
+
+int mppd_batch_set_strains(mppd_batch *b, const double *eps) {
+  if (!b || !eps) return fail(MPPD_ERR_ARG, "null argument");
+  for (int r = 0; r < b->n; ++r) std::memcpy(b->s[r].strain, eps + 6 * r, 6 * sizeof(double));
+  return MPPD_OK;
+}
+
+// homogenize (src/homogenize.cpp:68-99): FE_LINEAR -> ctan.eps; FE_ONE_WAY /
+// FE_FULL -> homogenize_fe_one_way / homogenize_fe_full (:111-282) for all
+// such RVEs together.
+int mppd_batch_homogenize(mppd_batch *B) {
+  if (!B) return fail(MPPD_ERR_ARG, "null batch");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int n = B->n;
+  int rc = MPPD_OK;
+  std::vector<int> fe(n, 0), its(n), sits(n), conv(n);
+  bool any = false;
+  for (int r = 0; r < n; ++r) {
+    BState &q = B->s[r];
+    if (q.coupling == MPPD_FE_LINEAR) {  // homogenize_linear
+      for (int i = 0; i < 6; ++i) {
+        q.stress[i] = 0.0;
+        for (int j = 0; j < 6; ++j) q.stress[i] += q.ctan[i * 6 + j] * q.strain[j];
+      }
+      continue;
+    }
+    fe[r] = 1;
+    any = true;
+  }
+  if (!any) return MPPD_OK;
+  // history in use as of this call
+  bool tab = false;
+  for (int r = 0; r < n; ++r)
+    if (B->h[r].use_vars != B->s[r].allocated) {
+      B->h[r].use_vars = B->s[r].allocated;
+      tab = true;
+    }
+  if (tab && (rc = upload_table(B))) return rc;
+  // u = u_n; newton(strain, vars_n); u_k = u
+  if ((rc = move(B, fe, 0))) return rc;
+  if ((rc = upload_eps(B, strains_of(B)))) return rc;
+  if ((rc = newton_batch(B, fe, its, sits, conv))) return rc;
+  if ((rc = move(B, fe, 1))) return rc;
+  std::vector<int> last_sits(sits);
+  for (int r = 0; r < n; ++r)
+    if (fe[r]) {
+      B->s[r].cost = sits[r];
+      B->s[r].converged = conv[r];
+      B->s[r].subiterated = 0;
+    }
+  // subiterations of the GPs whose Newton did not converge
+  if (B->prm.subiterations) {
+    std::vector<int> sub(n, 0);
+    bool anysub = false;
+    for (int r = 0; r < n; ++r)
+      if (fe[r] && !conv[r]) sub[r] = 1, anysub = true;
+    if (anysub) {
+      if ((rc = move(B, sub, 0))) return rc;
+      std::vector<double> e(6 * n, 0.0), de(6 * n, 0.0);
+      for (int r = 0; r < n; ++r)
+        if (sub[r]) {
+          B->s[r].subiterated = 1;
+          for (int i = 0; i < 6; ++i) {
+            e[r * 6 + i] = B->s[r].strain_old[i];
+            de[r * 6 + i] = (B->s[r].strain[i] - B->s[r].strain_old[i]) / B->prm.nsubiterations;
+          }
+        }
+      for (int k = 0; k < B->prm.nsubiterations; ++k) {
+        for (int q = 0; q < 6 * n; ++q) e[q] += de[q];
+        if ((rc = upload_eps(B, e))) return rc;
+        if ((rc = newton_batch(B, sub, its, sits, conv))) return rc;
+        for (int r = 0; r < n; ++r)
+          if (sub[r]) {
+            B->s[r].cost += sits[r];
+            last_sits[r] = sits[r];
+          }
+      }
+      for (int r = 0; r < n; ++r)
+        if (sub[r]) B->s[r].converged = conv[r];
+      if ((rc = move(B, sub, 1))) return rc;
+    }
+  }
+  // sigma
+  if (B->prm.lin_stress) {
+    for (int r = 0; r < n; ++r) {
+      if (!fe[r]) continue;
+      BState &q = B->s[r];
+      for (int i = 0; i < 6; ++i) {
+        q.stress[i] = 0.0;
+        for (int j = 0; j < 6; ++j) q.stress[i] += q.ctan[i * 6 + j] * q.strain[j];
+      }
+    }
+  } else {
+    if ((rc = ave_batch(B, fe, 1))) return rc;
+    for (int r = 0; r < n; ++r)
+      if (fe[r]) std::memcpy(B->s[r].stress, B->sig_h + 6 * r, 6 * sizeof(double));
+  }
+  // calc_vars_new; history allocated on the first nonlinearity
+  std::vector<int> nl_on(n, 0);
+  bool anynl = false;
+  for (int r = 0; r < n; ++r)
+    if (fe[r] && B->s[r].nonlinear_mat) nl_on[r] = 1, anynl = true;
+  if (anynl) {
+    // a GP without history evolutes into a zeroed scratch (vars_aux)
+    std::vector<int> fresh(n, 0);
+    for (int r = 0; r < n; ++r) fresh[r] = nl_on[r] && !B->s[r].allocated;
+    if ((rc = set_mask(B, M_MAIN, fresh))) return rc;
+    MPP_HIP(launch_b_vars_move(B->launch(M_MAIN), 1, B->st));
+    if ((rc = set_mask(B, M_MAIN, nl_on))) return rc;
+    MPP_HIP(launch_b_vars_new(B->launch(M_MAIN), B->flags_d, B->st));
+    MPP_HIP(hipMemcpyAsync(B->flags_h, B->flags_d, sizeof(int) * n, hipMemcpyDeviceToHost, B->st));
+    if ((rc = sync(B))) return rc;
+    std::vector<int> alloc(n, 0);
+    bool anyalloc = false;
+    for (int r = 0; r < n; ++r)
+      if (nl_on[r] && B->flags_h[r] && !B->s[r].allocated) {
+        alloc[r] = 1;
+        anyalloc = true;
+        B->s[r].allocated = 1;
+        B->h[r].use_vars = 1;
+      }
+    if (anyalloc) {
+      if ((rc = set_mask(B, M_MAIN, alloc))) return rc;
+      MPP_HIP(launch_b_vars_move(B->launch(M_MAIN), 0, B->st));
+      if ((rc = upload_table(B))) return rc;
+    }
+  }
+  // FE_FULL: ctan from six perturbed solves reusing u (GPs with history only)
+  std::vector<int> full(n, 0);
+  bool anyfull = false;
+  for (int r = 0; r < n; ++r)
+    if (fe[r] && B->s[r].coupling == MPPD_FE_FULL && B->s[r].allocated) full[r] = 1, anyfull = true;
+  if (anyfull) {
+    if ((rc = move(B, full, 2))) return rc;
+    std::vector<double> sig0(6 * n);
+    for (int r = 0; r < n; ++r) std::memcpy(&sig0[6 * r], B->s[r].stress, 6 * sizeof(double));
+    for (int i = 0; i < 6; ++i) {
+      std::vector<double> e = strains_of(B);
+      for (int r = 0; r < n; ++r) e[r * 6 + i] += kDEpsCtanAve;
+      if ((rc = upload_eps(B, e))) return rc;
+      if ((rc = newton_batch(B, full, its, sits, conv))) return rc;
+      if ((rc = ave_batch(B, full, 0))) return rc;
+      for (int r = 0; r < n; ++r) {
+        if (!full[r]) continue;
+        B->s[r].cost += last_sits[r];  // the reference adds the sigma solve's count again
+        for (int v = 0; v < 6; ++v) B->s[r].ctan[v * 6 + i] = (B->sig_h[r * 6 + v] - sig0[r * 6 + v]) / kDEpsCtanAve;
+      }
+    }
+  }
+  if ((rc = sync(B))) return rc;
+  B->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return MPPD_OK;
+}
+
+int mppd_batch_get_stress(const mppd_batch *b, double *sig) {
+  if (!b || !sig) return fail(MPPD_ERR_ARG, "null argument");
+  for (int r = 0; r < b->n; ++r) std::memcpy(sig + 6 * r, b->s[r].stress, 6 * sizeof(double));
+  return MPPD_OK;
+}
+
+int mppd_batch_get_ctan(const mppd_batch *b, double *ctan) {
+  if (!b || !ctan) return fail(MPPD_ERR_ARG, "null argument");
+  for (int r = 0; r < b->n; ++r) std::memcpy(ctan + 36 * r, b->s[r].ctan, 36 * sizeof(double));
+  return MPPD_OK;
+}
+
+int mppd_batch_get_info(const mppd_batch *b, int *info) {
+  if (!b || !info) return fail(MPPD_ERR_ARG, "null argument");
+  for (int r = 0; r < b->n; ++r) {
+    info[4 * r + 0] = b->s[r].allocated;
+    info[4 * r + 1] = (int)b->s[r].cost;
+    info[4 * r + 2] = b->s[r].converged;
+    info[4 * r + 3] = b->s[r].subiterated;
+  }
+  return MPPD_OK;
+}
+
+// gp_t::update_vars (include/gp.hpp:95-105) of every RVE
+int mppd_batch_update_vars(mppd_batch *b) {
+  if (!b) return fail(MPPD_ERR_ARG, "null batch");
+  for (int r = 0; r < b->n; ++r) {
+    std::swap(b->h[r].vars_n, b->h[r].vars_k);
+    std::swap(b->h[r].u_n, b->h[r].u_k);
+    std::memcpy(b->s[r].strain_old, b->s[r].strain, 6 * sizeof(double));
+  }
+  return upload_table(b);
+}
+
+int mppd_batch_stats(const mppd_batch *b, double stats[4]) {
+  if (!b || !stats) return fail(MPPD_ERR_ARG, "null argument");
+  stats[0] = b->cg_queued;
+  stats[1] = b->gchunk;
+  stats[2] = b->graph_launches;
+  stats[3] = b->last_ms;
+  return MPPD_OK;
+}
+
+}  // extern "C"

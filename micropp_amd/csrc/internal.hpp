@@ -63,6 +63,31 @@ int hip_fail(hipError_t e, const char *what);
     if (_e != hipSuccess) return mpp::hip_fail(_e, #call); \
   } while (0)
 
+// Device allocation of n elements, zero-filled unless asked otherwise.
+template <typename T>
+inline int dalloc(T **p, size_t n, bool zero = true) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  if (zero) {
+    e = hipMemset(*p, 0, n * sizeof(T));
+    // hipMemset runs on the null stream, which the solvers' non-blocking
+    // streams do not order against: finish it before the buffer is used
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "hipMemset");
+  }
+  return MPPD_OK;
+}
+
+inline void dfree(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+// material_base with the derived constants of the reference's constructors
+// (mppd_api.hip); MPPD_ERR_ARG for an unknown law.
+int make_material(const material_base &in, material_base &m);
+
 // ---------------- launchers (geometry.hip) ----------------
 hipError_t launch_bmat(const Grid &g, double *bmat, hipStream_t st);
 hipError_t launch_random_spheres(const double *geo, double *spheres, hipStream_t st);
@@ -80,7 +105,7 @@ hipError_t launch_material_eval(const material_base &m, int n, const double *eps
 hipError_t launch_set_displ_bc(const Grid &g, const double *eps6, double *u, hipStream_t st);
 // Element stiffness: per phase (phase_mode: one 24x24 block per material,
 // u/vars ignored) or per element.
-hipError_t launch_elem_ke(const Grid &g, const double *bmat, const material_base *mats, const int *et, const double *u,
+hipError_t launch_elem_ke(const Grid &g, const double *bmat, const material_base *mats_d, const int *et, const double *u,
                           const double *vars, int phase_mode, double *ke, hipStream_t st);
 hipError_t launch_gather_vals(const Grid &g, const int *et, const double *ke, int phase_mode, double *vals,
                               hipStream_t st);
@@ -128,7 +153,29 @@ hipError_t launch_m_spmv_only(const Grid &g, int K, const float *A, const float 
 hipError_t launch_m_accum(const Grid &g, int K, const float *X, double *U, hipStream_t st);
 hipError_t launch_m_ave(const Grid &g, int K, const double *bmat, const material_base *mats_d, const int *et,
                         const double *U, double *part, double *out, hipStream_t st);
+}  // namespace mpp
+
+// One RVE: geometry, materials, B, phase map (the opaque mppd_rve of mppd.h).
+struct mppd_rve {
+  mpp::Grid g;
+  int micro_type = 0;
+  double geo[4];
+  material_base mats[3];
+  material_base *mats_d = nullptr;  // the same three on the device
+  double *bmat = nullptr;
+  int *et = nullptr;
+  double *spheres = nullptr;
+  int nsph = 0;
+  double vf[2] = {0, 0};
+  int all_elastic = 1;
+};
+
+namespace mpp {
+
 void mixed_state_view(const void *state_h, int parity, int *its, int *all_done);
+// field k of the inner state: returns its done flag; *pn its preconditioned
+// residual norm, *target the norm at which it stops
+int mixed_state_field(const void *state_h, int parity, int k, double *pn, double *target);
 void mixed_outer_view(const void *state_h, int K, int *all_conv, double *pn, double *pn0);
 
 }  // namespace mpp

@@ -36,7 +36,7 @@ namespace mpp {
 
 constexpr int kMaxRhs = 16;
 constexpr int kNodeQuads = 61;  // 244 floats per node (243 coefficients + 1 pad)
-constexpr int kRedThreads = 1024;  // one-block reducers: one wave per row
+constexpr int kRedThreads = 128;  // threads of the one-block reducers
 
 struct MState {
   double rz[kMaxRhs], pn[kMaxRhs], pn0[kMaxRhs], tau[kMaxRhs], alpha[kMaxRhs], beta[kMaxRhs];
@@ -63,12 +63,39 @@ __device__ __forceinline__ int xcd_block(int per) {
   return (b & 7) * per + (b >> 3);
 }
 
-// Sum of one row of a partial table, by one wave, lanes striding the row in a
-// fixed order.
-__device__ __forceinline__ double wave_row_sum(const double *row, int n) {
-  double s = 0.0;
-  for (int b = threadIdx.x & 63; b < n; b += 64) s += row[b];
-  return wave_sum(s);
+// Row sums of a partial table part[row * stride + j], j < n, rows < NR
+// (NR <= 32), by one block of kRedThreads: every thread accumulates its
+// entries j = t, t + kRedThreads, ... of all rows (independent loads, all in
+// flight), the per-thread sums go through LDS, 4 threads per row add 32 of
+// them each, and thread t < NR adds its row's 4.  Fixed order throughout.
+// Thread t < NR returns the sum of row t (others return 0).
+template <int NR>
+__device__ __forceinline__ double rows_sum(const double *part, size_t stride, int n) {
+  static_assert(NR * 4 <= kRedThreads && kRedThreads == 128, "one block of 128 threads, NR <= 32");
+  __shared__ double lds[NR][kRedThreads];
+  __shared__ double l2[NR][4];
+  const int t = threadIdx.x;
+  double s[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) s[q] = 0.0;
+  for (int j = t; j < n; j += kRedThreads)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) s[q] += part[(size_t)q * stride + j];
+  __syncthreads();  // lds may still be read by a previous call
+#pragma unroll
+  for (int q = 0; q < NR; ++q) lds[q][t] = s[q];
+  __syncthreads();
+  if (t < NR * 4) {
+    const int q = t >> 2, sub = t & 3;
+    double a = 0.0;
+    for (int i = 0; i < kRedThreads / 4; ++i) a += lds[q][sub + 4 * i];
+    l2[q][sub] = a;
+  }
+  __syncthreads();
+  double r = 0.0;
+  if (t < NR)
+    for (int sub = 0; sub < 4; ++sub) r += l2[t][sub];
+  return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -257,11 +284,8 @@ __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const doubl
                                                          double abs_tol, double tau_min, int max_inner, MOuter *o,
                                                          MState *s0) {
   __shared__ double t[2 * kMaxRhs];
-  const int w = threadIdx.x >> 6;
-  for (int q = w; q < 2 * K; q += kRedThreads / 64) {
-    const double s = wave_row_sum(part + (size_t)q * nparts, nparts);
-    if ((threadIdx.x & 63) == 0) t[q] = s;
-  }
+  const double s = rows_sum<2 * K>(part, nparts, nparts);
+  if (threadIdx.x < 2 * K) t[threadIdx.x] = s;
   __syncthreads();
   if (threadIdx.x != 0) return;
   int all = 1;
@@ -379,15 +403,15 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
 template <int K>
 __global__ void __launch_bounds__(kRedThreads) k_m_alpha(int nsl, const double *part1, MState *cur) {
   if (cur->all_done) return;
-  const int w = threadIdx.x >> 6;
-  if (w >= K || cur->done[w]) return;  // whole wave
-  const double pap = wave_row_sum(part1 + (size_t)w * nsl, nsl);
-  if ((threadIdx.x & 63) == 0) cur->alpha[w] = cur->rz[w] / pap;
+  const double pap = rows_sum<K>(part1, nsl, nsl);
+  const int f = threadIdx.x;
+  if (f < K && !cur->done[f]) cur->alpha[f] = cur->rz[f] / pap;
 }
 
 // step 3: X += alpha P; R -= alpha AP; partials of z.z and R.z (z = kinv R).
 // Grid (float4 chunks of a component, component c = d * K + k); partial
-// tables pz / pr [c * gridDim.x + block].
+// tables pz / pr [(k * 3 + d) * gridDim.x + block], so the 3 * gridDim.x
+// partials of field k are contiguous; pr follows pz (pr = pz + 3 K nbx).
 template <int K>
 __global__ void __launch_bounds__(256) k_m_upd(int NP, const float4 *P, const float4 *AP, const float4 *kinv, float4 *X,
                                                float4 *R, double *pz, double *pr, const MState *cur) {
@@ -417,15 +441,15 @@ __global__ void __launch_bounds__(256) k_m_upd(int NP, const float4 *P, const fl
   }
   block_sum<2, 256>(v, lds);
   if (threadIdx.x == 0) {
-    pz[(size_t)c * gridDim.x + blockIdx.x] = v[0];
-    pr[(size_t)c * gridDim.x + blockIdx.x] = v[1];
+    const size_t o = ((size_t)f * 3 + d) * gridDim.x + blockIdx.x;
+    pz[o] = v[0];
+    pr[o] = v[1];
   }
 }
 
 // step 4 (one block): beta, stopping test, the next state
 template <int K>
-__global__ void __launch_bounds__(kRedThreads) k_m_beta(int nbx, const double *pz, const double *pr,
-                                                        const MState *cur, MState *nxt) {
+__global__ void __launch_bounds__(kRedThreads) k_m_beta(int nbx, const double *pz, const MState *cur, MState *nxt) {
   __shared__ double zz_s[kMaxRhs], rz_s[kMaxRhs];
   if (cur->all_done) {
     const int *s = reinterpret_cast<const int *>(cur);
@@ -433,23 +457,12 @@ __global__ void __launch_bounds__(kRedThreads) k_m_beta(int nbx, const double *p
     for (int q = threadIdx.x; q < (int)(sizeof(MState) / sizeof(int)); q += kRedThreads) t[q] = s[q];
     return;
   }
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (w < K && !cur->done[w]) {
-    double sz = 0.0, sr = 0.0;
-    for (int d = 0; d < 3; ++d) {
-      const size_t row = (size_t)(d * K + w) * nbx;
-      for (int b = lane; b < nbx; b += 64) {
-        sz += pz[row + b];
-        sr += pr[row + b];
-      }
-    }
-    sz = wave_sum(sz);
-    sr = wave_sum(sr);
-    if (lane == 0) {
-      zz_s[w] = sz;
-      rz_s[w] = sr;
-    }
-  }
+  // rows 0..K-1: z.z of each field; rows K..2K-1: R.z (the pr table follows pz)
+  const double s = rows_sum<2 * K>(pz, (size_t)3 * nbx, 3 * nbx);
+  if (threadIdx.x < K)
+    zz_s[threadIdx.x] = s;
+  else if (threadIdx.x < 2 * K)
+    rz_s[threadIdx.x - K] = s;
   __syncthreads();
   if (threadIdx.x != 0) return;
   int all = 1;
@@ -654,7 +667,7 @@ hipError_t launch_m_iter(const Grid &g, int K, const float *A, float *X, float *
     hipLaunchKernelGGL(k_m_upd<KK>, vgrid, dim3(256), 0, st, s.NP, reinterpret_cast<const float4 *>(P),
                        reinterpret_cast<const float4 *>(AP), reinterpret_cast<const float4 *>(kinv),
                        reinterpret_cast<float4 *>(X), reinterpret_cast<float4 *>(R), pz, pr, cur);
-    hipLaunchKernelGGL(k_m_beta<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nbx, pz, pr, cur, nxt);
+    hipLaunchKernelGGL(k_m_beta<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nbx, pz, cur, nxt);
     hipLaunchKernelGGL(k_m_pupd<KK>, vgrid, dim3(256), 0, st, s.NP, reinterpret_cast<const float4 *>(R),
                        reinterpret_cast<const float4 *>(kinv), reinterpret_cast<float4 *>(P), nxt);
   });
@@ -690,6 +703,12 @@ void mixed_state_view(const void *state_h, int parity, int *its, int *all_done) 
   const MState *ms = static_cast<const MState *>(state_h) + (parity & 1);
   *its = ms->its;
   *all_done = ms->all_done;
+}
+int mixed_state_field(const void *state_h, int parity, int k, double *pn, double *target) {
+  const MState *ms = static_cast<const MState *>(state_h) + (parity & 1);
+  *pn = ms->pn[k];
+  *target = ms->tau[k] * ms->pn0[k];
+  return ms->done[k];
 }
 void mixed_outer_view(const void *state_h, int K, int *all_conv, double *pn, double *pn0) {
   const MOuter *mo = reinterpret_cast<const MOuter *>(static_cast<const MState *>(state_h) + 2);

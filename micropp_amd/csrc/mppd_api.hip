@@ -28,30 +28,10 @@ int hip_fail(hipError_t e, const char *what) {
   return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? MPPD_ERR_NOMEM : MPPD_ERR_HIP;
 }
 
-template <typename T>
-static int dalloc(T **p, size_t n, bool zero = true) {
-  *p = nullptr;
-  if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
-  if (zero) {
-    e = hipMemset(*p, 0, n * sizeof(T));
-    // hipMemset runs on the null stream, which the solvers' non-blocking
-    // streams do not order against: finish it before the buffer is used
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) return hip_fail(e, "hipMemset");
-  }
-  return MPPD_OK;
-}
-
-static void dfree(void *p) {
-  if (p) (void)hipFree(p);
-}
-
 // make_material (src/material.cpp:26-46) with the constructors of
 // include/material.hpp:66-135: derived constants rebuilt from E and nu, the
 // fields a law does not use set to -1.
-static int make_material(const material_base &in, material_base &m) {
+int make_material(const material_base &in, material_base &m) {
   if (in.type < MATERIAL_ELASTIC || in.type > MATERIAL_DAMAGE) return fail(MPPD_ERR_ARG, "invalid material type");
   m = in;
   m.k = in.E / (3. * (1. - 2. * in.nu));
@@ -68,19 +48,6 @@ static int make_material(const material_base &in, material_base &m) {
 }  // namespace mpp
 
 using namespace mpp;
-
-struct mppd_rve {
-  Grid g;
-  int micro_type = 0;
-  double geo[4];
-  material_base mats[3];
-  double *bmat = nullptr;
-  int *et = nullptr;
-  double *spheres = nullptr;
-  int nsph = 0;
-  double vf[2] = {0, 0};
-  int all_elastic = 1;
-};
 
 struct mppd_solver {
   mppd_rve *r = nullptr;
@@ -195,7 +162,12 @@ int mppd_rve_create(mppd_rve **out, const int size[3], int micro_type, const dou
   double *vf_d = nullptr;
   do {
     if ((rc = dalloc(&r->bmat, 8 * 144))) break;
+    if ((rc = dalloc(&r->mats_d, 3))) break;
     if ((rc = dalloc(&r->et, (size_t)g.nelem))) break;
+    if (hipMemcpy(r->mats_d, r->mats, 3 * sizeof(material_base), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = fail(MPPD_ERR_HIP, "materials upload");
+      break;
+    }
     if ((rc = dalloc(&r->spheres, (size_t)kMaxSpheres * 4))) break;
     if ((rc = dalloc(&bad, 1))) break;
     if ((rc = dalloc(&vf_d, 2))) break;
@@ -233,6 +205,7 @@ int mppd_rve_create(mppd_rve **out, const int size[3], int micro_type, const dou
 int mppd_rve_destroy(mppd_rve *r) {
   if (!r) return MPPD_OK;
   dfree(r->bmat);
+  dfree(r->mats_d);
   dfree(r->et);
   dfree(r->spheres);
   delete r;
@@ -373,7 +346,7 @@ int mppd_set_displ_bc(mppd_solver *s, const double eps[6], double *u_d) {
 
 static hipError_t rhs_async(mppd_solver *s, const double *u_d, const double *vars_d, double *b_d) {
   const mppd_rve *r = s->r;
-  hipError_t e = launch_elem_be(r->g, r->bmat, r->mats, r->et, u_d, vars_d, s->be, s->st);
+  hipError_t e = launch_elem_be(r->g, r->bmat, r->mats_d, r->et, u_d, vars_d, s->be, s->st);
   if (e == hipSuccess) e = launch_gather_rhs(r->g, s->be, b_d, s->partials, s->red, s->st);
   return e;
 }
@@ -396,11 +369,11 @@ static hipError_t mat_async(mppd_solver *s, const double *u_d, const double *var
   hipError_t e = hipSuccess;
   if (s->phase_mode) {
     if (!s->ke_phase_ready) {
-      e = launch_elem_ke(r->g, r->bmat, r->mats, r->et, nullptr, nullptr, 1, s->ke, s->st);
+      e = launch_elem_ke(r->g, r->bmat, r->mats_d, r->et, nullptr, nullptr, 1, s->ke, s->st);
       if (e == hipSuccess) s->ke_phase_ready = 1;
     }
   } else {
-    e = launch_elem_ke(r->g, r->bmat, r->mats, r->et, u_d, vars_d, 0, s->ke, s->st);
+    e = launch_elem_ke(r->g, r->bmat, r->mats_d, r->et, u_d, vars_d, 0, s->ke, s->st);
   }
   if (e == hipSuccess) e = launch_gather_vals(r->g, r->et, s->ke, s->phase_mode, s->vals, s->st);
   return e;
@@ -474,7 +447,7 @@ int mppd_cg(mppd_solver *s, const mppd_solver *A0, const double *b_d, double *x_
 
 int mppd_ave_stress(mppd_solver *s, const double *u_d, const double *vars_d, double sig[6]) {
   const mppd_rve *r = s->r;
-  MPP_HIP(launch_ave_stress(r->g, r->bmat, r->mats, r->et, u_d, vars_d, s->partials, s->red, s->st));
+  MPP_HIP(launch_ave_stress(r->g, r->bmat, r->mats_d, r->et, u_d, vars_d, s->partials, s->red, s->st));
   MPP_HIP(read_red(s));
   for (int v = 0; v < 6; ++v) sig[v] = s->red_h->v[v];
   return MPPD_OK;
@@ -483,7 +456,7 @@ int mppd_ave_stress(mppd_solver *s, const double *u_d, const double *vars_d, dou
 int mppd_vars_new(mppd_solver *s, const double *u_d, const double *vars_old_d, double *vars_new_d, int *nonlinear) {
   const mppd_rve *r = s->r;
   MPP_HIP(hipMemsetAsync(&s->red->flag, 0, sizeof(int), s->st));
-  MPP_HIP(launch_vars_new(r->g, r->bmat, r->mats, r->et, u_d, vars_old_d, vars_new_d, s->red, s->st));
+  MPP_HIP(launch_vars_new(r->g, r->bmat, r->mats_d, r->et, u_d, vars_old_d, vars_new_d, s->red, s->st));
   MPP_HIP(read_red(s));
   if (nonlinear) *nonlinear = s->red_h->flag ? 1 : 0;
   return MPPD_OK;
@@ -611,7 +584,7 @@ int mppd_msolver_create(mppd_msolver **out, mppd_rve *r, int nrhs) {
     if (e == hipSuccess) e = hipHostMalloc((void **)&m->sig_h, 16 * 6 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemcpy(m->mats_d, r->mats, 3 * sizeof(material_base), hipMemcpyHostToDevice);
     // element stiffness per phase (FP64, the plain path's kernel), then the FP32 node-block matrix
-    if (e == hipSuccess) e = launch_elem_ke(g, r->bmat, r->mats, r->et, nullptr, nullptr, 1, m->ke, m->st);
+    if (e == hipSuccess) e = launch_elem_ke(g, r->bmat, r->mats_d, r->et, nullptr, nullptr, 1, m->ke, m->st);
     if (e == hipSuccess) e = launch_m_asm32(g, r->et, m->ke, 1, m->A, m->kinv, m->st);
     if (e == hipSuccess) e = hipStreamSynchronize(m->st);
     if (e != hipSuccess) rc = hip_fail(e, "msolver set-up");
@@ -656,9 +629,16 @@ int mppd_msolver_solve(mppd_msolver *m, int n, const double *eps_h, const mppd_m
     MPP_HIP(hipStreamSynchronize(m->st));
     mixed_outer_view(m->state_h, K, &all_conv, pn.data(), pn0.data());
     if (all_conv || outer >= p->max_outer) break;
-    // inner FP32 CG, queued in chunks; after q iterations the state is state[q & 1]
-    int queued = 0, its = 0, done = 0;
+    // inner FP32 CG, queued in chunks; after q iterations the state is state[q & 1].
+    // The next chunk is sized from each live field's observed convergence rate
+    // (iterations still needed, + 2), so few iterations are queued past the end.
+    int queued = 0, its = 0, done = 0, its_prev = 0;
     int chunk = 32;
+    std::vector<double> pn_prev(K, 0.0);
+    for (int k = 0; k < K; ++k) {  // the start state read with the outer test
+      double target;
+      (void)mixed_state_field(m->state_h, 0, k, &pn_prev[k], &target);
+    }
     while (!done) {
       const int nq = std::min(chunk, p->max_inner_its - queued);
       if (nq <= 0) break;
@@ -668,7 +648,23 @@ int mppd_msolver_solve(mppd_msolver *m, int n, const double *eps_h, const mppd_m
       MPP_HIP(hipMemcpyAsync(m->state_h, m->state, mixed_state_bytes(), hipMemcpyDeviceToHost, m->st));
       MPP_HIP(hipStreamSynchronize(m->st));
       mixed_state_view(m->state_h, queued, &its, &done);
-      chunk = std::min(128, chunk * 2);
+      if (done) break;
+      int next = 0;
+      const int dit = its - its_prev;
+      for (int k = 0; k < K; ++k) {
+        double pnk, target;
+        if (mixed_state_field(m->state_h, queued, k, &pnk, &target)) continue;
+        int need = chunk * 2;
+        if (dit > 0 && pn_prev[k] > 0 && pnk > 0 && pnk < pn_prev[k]) {
+          const double rate = std::log(pnk / pn_prev[k]) / dit;
+          const double est = std::log(target / pnk) / rate;
+          if (std::isfinite(est)) need = (int)std::ceil(est) + 2;
+        }
+        next = std::max(next, need);
+        pn_prev[k] = pnk;
+      }
+      its_prev = its;
+      chunk = std::max(4, std::min(256, next));
     }
     inner_total += its;
     MPP_HIP(launch_m_accum(g, K, m->X, m->U, m->st));
