@@ -118,6 +118,139 @@ def rel(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
 
 
+# ---------------------------------------------------------------------------
+# secondary workloads (not the headline): batched nonlinear RVEs and adjoint derivatives
+# ---------------------------------------------------------------------------
+def _batch_run(mp, descs, ramps, params, label):
+    """Batched homogenize over the load steps, then the same RVEs one micropp<3> instance after another (a
+    subset, for the per-RVE rate); returns the measurement dict."""
+    b = mp.Batch(descs, **params)
+    nrve, steps = len(descs), len(ramps[0])
+    mp.load().mppd_synchronize()
+    t0 = time.perf_counter()
+    for t in range(steps):
+        b.set_strains(np.array([r[t] for r in ramps]))
+        b.homogenize()
+        b.update_vars()
+        print(f"{label} batched step {t} done", file=sys.stderr, flush=True)
+    dt_b = time.perf_counter() - t0
+    info = b.info()
+    stats = b.stats()
+    sig_b = b.stress()
+    del b
+    nseq = min(nrve, 4)
+    kw = {k: v for k, v in params.items() if k != "graph_chunk"}
+    t0 = time.perf_counter()
+    sig_s = []
+    for i in range(nseq):
+        d = descs[i]
+        m = mp.Micropp3(d["size"], d["micro_type"], d["geo"], d["materials"], ngp=1, coupling=[d["coupling"]], **kw)
+        for t in range(steps):
+            m.set_strain(0, ramps[i][t])
+            m.homogenize()
+            m.update_vars()
+        sig_s.append(m.get_stress(0))
+        m.close()
+        print(f"{label} single RVE {i} done", file=sys.stderr, flush=True)
+    dt_s = (time.perf_counter() - t0) / nseq
+    return {"value": nrve * steps / dt_b, "unit": "RVE solves/s (sigma+ctan, FE_FULL)", "rves": nrve,
+            "load_steps": steps, "batched_s": dt_b, "one_by_one_rve_solves_per_s": steps / dt_s,
+            "speedup_vs_one_by_one": (nrve * steps / dt_b) / (steps / dt_s),
+            "batched_equals_one_by_one": bool(np.array_equal(np.array(sig_s), sig_b[:nseq])),
+            "nonlinear_rves": int(info[:, 0].sum()), "converged_rves": int(info[:, 2].sum()),
+            "graph_chunk": int(stats[1]), "cg_iterations_queued": int(stats[0])}
+
+
+def batch_j2(nrve, steps):
+    """RVEs of 30^3 hexes, a z-fibre of radius U[0.1, 0.3], J2 matrix (E 1e6, nu 0.3, Sy 1e4, Ka 1e5), elastic
+    fibre; cyclic uniaxial + shear strain of amplitude 2e-3."""
+    import micropp_amd as mp
+
+    nrve, steps = nrve or 32, steps or 2
+    rng = np.random.default_rng(3003)
+    mats = [mp.material(1, 1e6, 0.3, 1e5, 1e4), mp.material(0, 1e7, 0.25), mp.material(0, 1e7, 0.25)]
+    descs = [dict(size=31, micro_type=mp.MIC_CILI_FIB_Z, geo=(rng.uniform(0.1, 0.3), 0, 0, 0), materials=mats,
+                  coupling=mp.FE_FULL) for _ in range(nrve)]
+    ramps = []
+    for _ in range(nrve):
+        d = np.zeros(6)
+        d[0], d[3] = 2e-3, rng.uniform(-2e-3, 2e-3)
+        ramps.append([d * np.sin(2 * np.pi * (t + 1) / 20) / np.sin(2 * np.pi / 20) for t in range(steps)])
+    params = dict(lin_stress=0, nr_max_its=10, nr_rel_tol=1e-6, cg_rel_tol=1e-8, cg_max_its=2000)
+    out = {"metric": "batched J2 RVEs (cylindrical fibre), RVE solves/s", "workload":
+           f"{nrve} RVEs of 30^3 hexes, z-fibre radius U[0.1,0.3], J2 matrix, {steps} cyclic load steps"}
+    out.update(_batch_run(mp, descs, ramps, params, "batch_j2"))
+    return out
+
+
+def batch_damage(nrve, steps):
+    """RVEs of n in {20, 30, 40, 50} hexes (seeded mix), a y-layer of width U[0.1, 0.3], damage matrix (E 1e6,
+    nu 0.3, Xt = 1e3 sqrt(E)), elastic fibre; a seeded strain ramp to |eps| = 1e-2 over 10 steps."""
+    import micropp_amd as mp
+
+    nrve, steps = nrve or 16, steps or 2
+    rng = np.random.default_rng(2002)
+    mats = [mp.material(2, 1e6, 0.3, 0, 0, 1e3 * np.sqrt(1e6)), mp.material(0, 1e7, 0.25),
+            mp.material(0, 1e7, 0.25)]
+    descs = [dict(size=int(rng.choice([20, 30, 40, 50])) + 1, micro_type=mp.MIC_LAYER_Y,
+                  geo=(rng.uniform(0.1, 0.3), 0, 0, 0), materials=mats, coupling=mp.FE_FULL) for _ in range(nrve)]
+    ramps = []
+    for _ in range(nrve):
+        d = rng.normal(size=6)
+        d *= 1e-2 / np.linalg.norm(d)
+        ramps.append([d * (t + 1) / 10 for t in range(steps)])
+    params = dict(lin_stress=0, nr_max_its=10, nr_rel_tol=1e-6, cg_rel_tol=1e-8, cg_max_its=2000)
+    out = {"metric": "batched damage RVEs (fibre layer, uneven sizes), RVE solves/s", "workload":
+           f"{nrve} RVEs of 20..50^3 hexes, y-layer width U[0.1,0.3], damage matrix, {steps} ramp steps"}
+    out.update(_batch_run(mp, descs, ramps, params, "batch_damage"))
+    return out
+
+
+def adjoint_workload(nrve, _steps):
+    """Elastic RVEs of 40^3 hexes with 1-10 seeded spheres (radii U[0.05, 0.2]): d sigma / d eps and
+    d sigma / d (E, nu) per phase from the adjoint (one multi-RHS solve: 6 adjoint + 2 forward fields) vs the
+    forward approach (6 perturbed solves for ctan + central differences for the 4 material parameters)."""
+    import micropp_amd as mp
+
+    nrve = nrve or 4
+    rng = np.random.default_rng(4004)
+    spec = [(0, 1e6, 0.3), (0, 1e7, 0.25), (0, 1e7, 0.25)]
+    t_adj = t_fwd = 0.0
+    err_ctan = err_fd = 0.0
+    for i in range(nrve):
+        geo = (float(rng.integers(1, 1 << 30)), float(rng.integers(1, 11)), 0.05, 0.2)
+        strains = rng.uniform(-5e-3, 5e-3, (2, 6))
+        r = mp.Rve(41, mp.MIC3D_RANDOM_SPHERES, geo, [mp.material(*s) for s in spec])
+        ms = mp.MSolver(r, 8)
+        t0 = time.perf_counter()
+        ctan, sig, dmat, _, _, ok = ms.adjoint(strains)
+        t_adj += time.perf_counter() - t0
+        # forward: ctan from 6 unit-strain solves + sigma, then central differences (step 1e-6 relative) for
+        # (E, nu) of phases 0 and 1, each a fresh RVE and solve
+        t0 = time.perf_counter()
+        sg, ctan_f, _, _, _ = mp.homogenize_mixed(ms, strains)
+        fd = np.zeros((2, 6, 2, 2))
+        for p in range(2):
+            for w in (1, 2):
+                h = spec[p][w] * 1e-6
+                res = []
+                for sgn in (1, -1):
+                    sp = [list(s) for s in spec]
+                    sp[p][w] += sgn * h
+                    rr = mp.Rve(41, mp.MIC3D_RANDOM_SPHERES, geo, [mp.material(*s) for s in sp])
+                    res.append(mp.MSolver(rr, 8).solve(strains, rel_tol=1e-12, max_outer=30)[0])
+                fd[:, :, p, w - 1] = (res[0] - res[1]) / (2 * h)
+        t_fwd += time.perf_counter() - t0
+        err_ctan = max(err_ctan, rel(ctan, ctan_f))
+        err_fd = max(err_fd, rel(dmat[:, :, :2, :], fd))
+        print(f"adjoint RVE {i} done", file=sys.stderr, flush=True)
+    return {"metric": "adjoint derivatives (40^3 elastic RVEs), RVEs/s", "value": nrve / t_adj,
+            "unit": "RVEs/s (ctan + d sigma/d(E,nu) for 2 strains)", "rves": nrve,
+            "adjoint_ms_per_rve": 1e3 * t_adj / nrve, "forward_fd_ms_per_rve": 1e3 * t_fwd / nrve,
+            "speedup_vs_forward": t_fwd / t_adj, "ctan_rel_err_vs_forward": err_ctan,
+            "dmat_rel_err_vs_central_fd": err_fd}
+
+
 def pmc_traffic(summary):
     """HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE runs whose summary (commands, copy-probe
     scaling) is profiles/<summary>; None when the summary is absent."""
@@ -159,6 +292,11 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of COPY_PROBE_BYTES (scales HBM counter readings)")
+    ap.add_argument("--workload", default="mixed50",
+                    choices=["mixed50", "batch_j2", "batch_damage", "adjoint"],
+                    help="mixed50 (default, the headline); the others measure the batched and adjoint paths")
+    ap.add_argument("--rves", type=int, default=0, help="RVEs of a batch / adjoint workload (0: its default)")
+    ap.add_argument("--load-steps", type=int, default=0, help="load steps of a batch workload (0: its default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); RVEs are independent, so a "
@@ -166,6 +304,10 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.workload != "mixed50":
+        fn = {"batch_j2": batch_j2, "batch_damage": batch_damage, "adjoint": adjoint_workload}[args.workload]
+        print(json.dumps(fn(args.rves, args.load_steps)), flush=True)
         return
 
     import micropp_amd as mp

@@ -18,8 +18,8 @@ enum { MATERIAL_ELASTIC = 0, MATERIAL_PLASTIC, MATERIAL_DAMAGE };
 #define SQRT_2DIV3 0.816496581
 
 #ifdef __cplusplus
-extern "C" {
 #include <cstdio>
+extern "C" {
 #else
 #include <stdbool.h>
 #include <stdio.h>

@@ -149,6 +149,18 @@ int mppd_msolver_solve(mppd_msolver *m, int n, const double *eps_h, const mppd_m
                        int stats[2], double *rel_h);
 /* displacement field k (nrow doubles, reference DOF numbering) of the last solve */
 int mppd_msolver_get_u(mppd_msolver *m, int k, double *u_h);
+/* Adjoint derivatives of the volume-averaged stress (no reference
+ * counterpart: the reference differentiates by six perturbed solves,
+ * calc_ctan_lin_fe_models src/micropp.cpp:255-284 and the FE_FULL loop
+ * src/homogenize.cpp:256-275).  One multi-RHS solve carries the six adjoint
+ * fields K_II lambda_v = d(sigma_v)/du (one transposed solve per stress
+ * component; K is symmetric on the interior DOFs) and the n <= K - 6 forward
+ * fields of eps_h.  Outputs: ctan_h[36] = d sigma_v / d eps_i (v * 6 + i);
+ * sig_h[n * 6]; dmat_h[n * 36] = d sigma_v / d (E, nu) of phase p of strain s
+ * at (s * 6 + v) * 6 + p * 2 + {0: E, 1: nu}; rel_h (may be NULL): the 6 + n
+ * final residual ratios.  n = 0 gives ctan alone. */
+int mppd_msolver_adjoint(mppd_msolver *m, int n, const double *eps_h, const mppd_mixed_params *p, double *sig_h,
+                         double *ctan_h, double *dmat_h, int stats[2], double *rel_h);
 /* timing aid: average ms of the FP32 multi-RHS SpMV kernel and of a whole
  * inner iteration, over iters launches, events on the solver's stream */
 int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *iter_ms);

@@ -139,6 +139,7 @@ SIGNATURES = {
     "mppd_msolver_destroy": (_I, [_VP]),
     "mppd_msolver_solve": (_I, [_VP, _I, _PD, C.c_void_p, _PD, _PI, _PD]),
     "mppd_msolver_get_u": (_I, [_VP, _I, _PD]),
+    "mppd_msolver_adjoint": (_I, [_VP, _I, _PD, C.c_void_p, _PD, _PD, _PD, _PI, _PD]),
     "mppd_msolver_profile": (_I, [_VP, _I, _PD, _PD]),
     "mppd_batch_params_default": (None, [_VP]),
     "mppd_batch_create": (_I, [C.POINTER(C.c_void_p), _I, _VP, _VP]),
@@ -529,6 +530,25 @@ class MSolver:
         if rc not in (0, self.NOT_CONVERGED):
             _chk(rc, "mppd_msolver_solve")
         return sig, (int(stats[0]), int(stats[1])), rel, rc == 0
+
+    def adjoint(self, strains=None, rel_tol=1e-8, abs_tol=1e-50, inner_tol=1e-5, max_inner_its=2000, max_outer=10):
+        """Adjoint derivatives: (ctan 6x6 = dsigma/deps, sigma n x 6, dsigma/d(E, nu) n x 6 x 3 x 2 [s][v][phase][E|nu],
+        stats, residual ratios, converged)."""
+        e = np.zeros((0, 6)) if strains is None else np.ascontiguousarray(strains, dtype=np.float64).reshape(-1, 6)
+        n = e.shape[0]
+        ctan = np.zeros(36)
+        sig = np.zeros((max(n, 1), 6))
+        dmat = np.zeros((max(n, 1), 36))
+        stats = np.zeros(2, dtype=np.int32)
+        rel = np.zeros(6 + n)
+        p = MixedParams(rel_tol, abs_tol, inner_tol, max_inner_its, max_outer)
+        e_arg = e if n else np.zeros((1, 6))
+        rc = load().mppd_msolver_adjoint(self.h, n, _dp(np.ascontiguousarray(e_arg)), C.cast(C.byref(p), C.c_void_p),
+                                         _dp(sig), _dp(ctan), _dp(dmat), _ip(stats), _dp(rel))
+        if rc not in (0, self.NOT_CONVERGED):
+            _chk(rc, "mppd_msolver_adjoint")
+        return (ctan.reshape(6, 6), sig[:n], dmat[:n].reshape(n, 6, 3, 2), (int(stats[0]), int(stats[1])), rel,
+                rc == 0)
 
     def u(self, k):
         out = np.zeros(self.rve.nrow)

@@ -143,9 +143,19 @@ size_t mixed_state_bytes();
 hipError_t launch_m_asm32(const Grid &g, const int *et, const double *ke, int phase_mode, float *A, float *kinv,
                           hipStream_t st);
 hipError_t launch_m_setbc(const Grid &g, int K, const double *eps, double *U, hipStream_t st);
-hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke, const double *U, const float *kinv,
-                          float *R, float *X, float *P, double *part, int first, double rel, double abs_tol,
-                          double tau_min, int max_inner, void *state, hipStream_t st);
+// F: adjoint sources (FP64, the layout of U) or null
+hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke, const double *U, const double *F,
+                          const float *kinv, float *R, float *X, float *P, double *part, int first, double rel,
+                          double abs_tol, double tau_min, int max_inner, void *state, hipStream_t st);
+// adjoint: G (3 x 24 x 6 doubles) and the sources F of fields 0 .. nadj-1
+hipError_t launch_m_adj_src(const Grid &g, int K, int nadj, const double *bmat, const material_base *mats_d,
+                            const int *et, double *G, double *F, hipStream_t st);
+// adjoint derivatives from U (fields 0..5 = adjoint fields, s0 .. s0+ns-1 =
+// forward fields): ctan[36] = d sigma_v / d eps_i (v * 6 + i), dmat[ns * 36] =
+// d sigma_v / d (E, nu) of phase p ((s * 6 + v) * 6 + p * 2 + {0, 1})
+size_t mixed_adj_partials(const Grid &g, int ns);
+hipError_t launch_m_adj(const Grid &g, int K, const double *bmat, const material_base *mats_d, const int *et,
+                        const double *U, int s0, int ns, double *part, double *ctan, double *dmat, hipStream_t st);
 hipError_t launch_m_iter(const Grid &g, int K, const float *A, float *X, float *R, float *P, float *AP,
                          const float *kinv, double *part, void *state, int parity, hipStream_t st);
 hipError_t launch_m_spmv_only(const Grid &g, int K, const float *A, const float *P, float *AP, double *part,

@@ -195,15 +195,78 @@ __global__ void __launch_bounds__(256) k_m_setbc(Grid g, int NP, const double *e
 }
 
 // ---------------------------------------------------------------------------
-// FP64 residual r = -(K u) on interior DOFs (elastic: element blocks of the
-// phase table), rounded into the FP32 CG start: R = r, X = 0, P = kinv R.
+// Adjoint sources.  For sigma_v = (1/V) sum_e sum_gp wg [C B u_e]_v the
+// derivative with respect to the displacements is assembled from, per phase
+// p, G[p][q][v] = wg sum_m C_p[v][m] sum_gp B_gp[m][q] (q < 24 element DOFs).
+// The adjoint field v solves K_II lambda_v = (1/V) sum_e G[phase(e)][.][v] on
+// the interior DOFs, with lambda_v = 0 on the boundary.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_m_adj_G(const double *bmat, const material_base *mats, double wg,
+                                                 double *G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * 24 * 6) return;
+  const int p = t / 144, q = (t / 6) % 24, v = t % 6;
+  double s = 0.0;
+  for (int m = 0; m < 6; ++m) {
+    double e[6] = {0, 0, 0, 0, 0, 0}, sg[6];
+    e[m] = 1.0;
+    lin_stress(mats[p], e, sg);  // sg[v] = C[v][m]
+    double bs = 0.0;
+    for (int gp = 0; gp < 8; ++gp) bs += bmat[gp * 144 + m * 24 + q];
+    s += sg[v] * bs;
+  }
+  G[t] = wg * s;
+}
+
+// F[(d K + v) NP + n]: the adjoint source of field v < nadj at interior node n;
+// zero on the boundary, on the padding and for the fields v >= nadj
+template <int K>
+__global__ void __launch_bounds__(256) k_m_adj_src(Grid g, int NP, int nadj, const int *et, const double *G,
+                                                   double *F) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= NP) return;
+  double f[3][K];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int v = 0; v < K; ++v) f[d][v] = 0.0;
+  if (n < g.nn) {
+    const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
+    if (!node_on_boundary(g, i, j, k)) {
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+          for (int c = 0; c < 2; ++c) {
+            const int e = ((k - 1 + c) * g.ney + (j - 1 + b)) * g.nex + (i - 1 + a);
+            const double *Gp = G + (size_t)et[e] * 144 + local_node(1 - a, 1 - b, 1 - c) * 3 * 6;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+              for (int v = 0; v < 6 && v < K; ++v)
+                if (v < nadj) f[d][v] += Gp[d * 6 + v];
+          }
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int v = 0; v < K; ++v) f[d][v] /= g.vol_tot;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int v = 0; v < K; ++v) F[(size_t)(d * K + v) * NP + n] = f[d][v];
+}
+
+// ---------------------------------------------------------------------------
+// FP64 residual r = F - (K u) on interior DOFs (elastic: element blocks of the
+// phase table; F = 0 unless adjoint sources are given), rounded into the FP32
+// CG start: R = r, X = 0, P = kinv R.
 // Partials: part[q * gridDim.x + block], q < K: (kinv r)^2 in FP64 of field
 // q; q >= K: R.P of field q - K.
 // ---------------------------------------------------------------------------
 template <int K>
 __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const int *et, const double *ke,
-                                                 const double *U, const float *kinv, float *R, float *X, float *P,
-                                                 double *part) {
+                                                 const double *U, const double *F, const float *kinv, float *R,
+                                                 float *X, float *P, double *part) {
   __shared__ double ke_s[3 * 576];
   __shared__ double lds[4 * 2 * K];
   for (int t = threadIdx.x; t < 3 * 576; t += blockDim.x) ke_s[t] = ke[t];
@@ -252,10 +315,10 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const 
         const float kv = kinv[(size_t)d * NP + n];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const double r64 = -acc[d][q];
+          const size_t idx = (size_t)(d * K + kg + q) * NP + n;
+          const double r64 = (F ? F[idx] : 0.0) - acc[d][q];
           const float r32 = (float)r64;
           const float p32 = kv * r32;
-          const size_t idx = (size_t)(d * K + kg + q) * NP + n;
           R[idx] = r32;
           X[idx] = 0.0f;
           P[idx] = p32;
@@ -512,8 +575,8 @@ __global__ void __launch_bounds__(256) k_m_accum(size_t n, const float *X, doubl
 }
 
 // ---------------------------------------------------------------------------
-// calc_ave_stress (src/average.cpp:57-82) of the K fields, FP64: per block
-// partials [K][6], then a one-block sum.
+// calc_ave_stress (src/average.cpp:57-82) of the K fields, FP64: partials
+// part[(field * 6 + v) * gridDim.x + block], then one block per output.
 // ---------------------------------------------------------------------------
 template <int K>
 __global__ void __launch_bounds__(256) k_m_ave(Grid g, int NP, const double *bmat_g, const material_base *mats,
@@ -552,18 +615,191 @@ __global__ void __launch_bounds__(256) k_m_ave(Grid g, int NP, const double *bma
       }
     }
     block_sum<6, 256>(s6, lds);
-    if (threadIdx.x < 6) part[((size_t)blockIdx.x * K + r) * 6 + threadIdx.x] = s6[threadIdx.x];
+    if (threadIdx.x < 6) part[(size_t)(r * 6 + threadIdx.x) * gridDim.x + blockIdx.x] = s6[threadIdx.x];
   }
 }
 
-__global__ void __launch_bounds__(256) k_m_ave_finish(int nparts, int nv, const double *part, double vol,
-                                                      double *out) {
-  // nv = 6 K <= 96: one thread per output, summing its partials in order
-  const int q = threadIdx.x;
-  if (q >= nv) return;
-  double s = 0.0;
-  for (int b = 0; b < nparts; ++b) s += part[(size_t)b * nv + q];
-  out[q] = s / vol;
+// ---------------------------------------------------------------------------
+// Adjoint derivatives (elastic).  With R(u) = K u - F_bc the interior
+// equations and lambda_v the adjoint fields (fields 0..5 of U), for any
+// parameter p of the boundary values or of the stiffness
+//   d sigma_v / dp = sum_e sum_gp wg (e_v / V - B lambda_v)^T dC/dp-or-C (B du/dp-or-u)
+// so both derivatives are single passes over the elements:
+//   d sigma_v / d eps_i : C, and w_i = the boundary values of the unit strain
+//                         e_i (zero inside), which only boundary elements see;
+//   d sigma_v / d E_p, d nu_p : dC/dE, dC/dnu of the phase p of the element,
+//                         and the forward field u of the strain.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool elem_touches_boundary(const Grid &g, int ex, int ey, int ez) {
+  return ex == 0 || ey == 0 || ez == 0 || ex == g.nex - 1 || ey == g.ney - 1 || ez == g.nez - 1;
+}
+
+// boundary value of unit strain i at node (i, j, k): the coordinate rule of
+// set_displ_bc (src/micro3D.cpp:26-78)
+__device__ __forceinline__ void unit_bc(const Grid &g, int ui, int i, int j, int k, double *w) {
+  const double L = 1.0;
+  double x[3];
+  if (i == g.nx - 1) {
+    x[0] = L, x[1] = j * g.dy, x[2] = k * g.dz;
+  } else if (i == 0) {
+    x[0] = 0, x[1] = j * g.dy, x[2] = k * g.dz;
+  } else if (j == g.ny - 1) {
+    x[0] = i * g.dx, x[1] = L, x[2] = k * g.dz;
+  } else if (j == 0) {
+    x[0] = i * g.dx, x[1] = 0, x[2] = k * g.dz;
+  } else if (k == g.nz - 1) {
+    x[0] = i * g.dx, x[1] = j * g.dy, x[2] = L;
+  } else {
+    x[0] = i * g.dx, x[1] = j * g.dy, x[2] = 0;
+  }
+  w[0] = w[1] = w[2] = 0.0;
+  if (ui < 3) {
+    w[ui] = x[ui];
+  } else {
+    const int a = ui == 5 ? 1 : 0, b = ui == 3 ? 1 : 2;  // 3: xy, 4: xz, 5: yz
+    w[a] = 0.5 * x[b];
+    w[b] = 0.5 * x[a];
+  }
+}
+
+// grid (element blocks, 6): blockIdx.y = v; part[(v * 6 + i) * gridDim.x + block]
+template <int K>
+__global__ void __launch_bounds__(256) k_m_adj_ctan(Grid g, int NP, const double *bmat_g, const material_base *mats,
+                                                    const int *et, const double *U, double *part) {
+  __shared__ double bm[8 * 144];
+  __shared__ double lds[4 * 6];
+  for (int t = threadIdx.x; t < 8 * 144; t += blockDim.x) bm[t] = bmat_g[t];
+  __syncthreads();
+  const int v = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (e < g.nelem) {
+    const int ex = e % g.nex, ey = (e / g.nex) % g.ney, ez = e / (g.nex * g.ney);
+    if (elem_touches_boundary(g, ex, ey, ez)) {
+      const material_base m = mats[et[e]];
+      double lam[24];
+      int bnd[8], ni[8], nj[8], nk[8];
+      for (int a = 0; a < 8; ++a) {
+        ni[a] = ex + kHexPos[a][0];
+        nj[a] = ey + kHexPos[a][1];
+        nk[a] = ez + kHexPos[a][2];
+        bnd[a] = node_on_boundary(g, ni[a], nj[a], nk[a]);
+        const int node = (nk[a] * g.ny + nj[a]) * g.nx + ni[a];
+        for (int d = 0; d < 3; ++d) lam[a * 3 + d] = U[(size_t)(d * K + v) * NP + node];
+      }
+      for (int ui = 0; ui < 6; ++ui) {
+        double we[24];
+        for (int a = 0; a < 8; ++a) {
+          if (bnd[a]) {
+            unit_bc(g, ui, ni[a], nj[a], nk[a], &we[a * 3]);
+          } else {
+            we[a * 3] = we[a * 3 + 1] = we[a * 3 + 2] = 0.0;
+          }
+        }
+        for (int gp = 0; gp < 8; ++gp) {
+          const double *B = &bm[gp * 144];
+          double ew[6], el[6], s[6];
+          for (int q = 0; q < 6; ++q) {
+            double a1 = 0.0, a2 = 0.0;
+            for (int c = 0; c < 24; ++c) {
+              a1 += B[q * 24 + c] * we[c];
+              a2 += B[q * 24 + c] * lam[c];
+            }
+            ew[q] = a1;
+            el[q] = a2;
+          }
+          lin_stress(m, ew, s);
+          double dl = 0.0;
+          for (int q = 0; q < 6; ++q) dl += el[q] * s[q];
+          acc[ui] += g.wg * (s[v] / g.vol_tot - dl);
+        }
+      }
+    }
+  }
+  block_sum<6, 256>(acc, lds);
+  if (threadIdx.x < 6) part[(size_t)(v * 6 + threadIdx.x) * gridDim.x + blockIdx.x] = acc[threadIdx.x];
+}
+
+// d lambda / d p and d mu / d p of an elastic phase (p = E: which 0, nu: 1),
+// as a material whose lin_stress is dC/dp . eps
+__device__ __forceinline__ material_base dmat(const material_base &m, int which) {
+  material_base d = m;
+  const double E = m.E, nu = m.nu;
+  if (which == 0) {
+    d.lambda = nu / ((1 + nu) * (1 - 2 * nu));
+    d.mu = 1 / (2 * (1 + nu));
+  } else {
+    d.lambda = E * (1 + 2 * nu * nu) / ((1 + nu) * (1 + nu) * (1 - 2 * nu) * (1 - 2 * nu));
+    d.mu = -E / (2 * (1 + nu) * (1 + nu));
+  }
+  return d;
+}
+
+// grid (element blocks, ns * 6): blockIdx.y = s * 6 + v; forward field s0 + s;
+// part[((s * 6 + v) * 6 + phase * 2 + {E: 0, nu: 1}) * gridDim.x + block]
+template <int K>
+__global__ void __launch_bounds__(256) k_m_adj_dmat(Grid g, int NP, const double *bmat_g, const material_base *mats,
+                                                    const int *et, const double *U, int s0, double *part) {
+  __shared__ double bm[8 * 144];
+  __shared__ double lds[4 * 6];
+  for (int t = threadIdx.x; t < 8 * 144; t += blockDim.x) bm[t] = bmat_g[t];
+  __syncthreads();
+  const int s = blockIdx.y / 6, v = blockIdx.y % 6;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (e < g.nelem) {
+    const int ex = e % g.nex, ey = (e / g.nex) % g.ney, ez = e / (g.nex * g.ney);
+    const int p = et[e];
+    const material_base dE = dmat(mats[p], 0), dN = dmat(mats[p], 1);
+    double lam[24], ue[24];
+    for (int a = 0; a < 8; ++a) {
+      const int node = ((ez + kHexPos[a][2]) * g.ny + (ey + kHexPos[a][1])) * g.nx + ex + kHexPos[a][0];
+      for (int d = 0; d < 3; ++d) {
+        lam[a * 3 + d] = U[(size_t)(d * K + v) * NP + node];
+        ue[a * 3 + d] = U[(size_t)(d * K + s0 + s) * NP + node];
+      }
+    }
+    double cE = 0.0, cN = 0.0;
+    for (int gp = 0; gp < 8; ++gp) {
+      const double *B = &bm[gp * 144];
+      double eu[6], el[6], sE[6], sN[6];
+      for (int q = 0; q < 6; ++q) {
+        double a1 = 0.0, a2 = 0.0;
+        for (int c = 0; c < 24; ++c) {
+          a1 += B[q * 24 + c] * ue[c];
+          a2 += B[q * 24 + c] * lam[c];
+        }
+        eu[q] = a1;
+        el[q] = a2;
+      }
+      lin_stress(dE, eu, sE);
+      lin_stress(dN, eu, sN);
+      double dlE = 0.0, dlN = 0.0;
+      for (int q = 0; q < 6; ++q) {
+        dlE += el[q] * sE[q];
+        dlN += el[q] * sN[q];
+      }
+      cE += g.wg * (sE[v] / g.vol_tot - dlE);
+      cN += g.wg * (sN[v] / g.vol_tot - dlN);
+    }
+    for (int q = 0; q < 3; ++q)
+      if (p == q) {
+        acc[2 * q] = cE;
+        acc[2 * q + 1] = cN;
+      }
+  }
+  block_sum<6, 256>(acc, lds);
+  if (threadIdx.x < 6) part[(size_t)(blockIdx.y * 6 + threadIdx.x) * gridDim.x + blockIdx.x] = acc[threadIdx.x];
+}
+
+// one block per row: out[row] = (sum of part[row * n .. + n)) / div, fixed order
+__global__ void __launch_bounds__(256) k_m_rows(const double *part, int n, double div, double *out) {
+  __shared__ double lds[4];
+  double s[1] = {0.0};
+  const double *row = part + (size_t)blockIdx.x * n;
+  for (int j = threadIdx.x; j < n; j += 256) s[0] += row[j];
+  block_sum<1, 256>(s, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0] / div;
 }
 
 // ---------------------------------------------------------------------------
@@ -636,14 +872,15 @@ hipError_t launch_m_setbc(const Grid &g, int K, const double *eps, double *U, hi
 }
 
 // residual + outer test + inner start state
-hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke, const double *U, const float *kinv,
-                          float *R, float *X, float *P, double *part, int first, double rel, double abs_tol,
-                          double tau_min, int max_inner, void *state, hipStream_t st) {
+hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke, const double *U, const double *F,
+                          const float *kinv, float *R, float *X, float *P, double *part, int first, double rel,
+                          double abs_tol, double tau_min, int max_inner, void *state, hipStream_t st) {
   const MLaunch s = mshape(g);
   MState *ms = static_cast<MState *>(state);
   MOuter *mo = reinterpret_cast<MOuter *>(ms + 2);
   MPP_K_DISPATCH(K, {
-    hipLaunchKernelGGL(k_m_resid<KK>, dim3(s.nb), dim3(256), 0, st, g, s.NP, s.per, et, ke, U, kinv, R, X, P, part);
+    hipLaunchKernelGGL(k_m_resid<KK>, dim3(s.nb), dim3(256), 0, st, g, s.NP, s.per, et, ke, U, F, kinv, R, X, P,
+                       part);
     hipLaunchKernelGGL(k_m_outer<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nb, part, first, rel, abs_tol, tau_min,
                        max_inner, mo, ms);
   });
@@ -694,7 +931,35 @@ hipError_t launch_m_ave(const Grid &g, int K, const double *bmat, const material
   const MLaunch s = mshape(g);
   const unsigned nb = nblk(g.nelem, 256);
   MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_ave<KK>, dim3(nb), dim3(256), 0, st, g, s.NP, bmat, mats_d, et, U, part));
-  hipLaunchKernelGGL(k_m_ave_finish, dim3(1), dim3(128), 0, st, (int)nb, 6 * K, part, g.vol_tot, out);
+  hipLaunchKernelGGL(k_m_rows, dim3(6 * K), dim3(256), 0, st, part, (int)nb, g.vol_tot, out);
+  return hipGetLastError();
+}
+
+size_t mixed_adj_partials(const Grid &g, int ns) {
+  return (size_t)nblk(g.nelem, 256) * 36 * (size_t)std::max(1, ns);
+}
+
+hipError_t launch_m_adj_src(const Grid &g, int K, int nadj, const double *bmat, const material_base *mats_d,
+                            const int *et, double *G, double *F, hipStream_t st) {
+  const MLaunch s = mshape(g);
+  hipLaunchKernelGGL(k_m_adj_G, dim3(2), dim3(256), 0, st, bmat, mats_d, g.wg, G);
+  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_adj_src<KK>, dim3(s.NP / 256), dim3(256), 0, st, g, s.NP, nadj, et, G,
+                                       F));
+  return hipGetLastError();
+}
+
+hipError_t launch_m_adj(const Grid &g, int K, const double *bmat, const material_base *mats_d, const int *et,
+                        const double *U, int s0, int ns, double *part, double *ctan, double *dmat, hipStream_t st) {
+  const MLaunch s = mshape(g);
+  const unsigned nb = nblk(g.nelem, 256);
+  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_adj_ctan<KK>, dim3(nb, 6), dim3(256), 0, st, g, s.NP, bmat, mats_d, et,
+                                       U, part));
+  hipLaunchKernelGGL(k_m_rows, dim3(36), dim3(256), 0, st, part, (int)nb, 1.0, ctan);
+  if (ns > 0) {
+    MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_adj_dmat<KK>, dim3(nb, 6 * ns), dim3(256), 0, st, g, s.NP, bmat,
+                                         mats_d, et, U, s0, part));
+    hipLaunchKernelGGL(k_m_rows, dim3(36 * ns), dim3(256), 0, st, part, (int)nb, 1.0, dmat);
+  }
   return hipGetLastError();
 }
 

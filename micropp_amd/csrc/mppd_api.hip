@@ -545,6 +545,9 @@ struct mppd_msolver {
   double *U = nullptr, *eps_d = nullptr, *part = nullptr, *sig_d = nullptr;
   void *state = nullptr, *state_h = nullptr;
   double *eps_h = nullptr, *sig_h = nullptr;  // pinned staging
+  // adjoint (allocated on first use): sources, per-phase source table,
+  // partials, results (device and pinned)
+  double *F = nullptr, *G = nullptr, *apart = nullptr, *adj_d = nullptr, *adj_h = nullptr;
 };
 
 extern "C" {
@@ -602,29 +605,41 @@ int mppd_msolver_destroy(mppd_msolver *m) {
   if (m->st) (void)hipStreamSynchronize(m->st);
   for (void *p : {(void *)m->ke, (void *)m->mats_d, (void *)m->A, (void *)m->kinv, (void *)m->X, (void *)m->R,
                   (void *)m->P, (void *)m->AP, (void *)m->U, (void *)m->eps_d, (void *)m->sig_d, (void *)m->part,
-                  m->state})
+                  m->state, (void *)m->F, (void *)m->G, (void *)m->apart, (void *)m->adj_d})
     dfree(p);
-  for (void *p : {m->state_h, (void *)m->eps_h, (void *)m->sig_h})
+  for (void *p : {m->state_h, (void *)m->eps_h, (void *)m->sig_h, (void *)m->adj_h})
     if (p) (void)hipHostFree(p);
   if (m->st) (void)hipStreamDestroy(m->st);
   delete m;
   return MPPD_OK;
 }
 
-int mppd_msolver_solve(mppd_msolver *m, int n, const double *eps_h, const mppd_mixed_params *p, double *sig_h,
-                       int stats[2], double *rel_h) {
-  if (!m || !eps_h || !p || !sig_h || n < 1 || n > m->K) return fail(MPPD_ERR_ARG, "bad msolver_solve arguments");
+// The K fields: 0 .. nadj-1 adjoint fields (zero boundary values, source
+// a_v / V of field v), then n forward fields with the macro strains eps_h,
+// then idle fields.  FP64 defect correction with FP32 inner CG until every
+// field converges; the solution is left in m->U.
+static int msolver_run(mppd_msolver *m, int nadj, int n, const double *eps_h, const mppd_mixed_params *p,
+                       int stats[2], std::vector<double> &pn, std::vector<double> &pn0, int *all_conv_out) {
   const Grid &g = m->r->g;
   const int K = m->K;
-  for (int i = 0; i < K * 6; ++i) m->eps_h[i] = i < n * 6 ? eps_h[i] : 0.0;
+  for (int i = 0; i < K * 6; ++i) {
+    const int f = i / 6;
+    m->eps_h[i] = (f >= nadj && f < nadj + n) ? eps_h[(f - nadj) * 6 + i % 6] : 0.0;
+  }
   MPP_HIP(hipMemcpyAsync(m->eps_d, m->eps_h, K * 6 * sizeof(double), hipMemcpyHostToDevice, m->st));
   MPP_HIP(hipMemsetAsync(m->U, 0, mixed_vector_len(g, K) * sizeof(double), m->st));
   MPP_HIP(launch_m_setbc(g, K, m->eps_d, m->U, m->st));
+  const double *F = nullptr;
+  if (nadj > 0) {
+    MPP_HIP(launch_m_adj_src(g, K, nadj, m->r->bmat, m->mats_d, m->r->et, m->G, m->F, m->st));
+    F = m->F;
+  }
   int outer = 0, inner_total = 0, all_conv = 0;
-  std::vector<double> pn(K), pn0(K);
+  pn.assign(K, 0.0);
+  pn0.assign(K, 0.0);
   for (;;) {
-    MPP_HIP(launch_m_resid(g, K, m->r->et, m->ke, m->U, m->kinv, m->R, m->X, m->P, m->part, outer == 0, p->rel_tol,
-                           p->abs_tol, p->inner_tol, p->max_inner_its, m->state, m->st));
+    MPP_HIP(launch_m_resid(g, K, m->r->et, m->ke, m->U, F, m->kinv, m->R, m->X, m->P, m->part, outer == 0,
+                           p->rel_tol, p->abs_tol, p->inner_tol, p->max_inner_its, m->state, m->st));
     MPP_HIP(hipMemcpyAsync(m->state_h, m->state, mixed_state_bytes(), hipMemcpyDeviceToHost, m->st));
     MPP_HIP(hipStreamSynchronize(m->st));
     mixed_outer_view(m->state_h, K, &all_conv, pn.data(), pn0.data());
@@ -670,17 +685,64 @@ int mppd_msolver_solve(mppd_msolver *m, int n, const double *eps_h, const mppd_m
     MPP_HIP(launch_m_accum(g, K, m->X, m->U, m->st));
     ++outer;
   }
-  MPP_HIP(launch_m_ave(g, K, m->r->bmat, m->mats_d, m->r->et, m->U, m->part, m->sig_d, m->st));
-  MPP_HIP(hipMemcpyAsync(m->sig_h, m->sig_d, K * 6 * sizeof(double), hipMemcpyDeviceToHost, m->st));
-  MPP_HIP(hipStreamSynchronize(m->st));
-  for (int i = 0; i < n * 6; ++i) sig_h[i] = m->sig_h[i];
   if (stats) {
     stats[0] = outer;
     stats[1] = inner_total;
   }
+  *all_conv_out = all_conv;
+  return MPPD_OK;
+}
+
+// volume averages of all K fields into m->sig_h
+static int msolver_averages(mppd_msolver *m) {
+  const Grid &g = m->r->g;
+  MPP_HIP(launch_m_ave(g, m->K, m->r->bmat, m->mats_d, m->r->et, m->U, m->part, m->sig_d, m->st));
+  MPP_HIP(hipMemcpyAsync(m->sig_h, m->sig_d, m->K * 6 * sizeof(double), hipMemcpyDeviceToHost, m->st));
+  MPP_HIP(hipStreamSynchronize(m->st));
+  return MPPD_OK;
+}
+
+int mppd_msolver_solve(mppd_msolver *m, int n, const double *eps_h, const mppd_mixed_params *p, double *sig_h,
+                       int stats[2], double *rel_h) {
+  if (!m || !eps_h || !p || !sig_h || n < 1 || n > m->K) return fail(MPPD_ERR_ARG, "bad msolver_solve arguments");
+  std::vector<double> pn, pn0;
+  int all_conv = 0;
+  int rc = msolver_run(m, 0, n, eps_h, p, stats, pn, pn0, &all_conv);
+  if (!rc) rc = msolver_averages(m);
+  if (rc) return rc;
+  for (int i = 0; i < n * 6; ++i) sig_h[i] = m->sig_h[i];
   if (rel_h)
     for (int i = 0; i < n; ++i) rel_h[i] = pn0[i] > 0 ? pn[i] / pn0[i] : 0.0;
   return all_conv ? MPPD_OK : fail(MPPD_ERR_NOTCONV, "mixed solve: tolerance not reached");
+}
+
+int mppd_msolver_adjoint(mppd_msolver *m, int n, const double *eps_h, const mppd_mixed_params *p, double *sig_h,
+                         double *ctan_h, double *dmat_h, int stats[2], double *rel_h) {
+  if (!m || !p || !ctan_h || n < 0 || 6 + n > m->K || (n > 0 && (!eps_h || !sig_h || !dmat_h)))
+    return fail(MPPD_ERR_ARG, "bad msolver_adjoint arguments (needs 6 + n <= K)");
+  const Grid &g = m->r->g;
+  const int K = m->K;
+  int rc = MPPD_OK;
+  if (!m->F) {
+    if ((rc = dalloc(&m->F, mixed_vector_len(g, K)))) return rc;
+    if ((rc = dalloc(&m->G, 3 * 24 * 6))) return rc;
+    if ((rc = dalloc(&m->apart, mixed_adj_partials(g, K - 6)))) return rc;
+    if ((rc = dalloc(&m->adj_d, 36 * (size_t)(K - 5)))) return rc;
+    MPP_HIP(hipHostMalloc((void **)&m->adj_h, 36 * (K - 5) * sizeof(double), hipHostMallocDefault));
+  }
+  std::vector<double> pn, pn0;
+  int all_conv = 0;
+  if ((rc = msolver_run(m, 6, n, eps_h, p, stats, pn, pn0, &all_conv))) return rc;
+  MPP_HIP(launch_m_adj(g, K, m->r->bmat, m->mats_d, m->r->et, m->U, 6, n, m->apart, m->adj_d, m->adj_d + 36,
+                       m->st));
+  MPP_HIP(hipMemcpyAsync(m->adj_h, m->adj_d, 36 * (n + 1) * sizeof(double), hipMemcpyDeviceToHost, m->st));
+  if ((rc = msolver_averages(m))) return rc;  // synchronises the stream
+  for (int i = 0; i < 36; ++i) ctan_h[i] = m->adj_h[i];
+  for (int i = 0; i < 36 * n; ++i) dmat_h[i] = m->adj_h[36 + i];
+  for (int i = 0; i < 6 * n; ++i) sig_h[i] = m->sig_h[36 + i];
+  if (rel_h)
+    for (int i = 0; i < 6 + n; ++i) rel_h[i] = pn0[i] > 0 ? pn[i] / pn0[i] : 0.0;
+  return all_conv ? MPPD_OK : fail(MPPD_ERR_NOTCONV, "adjoint solve: tolerance not reached");
 }
 
 int mppd_msolver_get_u(mppd_msolver *m, int k, double *u_h) {
@@ -705,7 +767,7 @@ int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *it
   MPP_HIP(hipMemcpyAsync(m->eps_d, m->eps_h, K * 6 * sizeof(double), hipMemcpyHostToDevice, m->st));
   MPP_HIP(hipMemsetAsync(m->U, 0, mixed_vector_len(g, K) * sizeof(double), m->st));
   MPP_HIP(launch_m_setbc(g, K, m->eps_d, m->U, m->st));
-  MPP_HIP(launch_m_resid(g, K, m->r->et, m->ke, m->U, m->kinv, m->R, m->X, m->P, m->part, 1, 0.0, 0.0, 0.0,
+  MPP_HIP(launch_m_resid(g, K, m->r->et, m->ke, m->U, nullptr, m->kinv, m->R, m->X, m->P, m->part, 1, 0.0, 0.0, 0.0,
                          iters + 1, m->state, m->st));
   hipEvent_t ev[3];
   for (auto &x : ev) MPP_HIP(hipEventCreate(&x));

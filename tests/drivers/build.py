@@ -1,0 +1,46 @@
+"""Build the C and Fortran drivers of the micropp3_* interface into tests/drivers/bin/.
+
+c_driver.c (gcc) and f_driver.f90 (ROCm flang, when present) link
+micropp_amd/libmicropp_amd.so with an rpath relative to the binary, so the
+built tree can move.  Run after the library is built:
+
+    python tests/drivers/build.py
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+BIN = os.path.join(HERE, "bin")
+LIBDIR = os.path.join(ROOT, "micropp_amd")
+RPATH = "-Wl,-rpath,$ORIGIN/../../../micropp_amd"
+
+
+def flang():
+    for c in ("/opt/rocm/bin/amdflang", "/opt/rocm/llvm/bin/flang", shutil.which("flang") or ""):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build() -> dict:
+    os.makedirs(BIN, exist_ok=True)
+    out = {}
+    cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-Werror", f"-I{ROOT}/include", "-o", os.path.join(BIN, "c_driver"),
+           os.path.join(HERE, "c_driver.c"), f"-L{LIBDIR}", "-lmicropp_amd", RPATH]
+    subprocess.run(cmd, check=True)
+    out["c"] = os.path.join(BIN, "c_driver")
+    fc = flang()
+    if fc:
+        cmd = [fc, "-O2", "-o", os.path.join(BIN, "f_driver"), os.path.join(HERE, "f_driver.f90"),
+               f"-module-dir", BIN, f"-L{LIBDIR}", "-lmicropp_amd", RPATH]
+        subprocess.run(cmd, check=True)
+        out["fortran"] = os.path.join(BIN, "f_driver")
+    return out
+
+
+if __name__ == "__main__":
+    print(build())

@@ -85,3 +85,18 @@ def test_no_oracle_in_product():
             if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower(), f
+
+
+def test_c_and_fortran_callers_build(lib, tmp_path):
+    """The C header compiles as C99 and links; a Fortran iso_c_binding caller of micropp3_* links."""
+    import shutil
+
+    libdir = os.path.join(ROOT, "micropp_amd")
+    drv = os.path.join(ROOT, "tests", "drivers")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", f"-I{INC}", "-o", str(tmp_path / "c_driver"),
+                    os.path.join(drv, "c_driver.c"), f"-L{libdir}", "-lmicropp_amd"], check=True)
+    fc = next((c for c in ("/opt/rocm/bin/amdflang", shutil.which("flang") or "") if c and os.path.exists(c)), None)
+    if fc is None:
+        pytest.skip("no Fortran compiler")
+    subprocess.run([fc, "-o", str(tmp_path / "f_driver"), os.path.join(drv, "f_driver.f90"), "-module-dir",
+                    str(tmp_path), f"-L{libdir}", "-lmicropp_amd"], check=True)
