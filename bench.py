@@ -349,7 +349,7 @@ def main():
                    "outer_steps": stats[0], "inner_iterations": stats[1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": f"k_m_spmv<{K}> (FP32 node-block SpMV, {K} RHS, + p.Ap)",
+                     "kernel": f"k_m_spmv<{K}> (FP32 node-block SpMV, {K} RHS in 4-field groups, + p.Ap)",
                      "bytes_per_launch": spmv_bytes, "avg_ms": spmv_ms, "inner_iteration_ms": iter_ms},
     }
 

@@ -140,8 +140,9 @@ size_t mixed_matrix_floats(const Grid &g);
 size_t mixed_vector_len(const Grid &g, int K);
 size_t mixed_partials(const Grid &g, int K);
 size_t mixed_state_bytes();
-hipError_t launch_m_asm32(const Grid &g, const int *et, const double *ke, int phase_mode, float *A, float *kinv,
-                          hipStream_t st);
+// FP32 matrix A D^-1 and the Jacobi weights (scratch64: 3 NP doubles)
+hipError_t launch_m_asm32(const Grid &g, const int *et, const double *ke, int phase_mode, double *scratch64, float *A,
+                          float *kinv, hipStream_t st);
 hipError_t launch_m_setbc(const Grid &g, int K, const double *eps, double *U, hipStream_t st);
 // F: adjoint sources (FP64, the layout of U) or null
 hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke, const double *U, const double *F,
@@ -156,8 +157,9 @@ hipError_t launch_m_adj_src(const Grid &g, int K, int nadj, const double *bmat, 
 size_t mixed_adj_partials(const Grid &g, int ns);
 hipError_t launch_m_adj(const Grid &g, int K, const double *bmat, const material_base *mats_d, const int *et,
                         const double *U, int s0, int ns, double *part, double *ctan, double *dmat, hipStream_t st);
-hipError_t launch_m_iter(const Grid &g, int K, const float *A, float *X, float *R, float *P, float *AP,
-                         const float *kinv, double *part, void *state, int parity, hipStream_t st);
+// one inner CG iteration; R X P AP in the 4-field-group layout
+hipError_t launch_m_iter(const Grid &g, int K, const float *A, const float *kinv, float *R, float *X, float *P,
+                         float *AP, double *part, void *state, int parity, hipStream_t st);
 hipError_t launch_m_spmv_only(const Grid &g, int K, const float *A, const float *P, float *AP, double *part,
                               const void *state, hipStream_t st);
 hipError_t launch_m_accum(const Grid &g, int K, const float *X, double *U, hipStream_t st);

@@ -9,6 +9,11 @@
 // comes from an FP32 Jacobi-preconditioned CG on all K systems at once: one
 // pass over the FP32 matrix serves K vectors.  The volume averages are FP64.
 //
+// The inner CG keeps the reference's recurrence (alpha from (p, Ap) computed
+// directly: the single-reduction Chronopoulos-Gear form, tried here, loses
+// that denominator to cancellation in FP32 after ~150 iterations and
+// diverges), with the stopping test on ||z||, the reference CG's metric.
+//
 // FP32 matrix layout (node-blocked SELL-64): the 27 neighbour blocks of 3x3
 // coefficients of a node (slot s = nb*9 + fi*3 + fj, nb as in the reference
 // ELL) are padded to 244 floats and stored as 61 float4 per node, quad-major
@@ -17,11 +22,12 @@
 // 27-point stencil).  Rows and columns of Dirichlet nodes are dropped (the CG
 // vectors are zero there), which leaves the symmetric interior block.
 //
-// Vectors are component-major: component c = d * K + k (DOF direction d of
-// field k) of node n sits at c * NP + n, NP = nodes rounded up to 256.  A
-// wave of the SpMV owns 64 consecutive nodes and 4 fields, so every matrix
-// load and every neighbour gather is a contiguous 256-byte line.  The FP64
-// displacements U use the same layout.
+// FP32 vectors group 4 fields: float4 (d * K/4 + g) * NP + n holds fields
+// 4g .. 4g+3 of DOF direction d at node n (NP = nodes rounded up to 256).  A
+// wave of the SpMV owns 64 consecutive nodes and one field group, so every
+// matrix load and every neighbour gather is one contiguous 1 KB line.  The
+// FP64 displacements U are per field: component c = d * K + k of node n at
+// c * NP + n.
 //
 // Reductions are fence-free: blocks write FP64 partials in a [row][block]
 // table and a one-block kernel re-sums each row in a fixed order.  Iteration
@@ -36,10 +42,13 @@ namespace mpp {
 
 constexpr int kMaxRhs = 16;
 constexpr int kNodeQuads = 61;  // 244 floats per node (243 coefficients + 1 pad)
-constexpr int kRedThreads = 128;  // threads of the one-block reducers
+constexpr int kRedThreads = 256;  // threads of the one-block reducers
+constexpr int kVecNodes = 1024;   // nodes per block of the vector kernels
 
+// inner CG scalars per field: rz = (r, z), alpha / beta of the iteration, pn
+// the current ||z||, pn0 the start value, tau the relative stop
 struct MState {
-  double rz[kMaxRhs], pn[kMaxRhs], pn0[kMaxRhs], tau[kMaxRhs], alpha[kMaxRhs], beta[kMaxRhs];
+  double rz[kMaxRhs], alpha[kMaxRhs], beta[kMaxRhs], pn[kMaxRhs], pn0[kMaxRhs], tau[kMaxRhs];
   int done[kMaxRhs];
   int its, max_its, all_done, pad;
 };
@@ -63,18 +72,18 @@ __device__ __forceinline__ int xcd_block(int per) {
   return (b & 7) * per + (b >> 3);
 }
 
-// Row sums of a partial table part[row * stride + j], j < n, rows < NR
-// (NR <= 32), by one block of kRedThreads: every thread accumulates its
-// entries j = t, t + kRedThreads, ... of all rows (independent loads, all in
-// flight), the per-thread sums go through LDS, 4 threads per row add 32 of
-// them each, and thread t < NR adds its row's 4.  Fixed order throughout.
-// Thread t < NR returns the sum of row t (others return 0).
+// Row sums of a partial table part[row * stride + j], j < n, rows < NR, by
+// one block of kRedThreads: every thread accumulates its entries j = t,
+// t + kRedThreads, ... of all rows (independent loads, all in flight), each
+// wave adds its lanes with DPP moves, and thread t < NR adds the waves' sums
+// of row t.  Fixed order throughout.  Thread t < NR returns the sum of row t
+// (others return 0).
 template <int NR>
 __device__ __forceinline__ double rows_sum(const double *part, size_t stride, int n) {
-  static_assert(NR * 4 <= kRedThreads && kRedThreads == 128, "one block of 128 threads, NR <= 32");
-  __shared__ double lds[NR][kRedThreads];
-  __shared__ double l2[NR][4];
-  const int t = threadIdx.x;
+  static_assert(NR <= kRedThreads, "one row per thread at the end");
+  constexpr int kW = kRedThreads / 64;
+  __shared__ double lds[kW][NR];
+  const int t = threadIdx.x, w = t >> 6;
   double s[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) s[q] = 0.0;
@@ -83,78 +92,95 @@ __device__ __forceinline__ double rows_sum(const double *part, size_t stride, in
     for (int q = 0; q < NR; ++q) s[q] += part[(size_t)q * stride + j];
   __syncthreads();  // lds may still be read by a previous call
 #pragma unroll
-  for (int q = 0; q < NR; ++q) lds[q][t] = s[q];
-  __syncthreads();
-  if (t < NR * 4) {
-    const int q = t >> 2, sub = t & 3;
-    double a = 0.0;
-    for (int i = 0; i < kRedThreads / 4; ++i) a += lds[q][sub + 4 * i];
-    l2[q][sub] = a;
+  for (int q = 0; q < NR; ++q) {
+    const double v = wave_sum_dpp(s[q]);
+    if ((t & 63) == 0) lds[w][q] = v;
   }
   __syncthreads();
   double r = 0.0;
   if (t < NR)
-    for (int sub = 0; sub < 4; ++sub) r += l2[t][sub];
+    for (int i = 0; i < kW; ++i) r += lds[i][t];
   return r;
 }
 
 // ---------------------------------------------------------------------------
-// FP32 matrix from the element blocks (phase table or per element), and the
-// FP32 Jacobi weights kinv[d * NP + n].  Per slot, the element contributions
-// are summed in FP64 in the reference element order, then rounded once.
+// The element blocks (phase table or per element) around interior node n:
+// blk[q] the 24x24 block of element q = (a*2+b)*2+c, la[q] the node's local
+// index in it.  Entry (fi, fj) of the node block towards neighbour nb is the
+// FP64 sum of the element contributions in the reference element order.
 // ---------------------------------------------------------------------------
+struct NodeBlocks {
+  const double *blk[8];
+  int la[8];
+  __device__ NodeBlocks(const Grid &g, int i, int j, int k, const int *et, const double *ke, int phase_mode) {
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 2; ++c) {
+          const int q = (a * 2 + b) * 2 + c;
+          const int e = ((k - 1 + c) * g.ney + (j - 1 + b)) * g.nex + (i - 1 + a);
+          blk[q] = ke + (size_t)(phase_mode ? et[e] : e) * 576;
+          la[q] = local_node(1 - a, 1 - b, 1 - c);
+        }
+  }
+  __device__ double entry(int nb, int fi, int fj) const {
+    const int dz = nb / 9 - 1, dy = (nb / 3) % 3 - 1, dx = nb % 3 - 1;
+    double acc = 0.0;
+    for (int a = 0; a < 2; ++a) {
+      const int px = 1 - a + dx;
+      if (px < 0 || px > 1) continue;
+      for (int b = 0; b < 2; ++b) {
+        const int py = 1 - b + dy;
+        if (py < 0 || py > 1) continue;
+        for (int c = 0; c < 2; ++c) {
+          const int pz = 1 - c + dz;
+          if (pz < 0 || pz > 1) continue;
+          const int q = (a * 2 + b) * 2 + c;
+          acc += blk[q][(la[q] * 3 + fi) * 24 + local_node(px, py, pz) * 3 + fj];
+        }
+      }
+    }
+    return acc;
+  }
+};
+
+// Jacobi weights: kinv64[d * NP + n] = 1 / A_nn(d, d) (FP64, zero on
+// Dirichlet DOFs and padding) and its FP32 copy kinv
+__global__ void __launch_bounds__(256) k_m_kinv(Grid g, int NP, const int *et, const double *ke, int phase_mode,
+                                                double *kinv64, float *kinv) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= NP) return;
+  double w[3] = {0.0, 0.0, 0.0};
+  if (n < g.nn) {
+    const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
+    if (!node_on_boundary(g, i, j, k)) {
+      const NodeBlocks B(g, i, j, k, et, ke, phase_mode);
+      for (int d = 0; d < 3; ++d) w[d] = 1.0 / B.entry(13, d, d);
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    kinv64[(size_t)d * NP + n] = w[d];
+    kinv[(size_t)d * NP + n] = (float)w[d];
+  }
+}
+
+// FP32 matrix: slot (nb, fi, fj) of node n = A(n fi, m fj) for neighbour m,
+// rounded once; rows and columns of Dirichlet DOFs are zero
 __global__ void __launch_bounds__(256) k_m_asm32(Grid g, int NP, const int *et, const double *ke, int phase_mode,
-                                                 float *A, float *kinv) {
+                                                 float *A) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= NP) return;
   float *base = A + ((size_t)(n >> 6) * kNodeQuads * 64 + (n & 63)) * 4;
   auto put = [&](int s, float v) { base[(size_t)(s >> 2) * 256 + (s & 3)] = v; };
   for (int s = 0; s < kNodeQuads * 4; ++s) put(s, 0.0f);
-  if (n >= g.nn) {
-    for (int d = 0; d < 3; ++d) kinv[(size_t)d * NP + n] = 0.0f;
-    return;
-  }
+  if (n >= g.nn) return;
   const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
-  if (node_on_boundary(g, i, j, k)) {
-    for (int d = 0; d < 3; ++d) {
-      put(13 * 9 + d * 3 + d, 1.0f);
-      kinv[(size_t)d * NP + n] = 0.0f;  // keeps z, p and Ap zero on Dirichlet DOFs
-    }
-    return;
-  }
-  const double *blk[8];
-  int la[8];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int c = 0; c < 2; ++c) {
-        const int q = (a * 2 + b) * 2 + c;
-        const int e = ((k - 1 + c) * g.ney + (j - 1 + b)) * g.nex + (i - 1 + a);
-        blk[q] = ke + (size_t)(phase_mode ? et[e] : e) * 576;
-        la[q] = local_node(1 - a, 1 - b, 1 - c);
-      }
+  if (node_on_boundary(g, i, j, k)) return;
+  const NodeBlocks B(g, i, j, k, et, ke, phase_mode);
   for (int nb = 0; nb < 27; ++nb) {
     const int dz = nb / 9 - 1, dy = (nb / 3) % 3 - 1, dx = nb % 3 - 1;
     if (node_on_boundary(g, i + dx, j + dy, k + dz)) continue;  // column of a Dirichlet DOF: dropped
     for (int fi = 0; fi < 3; ++fi)
-      for (int fj = 0; fj < 3; ++fj) {
-        double acc = 0.0;
-        for (int a = 0; a < 2; ++a) {
-          const int px = 1 - a + dx;
-          if (px < 0 || px > 1) continue;
-          for (int b = 0; b < 2; ++b) {
-            const int py = 1 - b + dy;
-            if (py < 0 || py > 1) continue;
-            for (int c = 0; c < 2; ++c) {
-              const int pz = 1 - c + dz;
-              if (pz < 0 || pz > 1) continue;
-              const int q = (a * 2 + b) * 2 + c;
-              acc += blk[q][(la[q] * 3 + fi) * 24 + local_node(px, py, pz) * 3 + fj];
-            }
-          }
-        }
-        put(nb * 9 + fi * 3 + fj, (float)acc);
-        if (nb == 13 && fi == fj) kinv[(size_t)fi * NP + n] = (float)(1.0 / acc);
-      }
+      for (int fj = 0; fj < 3; ++fj) put(nb * 9 + fi * 3 + fj, (float)B.entry(nb, fi, fj));
   }
 }
 
@@ -260,13 +286,14 @@ __global__ void __launch_bounds__(256) k_m_adj_src(Grid g, int NP, int nadj, con
 // FP64 residual r = F - (K u) on interior DOFs (elastic: element blocks of the
 // phase table; F = 0 unless adjoint sources are given), rounded into the FP32
 // CG start: R = r, X = 0, P = kinv R.
-// Partials: part[q * gridDim.x + block], q < K: (kinv r)^2 in FP64 of field
-// q; q >= K: R.P of field q - K.
+// Partials: part[q * gridDim.x + block], q < K: ||kinv r||^2 (FP64) of field
+// q; q >= K: (R, P) of field q - K.
 // ---------------------------------------------------------------------------
 template <int K>
 __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const int *et, const double *ke,
-                                                 const double *U, const double *F, const float *kinv, float *R,
-                                                 float *X, float *P, double *part) {
+                                                 const double *U, const double *F, const float *kinv, float4 *R,
+                                                 float4 *X, float4 *P, double *part) {
+  constexpr int KG = K / 4;
   __shared__ double ke_s[3 * 576];
   __shared__ double lds[4 * 2 * K];
   for (int t = threadIdx.x; t < 3 * 576; t += blockDim.x) ke_s[t] = ke[t];
@@ -313,19 +340,21 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const 
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const float kv = kinv[(size_t)d * NP + n];
+        float rr[4], pp[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const size_t idx = (size_t)(d * K + kg + q) * NP + n;
           const double r64 = (F ? F[idx] : 0.0) - acc[d][q];
-          const float r32 = (float)r64;
-          const float p32 = kv * r32;
-          R[idx] = r32;
-          X[idx] = 0.0f;
-          P[idx] = p32;
+          rr[q] = (float)r64;
+          pp[q] = kv * rr[q];
           const double z64 = (double)kv * r64;
           zz[kg + q] += z64 * z64;
-          rz[kg + q] += (double)r32 * (double)p32;
+          rz[kg + q] += (double)rr[q] * (double)pp[q];
         }
+        const size_t o = (size_t)(d * KG + kg / 4) * NP + n;
+        R[o] = make_float4(rr[0], rr[1], rr[2], rr[3]);
+        X[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P[o] = make_float4(pp[0], pp[1], pp[2], pp[3]);
       }
     }
   }
@@ -347,61 +376,74 @@ __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const doubl
                                                          double abs_tol, double tau_min, int max_inner, MOuter *o,
                                                          MState *s0) {
   __shared__ double t[2 * kMaxRhs];
+  __shared__ int conv_s[kMaxRhs];
   const double s = rows_sum<2 * K>(part, nparts, nparts);
   if (threadIdx.x < 2 * K) t[threadIdx.x] = s;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  int all = 1;
-#pragma unroll
-  for (int r = 0; r < K; ++r) {
+  // one thread per field; every read of the old state comes before any write
+  const int r = threadIdx.x;
+  if (r < K) {
     const double pn = sqrt(t[r]);
+    const double target = first ? fmax(abs_tol, rel * pn) : o->target[r];
+    const int conv = pn < target || pn == 0.0;
+    conv_s[r] = conv;
     if (first) {
       o->pn0[r] = pn;
-      o->target[r] = fmax(abs_tol, rel * pn);
+      o->target[r] = target;
     }
     o->pn[r] = pn;
-    const int conv = pn < o->target[r] || pn == 0.0;
     o->conv[r] = conv;
-    all &= conv;
     s0->rz[r] = t[K + r];
     s0->pn[r] = pn;
     s0->pn0[r] = pn;
     // aim a little below what the outer test needs, but not below what FP32 can resolve
-    s0->tau[r] = (pn > 0.0) ? fmax(tau_min, 0.3 * o->target[r] / pn) : 1.0;
+    s0->tau[r] = (pn > 0.0) ? fmax(tau_min, 0.3 * target / pn) : 1.0;
     s0->alpha[r] = 0.0;
     s0->beta[r] = 0.0;
     s0->done[r] = conv;
   }
-  o->all_conv = all;
-  s0->its = 0;
-  s0->max_its = max_inner;
-  s0->all_done = all;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int all = 1;
+    for (int q = 0; q < K; ++q) all &= conv_s[q];
+    o->all_conv = all;
+    s0->its = 0;
+    s0->max_its = max_inner;
+    s0->all_done = all;
+  }
 }
 
 // ---------------------------------------------------------------------------
-// inner CG, step 1: AP = A P and the partials of P.AP (FP64).  One block per
-// slice of 64 nodes; wave w handles fields 4w .. 4w+3, so a block has K/4
-// waves.  Partial table part1[field * nsl + slice].
+// Inner CG (the reference's Jacobi-PCG recurrence, src/ell.cpp:80-117), one
+// iteration = k_m_spmv, k_m_alpha, k_m_upd, k_m_beta, k_m_pupd.
+//
+// k_m_spmv: AP = A P for every field and the partials of (p, Ap).  One block
+// per slice of 64 nodes; wave g handles field group g (fields 4g .. 4g+3), so
+// a block has K/4 waves.  Partial table part1[field * nsl + slice].
 // ---------------------------------------------------------------------------
 template <int K>
-__global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per, const float4 *A, const float *P,
-                                                float *AP, double *part1, const MState *cur) {
+__global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per, const float4 *A, const float4 *P,
+                                                float4 *AP, double *part1, const MState *cur) {
+  constexpr int KG = K / 4;
   if (cur->all_done) return;
   const int slice = xcd_block(per);
   if (slice >= nsl) return;
-  const int r0 = (threadIdx.x >> 6) * 4, lane = threadIdx.x & 63;
+  const int grp = threadIdx.x >> 6, lane = threadIdx.x & 63, r0 = grp * 4;
   if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole wave
   const int n = slice * 64 + lane;
-  float acc[3][4], pc[3][4];
+  float acc[3][4];
+  float4 pc[3];  // p at the node itself (neighbour 13)
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
+  for (int d = 0; d < 3; ++d) {
+    pc[d] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc[d][r] = pc[d][r] = 0.0f;
+    for (int r = 0; r < 4; ++r) acc[d][r] = 0.0f;
+  }
   if (n < g.nn) {
     const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
     const int nxny = g.nx * g.ny;
     const f4v *arow = reinterpret_cast<const f4v *>(A) + (size_t)slice * kNodeQuads * 64 + lane;
-    const float *pf = P + (size_t)r0 * NP;
+    const float4 *pg = P + (size_t)grp * NP;
     // 7 chunks of up to 4 neighbours = 9 quads each (the last: 3 neighbours,
     // 7 quads); the chunk loop stays rolled to bound the live registers
 #pragma unroll 1
@@ -416,7 +458,7 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
           a[q * 4 + 2] = v.z;
           a[q * 4 + 3] = v.w;
         }
-      float pv[4][3][4];
+      float4 pv[4][3];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int nb = ch * 4 + t;
@@ -425,15 +467,11 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
                         (k + dz >= 0 && k + dz < g.nz);
         const int m = in ? n + dz * nxny + dy * g.nx + dx : n;  // outside: zero block, any valid node
 #pragma unroll
-        for (int fj = 0; fj < 3; ++fj)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) pv[t][fj][r] = pf[(size_t)(fj * K + r) * NP + m];
+        for (int fj = 0; fj < 3; ++fj) pv[t][fj] = pg[(size_t)fj * KG * NP + m];
       }
-      if (ch == 3)  // neighbour 13 = ch 3, t 1 is the node itself: keep p for p.Ap
+      if (ch == 3)
 #pragma unroll
-        for (int fj = 0; fj < 3; ++fj)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) pc[fj][r] = pv[1][fj][r];
+        for (int fj = 0; fj < 3; ++fj) pc[fj] = pv[1][fj];
 #pragma unroll
       for (int t = 0; t < 4; ++t)
         if (ch * 4 + t < 27)
@@ -442,27 +480,30 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
 #pragma unroll
             for (int fi = 0; fi < 3; ++fi) {
               const float c = a[t * 9 + fi * 3 + fj];
-#pragma unroll
-              for (int r = 0; r < 4; ++r) acc[fi][r] += c * pv[t][fj][r];
+              acc[fi][0] += c * pv[t][fj].x;
+              acc[fi][1] += c * pv[t][fj].y;
+              acc[fi][2] += c * pv[t][fj].z;
+              acc[fi][3] += c * pv[t][fj].w;
             }
     }
   }
-  float *apf = AP + (size_t)r0 * NP + n;
+  double pap[4] = {0, 0, 0, 0};
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) apf[(size_t)(d * K + r) * NP] = acc[d][r];
+  for (int d = 0; d < 3; ++d) {
+    AP[(size_t)(d * KG + grp) * NP + n] = make_float4(acc[d][0], acc[d][1], acc[d][2], acc[d][3]);
+    pap[0] += (double)pc[d].x * acc[d][0];
+    pap[1] += (double)pc[d].y * acc[d][1];
+    pap[2] += (double)pc[d].z * acc[d][2];
+    pap[3] += (double)pc[d].w * acc[d][3];
+  }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    double s = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) s += (double)pc[d][r] * (double)acc[d][r];
-    s = wave_sum(s);
+    const double s = wave_sum_dpp(pap[r]);
     if (lane == 0) part1[(size_t)(r0 + r) * nsl + slice] = s;
   }
 }
 
-// step 2 (one block): alpha = rz / p.Ap of every live field
+// k_m_alpha (one block): alpha = (r, z) / (p, Ap) of every live field
 template <int K>
 __global__ void __launch_bounds__(kRedThreads) k_m_alpha(int nsl, const double *part1, MState *cur) {
   if (cur->all_done) return;
@@ -471,46 +512,70 @@ __global__ void __launch_bounds__(kRedThreads) k_m_alpha(int nsl, const double *
   if (f < K && !cur->done[f]) cur->alpha[f] = cur->rz[f] / pap;
 }
 
-// step 3: X += alpha P; R -= alpha AP; partials of z.z and R.z (z = kinv R).
-// Grid (float4 chunks of a component, component c = d * K + k); partial
-// tables pz / pr [(k * 3 + d) * gridDim.x + block], so the 3 * gridDim.x
-// partials of field k are contiguous; pr follows pz (pr = pz + 3 K nbx).
+// k_m_upd: X += alpha P; R -= alpha AP; partials of (z, z) and (r, z) with
+// z = kinv R.  Grid (blocks of kVecNodes nodes, 3 K/4 components
+// c = d * K/4 + g), each thread kVecNodes / 256 nodes; partial tables pz / pr
+// [(field * 3 + d) * gridDim.x + block] (the 3 gridDim.x partials of a field
+// contiguous; pr = pz + 3 K gridDim.x).
 template <int K>
-__global__ void __launch_bounds__(256) k_m_upd(int NP, const float4 *P, const float4 *AP, const float4 *kinv, float4 *X,
-                                               float4 *R, double *pz, double *pr, const MState *cur) {
-  __shared__ double lds[4 * 2];
-  const int c = blockIdx.y, f = c % K, d = c / K;
-  if (cur->all_done || cur->done[f]) return;  // whole block
-  const float al = (float)cur->alpha[f];
-  const int q = blockIdx.x * 256 + threadIdx.x, nq = NP / 4;
-  double v[2] = {0.0, 0.0};
-  if (q < nq) {
-    const size_t o = (size_t)c * nq + q;
-    const float4 p = P[o], ap = AP[o], kv = kinv[(size_t)d * nq + q];
+__global__ void __launch_bounds__(256) k_m_upd(int NP, const float *kinv, const float4 *P, const float4 *AP,
+                                               float4 *X, float4 *R, double *pz, double *pr, const MState *cur) {
+  constexpr int KG = K / 4;
+  __shared__ double lds[4 * 8];
+  const int c = blockIdx.y, grp = c % KG, d = c / KG, r0 = grp * 4;
+  if (cur->all_done) return;
+  if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole block
+  const float4 al = make_float4(cur->done[r0] ? 0.0f : (float)cur->alpha[r0],
+                                cur->done[r0 + 1] ? 0.0f : (float)cur->alpha[r0 + 1],
+                                cur->done[r0 + 2] ? 0.0f : (float)cur->alpha[r0 + 2],
+                                cur->done[r0 + 3] ? 0.0f : (float)cur->alpha[r0 + 3]);
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int jn = 0; jn < kVecNodes / 256; ++jn) {
+    const int n = blockIdx.x * kVecNodes + jn * 256 + threadIdx.x;
+    if (n >= NP) break;
+    const size_t o = (size_t)c * NP + n;
+    const float kv = kinv[(size_t)d * NP + n];
+    const float4 p = P[o], ap = AP[o];
     float4 x = X[o], r = R[o];
-    x.x += al * p.x;
-    x.y += al * p.y;
-    x.z += al * p.z;
-    x.w += al * p.w;
-    r.x -= al * ap.x;
-    r.y -= al * ap.y;
-    r.z -= al * ap.z;
-    r.w -= al * ap.w;
+    x.x += al.x * p.x;
+    x.y += al.y * p.y;
+    x.z += al.z * p.z;
+    x.w += al.w * p.w;
+    r.x -= al.x * ap.x;
+    r.y -= al.y * ap.y;
+    r.z -= al.z * ap.z;
+    r.w -= al.w * ap.w;
     X[o] = x;
     R[o] = r;
-    const float z0 = kv.x * r.x, z1 = kv.y * r.y, z2 = kv.z * r.z, z3 = kv.w * r.w;
-    v[0] = (double)z0 * z0 + (double)z1 * z1 + (double)z2 * z2 + (double)z3 * z3;
-    v[1] = (double)r.x * z0 + (double)r.y * z1 + (double)r.z * z2 + (double)r.w * z3;
+    const float rv[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float z = kv * rv[q];
+      v[q] += (double)z * z;
+      v[4 + q] += (double)rv[q] * z;
+    }
   }
-  block_sum<2, 256>(v, lds);
-  if (threadIdx.x == 0) {
-    const size_t o = ((size_t)f * 3 + d) * gridDim.x + blockIdx.x;
-    pz[o] = v[0];
-    pr[o] = v[1];
+  // block sums: DPP within each wave, then the 4 waves in order
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double w = wave_sum_dpp(v[q]);
+    if ((threadIdx.x & 63) == 0) lds[(threadIdx.x >> 6) * 8 + q] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int q = threadIdx.x;
+    const double t = lds[q] + lds[8 + q] + lds[16 + q] + lds[24 + q];
+    const size_t o = ((size_t)(r0 + (q & 3)) * 3 + d) * gridDim.x + blockIdx.x;
+    if (q < 4)
+      pz[o] = t;
+    else
+      pr[o] = t;
   }
 }
 
-// step 4 (one block): beta, stopping test, the next state
+// k_m_beta (one block): beta, the stopping test on ||z|| (src/ell.cpp:95),
+// the next state
 template <int K>
 __global__ void __launch_bounds__(kRedThreads) k_m_beta(int nbx, const double *pz, const MState *cur, MState *nxt) {
   __shared__ double zz_s[kMaxRhs], rz_s[kMaxRhs];
@@ -520,58 +585,84 @@ __global__ void __launch_bounds__(kRedThreads) k_m_beta(int nbx, const double *p
     for (int q = threadIdx.x; q < (int)(sizeof(MState) / sizeof(int)); q += kRedThreads) t[q] = s[q];
     return;
   }
-  // rows 0..K-1: z.z of each field; rows K..2K-1: R.z (the pr table follows pz)
+  // rows 0..K-1: z.z of each field; rows K..2K-1: r.z (the pr table follows pz)
   const double s = rows_sum<2 * K>(pz, (size_t)3 * nbx, 3 * nbx);
   if (threadIdx.x < K)
     zz_s[threadIdx.x] = s;
   else if (threadIdx.x < 2 * K)
     rz_s[threadIdx.x - K] = s;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  int all = 1;
-#pragma unroll
-  for (int r = 0; r < K; ++r) {
+  // one thread per field; every read of the current state comes before any write
+  __shared__ int dn_s[kMaxRhs];
+  const int r = threadIdx.x;
+  const int its = cur->its + 1, max_its = cur->max_its;
+  if (r < K) {
     const int was = cur->done[r];
-    const double pn = was ? cur->pn[r] : sqrt(zz_s[r]);
-    const int dn = was || pn < cur->tau[r] * cur->pn0[r] || !(pn == pn);
-    nxt->rz[r] = was ? cur->rz[r] : rz_s[r];
-    nxt->beta[r] = was ? 0.0 : rz_s[r] / cur->rz[r];
-    nxt->alpha[r] = cur->alpha[r];
+    const double rz_old = cur->rz[r], pn_old = cur->pn[r], pn0 = cur->pn0[r], tau = cur->tau[r];
+    const double alpha = cur->alpha[r];
+    const double pn = was ? pn_old : sqrt(zz_s[r]);
+    const int dn = was || pn < tau * pn0 || !(pn == pn);
+    dn_s[r] = dn;
+    nxt->rz[r] = was ? rz_old : rz_s[r];
+    nxt->beta[r] = was ? 0.0 : rz_s[r] / rz_old;
+    nxt->alpha[r] = alpha;
     nxt->pn[r] = pn;
-    nxt->pn0[r] = cur->pn0[r];
-    nxt->tau[r] = cur->tau[r];
+    nxt->pn0[r] = pn0;
+    nxt->tau[r] = tau;
     nxt->done[r] = dn;
-    all &= dn;
   }
-  const int its = cur->its + 1;
-  nxt->its = its;
-  nxt->max_its = cur->max_its;
-  nxt->all_done = all || its >= cur->max_its;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int all = 1;
+    for (int q = 0; q < K; ++q) all &= dn_s[q];
+    nxt->its = its;
+    nxt->max_its = max_its;
+    nxt->all_done = all || its >= max_its;
+  }
 }
 
-// step 5: P = kinv R + beta P for the fields still running
+// k_m_pupd: P = kinv R + beta P for the field groups with a live field
+// (stopped fields keep P).  Grid as k_m_upd.
 template <int K>
-__global__ void __launch_bounds__(256) k_m_pupd(int NP, const float4 *R, const float4 *kinv, float4 *P,
+__global__ void __launch_bounds__(256) k_m_pupd(int NP, const float *kinv, const float4 *R, float4 *P,
                                                 const MState *nxt) {
-  const int c = blockIdx.y, f = c % K, d = c / K;
-  if (nxt->all_done || nxt->done[f]) return;
-  const float be = (float)nxt->beta[f];
-  const int q = blockIdx.x * 256 + threadIdx.x, nq = NP / 4;
-  if (q >= nq) return;
-  const size_t o = (size_t)c * nq + q;
-  const float4 r = R[o], kv = kinv[(size_t)d * nq + q];
-  float4 p = P[o];
-  p.x = kv.x * r.x + be * p.x;
-  p.y = kv.y * r.y + be * p.y;
-  p.z = kv.z * r.z + be * p.z;
-  p.w = kv.w * r.w + be * p.w;
-  P[o] = p;
+  constexpr int KG = K / 4;
+  const int c = blockIdx.y, grp = c % KG, d = c / KG, r0 = grp * 4;
+  if (nxt->all_done) return;
+  if (nxt->done[r0] && nxt->done[r0 + 1] && nxt->done[r0 + 2] && nxt->done[r0 + 3]) return;  // whole block
+  const float be[4] = {(float)nxt->beta[r0], (float)nxt->beta[r0 + 1], (float)nxt->beta[r0 + 2],
+                       (float)nxt->beta[r0 + 3]};
+  const int dn[4] = {nxt->done[r0], nxt->done[r0 + 1], nxt->done[r0 + 2], nxt->done[r0 + 3]};
+#pragma unroll
+  for (int jn = 0; jn < kVecNodes / 256; ++jn) {
+    const int n = blockIdx.x * kVecNodes + jn * 256 + threadIdx.x;
+    if (n >= NP) break;
+    const size_t o = (size_t)c * NP + n;
+    const float kv = kinv[(size_t)d * NP + n];
+    const float4 r = R[o];
+    float4 p = P[o];
+    if (!dn[0]) p.x = kv * r.x + be[0] * p.x;
+    if (!dn[1]) p.y = kv * r.y + be[1] * p.y;
+    if (!dn[2]) p.z = kv * r.z + be[2] * p.z;
+    if (!dn[3]) p.w = kv * r.w + be[3] * p.w;
+    P[o] = p;
+  }
 }
 
-// U += X (FP64 update of the Newton / defect-correction step)
-__global__ void __launch_bounds__(256) k_m_accum(size_t n, const float *X, double *U) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) U[t] += (double)X[t];
+// U += X (FP64 update of the defect-correction step): grid (node blocks,
+// 3 K/4 components); field 4g + q of component (d, g) is U row d * K + 4g + q
+template <int K>
+__global__ void __launch_bounds__(256) k_m_accum(int NP, const float4 *X, double *U) {
+  constexpr int KG = K / 4;
+  const int c = blockIdx.y, grp = c % KG, d = c / KG;
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= NP) return;
+  const float4 x = X[(size_t)c * NP + n];
+  double *u = U + (size_t)(d * K + grp * 4) * NP + n;
+  u[0] += (double)x.x;
+  u[(size_t)NP] += (double)x.y;
+  u[(size_t)2 * NP] += (double)x.z;
+  u[(size_t)3 * NP] += (double)x.w;
 }
 
 // ---------------------------------------------------------------------------
@@ -831,15 +922,16 @@ size_t mixed_partials(const Grid &g, int K) {
   const MLaunch s = mshape(g);
   const size_t resid = (size_t)s.nb * 2 * K;
   const size_t ave = (size_t)nblk(g.nelem, 256) * 6 * K;
-  const size_t iter = (size_t)K * s.nsl + 2 * (size_t)3 * K * s.nbx;
+  const size_t iter = (size_t)K * s.nsl + 2 * (size_t)3 * K * (s.NP / 256);
   return std::max(std::max(resid, ave), iter);
 }
 size_t mixed_state_bytes() { return 2 * sizeof(MState) + sizeof(MOuter); }
 
-hipError_t launch_m_asm32(const Grid &g, const int *et, const double *ke, int phase_mode, float *A, float *kinv,
-                          hipStream_t st) {
+hipError_t launch_m_asm32(const Grid &g, const int *et, const double *ke, int phase_mode, double *scratch64, float *A,
+                          float *kinv, hipStream_t st) {
   const MLaunch s = mshape(g);
-  hipLaunchKernelGGL(k_m_asm32, dim3(s.NP / 256), dim3(256), 0, st, g, s.NP, et, ke, phase_mode, A, kinv);
+  hipLaunchKernelGGL(k_m_kinv, dim3(s.NP / 256), dim3(256), 0, st, g, s.NP, et, ke, phase_mode, scratch64, kinv);
+  hipLaunchKernelGGL(k_m_asm32, dim3(s.NP / 256), dim3(256), 0, st, g, s.NP, et, ke, phase_mode, A);
   return hipGetLastError();
 }
 
@@ -879,7 +971,8 @@ hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke,
   MState *ms = static_cast<MState *>(state);
   MOuter *mo = reinterpret_cast<MOuter *>(ms + 2);
   MPP_K_DISPATCH(K, {
-    hipLaunchKernelGGL(k_m_resid<KK>, dim3(s.nb), dim3(256), 0, st, g, s.NP, s.per, et, ke, U, F, kinv, R, X, P,
+    hipLaunchKernelGGL(k_m_resid<KK>, dim3(s.nb), dim3(256), 0, st, g, s.NP, s.per, et, ke, U, F, kinv,
+                       reinterpret_cast<float4 *>(R), reinterpret_cast<float4 *>(X), reinterpret_cast<float4 *>(P),
                        part);
     hipLaunchKernelGGL(k_m_outer<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nb, part, first, rel, abs_tol, tau_min,
                        max_inner, mo, ms);
@@ -887,26 +980,28 @@ hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke,
   return hipGetLastError();
 }
 
-hipError_t launch_m_iter(const Grid &g, int K, const float *A, float *X, float *R, float *P, float *AP,
-                         const float *kinv, double *part, void *state, int parity, hipStream_t st) {
+hipError_t launch_m_iter(const Grid &g, int K, const float *A, const float *kinv, float *R, float *X, float *P,
+                         float *AP, double *part, void *state, int parity, hipStream_t st) {
   const MLaunch s = mshape(g);
   MState *ms = static_cast<MState *>(state);
   MState *cur = ms + (parity & 1);
   MState *nxt = ms + ((parity + 1) & 1);
-  double *part1 = part;                               // K x nsl
-  double *pz = part + (size_t)K * s.nsl;              // 3K x nbx
-  double *pr = pz + (size_t)3 * K * s.nbx;            // 3K x nbx
-  const dim3 vgrid(s.nbx, 3 * K);
+  const int nbx = (int)nblk(s.NP, kVecNodes);
+  double *part1 = part;                          // K x nsl
+  double *pz = part + (size_t)K * s.nsl;         // 3K x nbx
+  double *pr = pz + (size_t)3 * K * nbx;         // 3K x nbx
+  const dim3 vgrid(nbx, 3 * K / 4);
   MPP_K_DISPATCH(K, {
     hipLaunchKernelGGL(k_m_spmv<KK>, dim3(8 * s.pers), dim3(16 * KK), 0, st, g, s.NP, s.nsl, s.pers,
-                       reinterpret_cast<const float4 *>(A), P, AP, part1, cur);
+                       reinterpret_cast<const float4 *>(A), reinterpret_cast<const float4 *>(P),
+                       reinterpret_cast<float4 *>(AP), part1, cur);
     hipLaunchKernelGGL(k_m_alpha<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nsl, part1, cur);
-    hipLaunchKernelGGL(k_m_upd<KK>, vgrid, dim3(256), 0, st, s.NP, reinterpret_cast<const float4 *>(P),
-                       reinterpret_cast<const float4 *>(AP), reinterpret_cast<const float4 *>(kinv),
-                       reinterpret_cast<float4 *>(X), reinterpret_cast<float4 *>(R), pz, pr, cur);
-    hipLaunchKernelGGL(k_m_beta<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nbx, pz, cur, nxt);
-    hipLaunchKernelGGL(k_m_pupd<KK>, vgrid, dim3(256), 0, st, s.NP, reinterpret_cast<const float4 *>(R),
-                       reinterpret_cast<const float4 *>(kinv), reinterpret_cast<float4 *>(P), nxt);
+    hipLaunchKernelGGL(k_m_upd<KK>, vgrid, dim3(256), 0, st, s.NP, kinv, reinterpret_cast<const float4 *>(P),
+                       reinterpret_cast<const float4 *>(AP), reinterpret_cast<float4 *>(X),
+                       reinterpret_cast<float4 *>(R), pz, pr, cur);
+    hipLaunchKernelGGL(k_m_beta<KK>, dim3(1), dim3(kRedThreads), 0, st, nbx, pz, cur, nxt);
+    hipLaunchKernelGGL(k_m_pupd<KK>, vgrid, dim3(256), 0, st, s.NP, kinv, reinterpret_cast<const float4 *>(R),
+                       reinterpret_cast<float4 *>(P), nxt);
   });
   return hipGetLastError();
 }
@@ -916,13 +1011,15 @@ hipError_t launch_m_spmv_only(const Grid &g, int K, const float *A, const float 
   const MLaunch s = mshape(g);
   const MState *cur = static_cast<const MState *>(state);
   MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_spmv<KK>, dim3(8 * s.pers), dim3(16 * KK), 0, st, g, s.NP, s.nsl, s.pers,
-                                       reinterpret_cast<const float4 *>(A), P, AP, part, cur));
+                                       reinterpret_cast<const float4 *>(A), reinterpret_cast<const float4 *>(P),
+                                       reinterpret_cast<float4 *>(AP), part, cur));
   return hipGetLastError();
 }
 
 hipError_t launch_m_accum(const Grid &g, int K, const float *X, double *U, hipStream_t st) {
-  const size_t n = mixed_vector_len(g, K);
-  hipLaunchKernelGGL(k_m_accum, dim3(nblk(n, 256)), dim3(256), 0, st, n, X, U);
+  const MLaunch s = mshape(g);
+  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_accum<KK>, dim3(s.NP / 256, 3 * KK / 4), dim3(256), 0, st, s.NP,
+                                       reinterpret_cast<const float4 *>(X), U));
   return hipGetLastError();
 }
 

@@ -541,7 +541,7 @@ struct mppd_msolver {
   double *ke = nullptr;         // phase table 3 x 576 (FP64)
   material_base *mats_d = nullptr;
   float *A = nullptr, *kinv = nullptr;
-  float *X = nullptr, *R = nullptr, *P = nullptr, *AP = nullptr;
+  float *X = nullptr, *R = nullptr, *P = nullptr, *AP = nullptr;  // 4-field-group layout
   double *U = nullptr, *eps_d = nullptr, *part = nullptr, *sig_d = nullptr;
   void *state = nullptr, *state_h = nullptr;
   double *eps_h = nullptr, *sig_h = nullptr;  // pinned staging
@@ -588,7 +588,8 @@ int mppd_msolver_create(mppd_msolver **out, mppd_rve *r, int nrhs) {
     if (e == hipSuccess) e = hipMemcpy(m->mats_d, r->mats, 3 * sizeof(material_base), hipMemcpyHostToDevice);
     // element stiffness per phase (FP64, the plain path's kernel), then the FP32 node-block matrix
     if (e == hipSuccess) e = launch_elem_ke(g, r->bmat, r->mats_d, r->et, nullptr, nullptr, 1, m->ke, m->st);
-    if (e == hipSuccess) e = launch_m_asm32(g, r->et, m->ke, 1, m->A, m->kinv, m->st);
+    // (U serves as FP64 scratch for the Jacobi weights: 3 NP <= 3 K NP doubles)
+    if (e == hipSuccess) e = launch_m_asm32(g, r->et, m->ke, 1, m->U, m->A, m->kinv, m->st);
     if (e == hipSuccess) e = hipStreamSynchronize(m->st);
     if (e != hipSuccess) rc = hip_fail(e, "msolver set-up");
   } while (0);
@@ -658,7 +659,7 @@ static int msolver_run(mppd_msolver *m, int nadj, int n, const double *eps_h, co
       const int nq = std::min(chunk, p->max_inner_its - queued);
       if (nq <= 0) break;
       for (int i = 0; i < nq; ++i)
-        MPP_HIP(launch_m_iter(g, K, m->A, m->X, m->R, m->P, m->AP, m->kinv, m->part, m->state, queued + i, m->st));
+        MPP_HIP(launch_m_iter(g, K, m->A, m->kinv, m->R, m->X, m->P, m->AP, m->part, m->state, queued + i, m->st));
       queued += nq;
       MPP_HIP(hipMemcpyAsync(m->state_h, m->state, mixed_state_bytes(), hipMemcpyDeviceToHost, m->st));
       MPP_HIP(hipStreamSynchronize(m->st));
@@ -776,7 +777,7 @@ int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *it
     e = launch_m_spmv_only(g, K, m->A, m->P, m->AP, m->part, m->state, m->st);
   if (e == hipSuccess) e = hipEventRecord(ev[1], m->st);
   for (int i = 0; i < iters && e == hipSuccess; ++i)
-    e = launch_m_iter(g, K, m->A, m->X, m->R, m->P, m->AP, m->kinv, m->part, m->state, i, m->st);
+    e = launch_m_iter(g, K, m->A, m->kinv, m->R, m->X, m->P, m->AP, m->part, m->state, i, m->st);
   if (e == hipSuccess) e = hipEventRecord(ev[2], m->st);
   if (e == hipSuccess) e = hipStreamSynchronize(m->st);
   float ms1 = 0.f, ms2 = 0.f;
