@@ -43,9 +43,14 @@ __global__ void __launch_bounds__(256) k_cg_init_state(const double *part, int n
   cg_init_state_body(part, nparts, min_err, rel_err, max_its, state);
 }
 
-__global__ void __launch_bounds__(256) k_cg_step1(const int *cols, const double *vals, const double *p, double *Ap,
-                                                  double *part1, const CgState *cur) {
-  cg_step1_body(blockIdx.x, cols, vals, p, Ap, part1, cur);
+// Blocks are dispatched round-robin over the 8 XCDs: the 8 * per launched
+// blocks give XCD x the row blocks [x * per, (x + 1) * per), so each XCD's
+// L2 holds a contiguous slab of the gathered vector (2-5% faster SpMV).
+__global__ void __launch_bounds__(256) k_cg_step1(int nb, int per, const int *cols, const double *vals,
+                                                  const double *p, double *Ap, double *part1, const CgState *cur) {
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (lb >= nb) return;
+  cg_step1_body(lb, cols, vals, p, Ap, part1, cur);
 }
 
 __global__ void __launch_bounds__(256) k_cg_step2(double *x, double *r, double *z, const double *p, const double *Ap,
@@ -89,7 +94,8 @@ hipError_t launch_cg_init(const Grid &g, const double *vals, const double *b, do
 
 hipError_t launch_cg_step1(const Grid &g, const int *cols, const double *vals, const double *p, double *Ap,
                            double *part, CgState *state, int parity, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_step1, dim3(nblk(g.nrow_pad, 256)), dim3(256), 0, st, cols, vals, p, Ap, part,
+  const int nb = (int)nblk(g.nrow_pad, 256), per = (nb + 7) / 8;
+  hipLaunchKernelGGL(k_cg_step1, dim3(8 * per), dim3(256), 0, st, nb, per, cols, vals, p, Ap, part,
                      state + (parity & 1));
   return hipGetLastError();
 }
