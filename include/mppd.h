@@ -51,6 +51,8 @@ int mppd_memcpy_d2h(void *dst_h, const void *src_d, size_t bytes);
 int mppd_memcpy_d2d(void *dst_d, const void *src_d, size_t bytes);
 int mppd_memset(void *ptr_d, int value, size_t bytes);
 int mppd_synchronize(void);
+/* free and total device memory of the current device, bytes (hipMemGetInfo) */
+int mppd_mem_info(size_t *free_bytes, size_t *total_bytes);
 
 /* ---- RVE geometry ----
  * Replaces the geometric part of the micropp<3> constructor: calc_bmat
@@ -216,6 +218,30 @@ int mppd_batch_get_info(const mppd_batch *b, int *info);
 int mppd_batch_update_vars(mppd_batch *b);
 /* stats: {CG iterations queued, graph chunk length, graph launches, device ms of the last homogenize} */
 int mppd_batch_stats(const mppd_batch *b, double stats[4]);
+
+/* ---- a batch as working slots for GPs whose state the caller keeps ----
+ * micropp<3>::homogenize (src/homogenize.cpp:68-99, OpenMP over the GPs of
+ * one instance) runs its FE GPs through a batch of identical RVEs: the GP
+ * state (strains, ctan, flags on the host; u_n, u_k, vars_n, vars_k on the
+ * device) stays with the instance, and each slot is loaded before and stored
+ * after mppd_batch_homogenize.  A slot loaded with coupling MPPD_FE_LINEAR
+ * does no FE work (an idle slot). */
+typedef struct {
+  double strain[6], strain_old[6], stress[6], ctan[36];
+  int coupling;   /* MPPD_FE_* */
+  int allocated;  /* the GP has history (gp_t::allocated) */
+  int cost, converged, subiterated;
+} mppd_batch_gp;
+/* device bytes one slot of this geometry takes in a batch (columns shared
+ * by RVEs of one size not included) */
+int mppd_batch_slot_bytes(const mppd_rve *r, size_t *bytes);
+/* slot r <- gp; u_n_d (nrow doubles) is copied in when not NULL (else the
+ * slot's u_n is zeroed); vars_n_d (nvars doubles) is required when
+ * gp->allocated and only then read. */
+int mppd_batch_load(mppd_batch *b, int r, const mppd_batch_gp *gp, const double *u_n_d, const double *vars_n_d);
+/* gp <- slot r; u_k_d (nrow) is written when not NULL; vars_k_d (nvars) is
+ * required when the slot holds history (mppd_batch_get_info tells which). */
+int mppd_batch_store(mppd_batch *b, int r, mppd_batch_gp *gp, double *u_k_d, double *vars_k_d);
 
 #ifdef __cplusplus
 }

@@ -115,6 +115,7 @@ SIGNATURES = {
     "mppd_memcpy_d2d": (_I, [_VP, _VP, C.c_size_t]),
     "mppd_memset": (_I, [_VP, _I, C.c_size_t]),
     "mppd_synchronize": (_I, []),
+    "mppd_mem_info": (_I, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "mppd_rve_create": (_I, [C.POINTER(C.c_void_p), _PI, _I, _PD, C.POINTER(MaterialBase)]),
     "mppd_rve_destroy": (_I, [_VP]),
     "mppd_rve_info": (_I, [_VP, _PI, _PI, _PI]),
@@ -151,6 +152,9 @@ SIGNATURES = {
     "mppd_batch_get_info": (_I, [_VP, _PI]),
     "mppd_batch_update_vars": (_I, [_VP]),
     "mppd_batch_stats": (_I, [_VP, _PD]),
+    "mppd_batch_slot_bytes": (_I, [_VP, C.POINTER(C.c_size_t)]),
+    "mppd_batch_load": (_I, [_VP, _I, _VP, _VP, _VP]),
+    "mppd_batch_store": (_I, [_VP, _I, _VP, _VP, _VP]),
 }
 
 

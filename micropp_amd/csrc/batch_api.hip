@@ -239,11 +239,14 @@ int newton_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &it
     bool any = false;
     for (int r = 0; r < n; ++r) {
       if (!run[r]) continue;
+      // the reference tests the residual only at the top of an iteration
+      // it is about to run: at the iteration cap it stops unconverged, even
+      // when the last residual passes (src/solve.cpp:43-78)
       const double norm = B->norms_h[r];
-      if (norm < p.nr_max_tol || norm < norm0[r] * p.nr_rel_tol) {
-        conv[r] = 1;
+      if (its[r] >= p.nr_max_its) {
         run[r] = 0;
-      } else if (its[r] >= p.nr_max_its) {
+      } else if (norm < p.nr_max_tol || norm < norm0[r] * p.nr_rel_tol) {
+        conv[r] = 1;
         run[r] = 0;
       } else {
         any = true;
@@ -742,6 +745,70 @@ int mppd_batch_stats(const mppd_batch *b, double stats[4]) {
   stats[2] = b->graph_launches;
   stats[3] = b->last_ms;
   return MPPD_OK;
+}
+
+// The arena shares setup() gives one RVE, plus its mppd_rve tables.
+int mppd_batch_slot_bytes(const mppd_rve *r, size_t *bytes) {
+  if (!r || !bytes) return fail(MPPD_ERR_ARG, "null argument");
+  const Grid &g = r->g;
+  const size_t nb_row = g.nrow_pad / 256, nb_elem = (g.nelem + 255) / 256;
+  size_t b = (size_t)g.nrow_pad * (kSlots + 10) * sizeof(double);
+  b += (size_t)(r->all_elastic ? 3 : g.nelem) * 576 * sizeof(double);
+  b += (size_t)g.nelem * (24 * sizeof(double) + sizeof(int));
+  if (!r->all_elastic) b += (size_t)g.nelem * 8 * NVAR * 3 * sizeof(double);
+  b += std::max(3 * nb_row, 6 * nb_elem) * sizeof(double) + 2 * sizeof(CgState) + 8 * 144 * sizeof(double);
+  *bytes = b;
+  return MPPD_OK;
+}
+
+int mppd_batch_load(mppd_batch *B, int r, const mppd_batch_gp *gp, const double *u_n_d, const double *vars_n_d) {
+  if (!B || !gp || r < 0 || r >= B->n) return fail(MPPD_ERR_ARG, "bad batch slot");
+  if (gp->coupling != MPPD_FE_LINEAR && gp->coupling != MPPD_FE_ONE_WAY && gp->coupling != MPPD_FE_FULL)
+    return fail(MPPD_ERR_ARG, "batch coupling must be FE_LINEAR, FE_ONE_WAY or FE_FULL");
+  BState &q = B->s[r];
+  const BRve &R = B->h[r];
+  if (gp->allocated && (!vars_n_d || !q.nonlinear_mat))
+    return fail(MPPD_ERR_ARG, "a GP with history needs vars_n and a slot with a nonlinear phase");
+  std::memcpy(q.strain, gp->strain, sizeof(q.strain));
+  std::memcpy(q.strain_old, gp->strain_old, sizeof(q.strain_old));
+  std::memcpy(q.stress, gp->stress, sizeof(q.stress));
+  std::memcpy(q.ctan, gp->ctan, sizeof(q.ctan));
+  q.coupling = gp->coupling;
+  q.allocated = gp->allocated != 0;
+  q.cost = gp->cost;
+  q.converged = gp->converged;
+  q.subiterated = gp->subiterated;
+  const size_t nrow = (size_t)R.g.nrow;
+  if (u_n_d)
+    MPP_HIP(hipMemcpyAsync(R.u_n, u_n_d, sizeof(double) * nrow, hipMemcpyDeviceToDevice, B->st));
+  else
+    MPP_HIP(hipMemsetAsync(R.u_n, 0, sizeof(double) * nrow, B->st));
+  if (q.allocated)
+    MPP_HIP(hipMemcpyAsync(R.vars_n, vars_n_d, sizeof(double) * R.g.nelem * 8 * NVAR, hipMemcpyDeviceToDevice,
+                           B->st));
+  return MPPD_OK;
+}
+
+int mppd_batch_store(mppd_batch *B, int r, mppd_batch_gp *gp, double *u_k_d, double *vars_k_d) {
+  if (!B || !gp || r < 0 || r >= B->n) return fail(MPPD_ERR_ARG, "bad batch slot");
+  const BState &q = B->s[r];
+  const BRve &R = B->h[r];
+  if (q.allocated && !vars_k_d) return fail(MPPD_ERR_ARG, "the slot holds history: vars_k needed");
+  std::memcpy(gp->strain, q.strain, sizeof(q.strain));
+  std::memcpy(gp->strain_old, q.strain_old, sizeof(q.strain_old));
+  std::memcpy(gp->stress, q.stress, sizeof(q.stress));
+  std::memcpy(gp->ctan, q.ctan, sizeof(q.ctan));
+  gp->coupling = q.coupling;
+  gp->allocated = q.allocated;
+  gp->cost = (int)q.cost;
+  gp->converged = q.converged;
+  gp->subiterated = q.subiterated;
+  if (u_k_d)
+    MPP_HIP(hipMemcpyAsync(u_k_d, R.u_k, sizeof(double) * R.g.nrow, hipMemcpyDeviceToDevice, B->st));
+  if (q.allocated)
+    MPP_HIP(hipMemcpyAsync(vars_k_d, R.vars_k, sizeof(double) * R.g.nelem * 8 * NVAR, hipMemcpyDeviceToDevice,
+                           B->st));
+  return sync(B);
 }
 
 }  // extern "C"

@@ -6,6 +6,7 @@
 // history) runs in the gfx950 kernels.  Host code keeps the reference's
 // control flow: the Newton loop, subiterations, the FE_FULL perturbation loop,
 // cost bookkeeping, and the 6x6 products of homogenize_linear.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
@@ -62,8 +63,13 @@ struct micropp<tdim>::impl {
   mppd_solver *A0 = nullptr;
   double *u = nullptr;        // working displacement (device)
   double *vars_aux = nullptr;  // vars_new when the GP has no history yet (device)
+  std::vector<int> fe_ids;     // GPs with FE_ONE_WAY / FE_FULL coupling
+  mppd_batch *batch = nullptr;  // working slots for those GPs (homogenize)
+  int batch_slots = 0;
+  bool batch_tried = false;
 
   ~impl() {
+    mppd_batch_destroy(batch);
     for (auto &q : gp)
       for (double *p : {q.vars_n, q.vars_k, q.u_n, q.u_k}) mppd_free(p);
     mppd_free(u);
@@ -162,6 +168,106 @@ struct micropp<tdim>::impl {
     }
   }
 
+  // The FE GPs of homogenize run as one batch of identical RVE slots, in
+  // groups of batch_slots GPs: every finite-element step of a group is one
+  // launch over all its slots, in place of the reference's OpenMP loop over
+  // GPs (src/homogenize.cpp:72).  Slots hold the same RVE and run the same
+  // kernel bodies as the one-GP path, so the results are bit-identical.
+  // The batch is made at the first homogenize, sized to half the free device
+  // memory; MPPD_GP_BATCH=0 in the environment keeps the one-GP-at-a-time
+  // path, MPPD_GP_BATCH=n caps the slots at n.  With A0 (Newton on the
+  // initial matrix) or fewer than two FE GPs the one-GP path runs.
+  bool batch_ready() {
+    if (batch) return true;
+    if (batch_tried) return false;
+    batch_tried = true;
+    if (use_A0 || fe_ids.size() < 2) return false;
+    const char *env = getenv("MPPD_GP_BATCH");
+    const long cap = env ? strtol(env, nullptr, 10) : -1;
+    if (cap == 0) return false;
+    size_t per = 0, fr = 0, tot = 0;
+    check(mppd_batch_slot_bytes(rve, &per), "batch sizing");
+    check(mppd_mem_info(&fr, &tot), "device memory");
+    size_t n = std::min(fe_ids.size(), fr / 2 / std::max<size_t>(per, 1));
+    if (cap > 0) n = std::min<size_t>(n, (size_t)cap);
+    if (n < 2) return false;
+    mppd_batch_rve d;
+    memset(&d, 0, sizeof(d));
+    d.size[0] = nx;
+    d.size[1] = ny;
+    d.size[2] = nz;
+    d.micro_type = micro_type;
+    for (int i = 0; i < 4; ++i) d.geo_params[i] = geo[i];
+    for (int i = 0; i < MAX_MATERIALS; ++i) d.materials[i] = mats[i];
+    d.coupling = MPPD_FE_ONE_WAY;
+    std::vector<mppd_batch_rve> ds(n, d);
+    mppd_batch_params p;
+    mppd_batch_params_default(&p);
+    p.newton = np;
+    p.lin_stress = lin_stress;
+    p.calc_ctan_lin = 0;  // the instance's ctan_lin_fe is loaded into every slot
+    p.subiterations = subiterations;
+    p.nsubiterations = nsubiterations;
+    if (mppd_batch_create(&batch, (int)n, ds.data(), &p) != MPPD_OK) {
+      cerr << "micropp_amd: GP batch not available (" << mppd_last_error() << "); GPs run one at a time" << endl;
+      batch = nullptr;
+      return false;
+    }
+    batch_slots = (int)n;
+    return true;
+  }
+
+  void fe_batched() {
+    const int S = batch_slots, nfe = (int)fe_ids.size();
+    std::vector<int> info(4 * (size_t)S);
+    for (int c0 = 0; c0 < nfe; c0 += S) {
+      const int m = std::min(S, nfe - c0);
+      try {
+        for (int r = 0; r < S; ++r) {
+          mppd_batch_gp b;
+          memset(&b, 0, sizeof(b));
+          b.coupling = MPPD_FE_LINEAR;  // idle slot
+          b.converged = 1;
+          if (r >= m) {
+            check(mppd_batch_load(batch, r, &b, nullptr, nullptr), "batch load");
+            continue;
+          }
+          const gp_state &q = gp[fe_ids[c0 + r]];
+          memcpy(b.strain, q.strain, sizeof(b.strain));
+          memcpy(b.strain_old, q.strain_old, sizeof(b.strain_old));
+          memcpy(b.stress, q.stress, sizeof(b.stress));
+          memcpy(b.ctan, q.ctan, sizeof(b.ctan));
+          b.coupling = q.coupling == FE_FULL ? MPPD_FE_FULL : MPPD_FE_ONE_WAY;
+          b.allocated = q.allocated;
+          b.cost = (int)q.cost;
+          b.converged = q.converged;
+          b.subiterated = q.subiterated;
+          check(mppd_batch_load(batch, r, &b, q.u_n, q.allocated ? q.vars_n : nullptr), "batch load");
+        }
+        check(mppd_batch_homogenize(batch), "homogenize");
+        check(mppd_batch_get_info(batch, info.data()), "batch info");
+        for (int r = 0; r < m; ++r) {
+          gp_state &q = gp[fe_ids[c0 + r]];
+          if (info[4 * r] && !q.allocated) {  // history allocated on the first nonlinearity
+            q.vars_n = dzeros(nvars);
+            q.vars_k = dzeros(nvars);
+          }
+          mppd_batch_gp b;
+          check(mppd_batch_store(batch, r, &b, q.u_k, info[4 * r] ? q.vars_k : nullptr), "batch store");
+          memcpy(q.stress, b.stress, sizeof(q.stress));
+          memcpy(q.ctan, b.ctan, sizeof(q.ctan));
+          q.allocated = b.allocated != 0;
+          q.cost = b.cost;
+          q.converged = b.converged != 0;
+          q.subiterated = b.subiterated != 0;
+        }
+      } catch (const std::exception &ex) {
+        for (int r = 0; r < m; ++r) gp[fe_ids[c0 + r]].converged = false;
+        cerr << ex.what() << " (gps " << fe_ids[c0] << " .. " << fe_ids[c0 + m - 1] << ")" << endl;
+      }
+    }
+  }
+
   void chamis(double c[36]) const;
 };
 
@@ -245,6 +351,7 @@ micropp<tdim>::micropp(const micropp_params_t &params) : p_(new impl()) {
       gp_state &q = s.gp[g];
       q.coupling = params.coupling ? params.coupling[g] : FE_ONE_WAY;
       if (q.coupling == FE_LINEAR || q.coupling == FE_ONE_WAY || q.coupling == FE_FULL) n_fe++;
+      if (q.coupling == FE_ONE_WAY || q.coupling == FE_FULL) s.fe_ids.push_back(g);
       if (!params.coupling || q.coupling == FE_ONE_WAY || q.coupling == FE_FULL) {
         q.u_n = dzeros(s.nndim);
         q.u_k = dzeros(s.nndim);
@@ -298,17 +405,27 @@ void micropp<tdim>::get_ctan(const int gp_id, double *ctan) const {
   memcpy(ctan, p_->gp[gp_id].ctan, 36 * sizeof(double));
 }
 
-// homogenize (src/homogenize.cpp:68-99).  The GPs are solved one after the
-// other, each solve filling the GPU.  A failed device call marks the GP as
-// not converged and is reported on stderr; the other GPs still run.
+// homogenize (src/homogenize.cpp:68-99).  FE GPs run as a batch (see
+// impl::batch_ready) or one after the other, each solve filling the GPU.  A
+// failed device call marks the GP (or the batch group) as not converged and
+// is reported on stderr; the other GPs still run.
 template <int tdim>
 void micropp<tdim>::homogenize() {
   impl &s = *p_;
+  bool batched = false;
+  try {
+    batched = s.batch_ready();
+  } catch (const std::exception &ex) {
+    cerr << ex.what() << "; GPs run one at a time" << endl;
+  }
+  if (batched) s.fe_batched();
   for (int g = 0; g < s.ngp; ++g) {
     gp_state &q = s.gp[g];
     try {
       if (q.coupling == FE_LINEAR || q.coupling == MIX_RULE_CHAMIS)
         impl::linear(q);
+      else if (batched)
+        continue;
       else if (q.coupling == FE_ONE_WAY)
         s.fe(q, false);
       else if (q.coupling == FE_FULL)

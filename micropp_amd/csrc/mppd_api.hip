@@ -115,6 +115,12 @@ int mppd_synchronize(void) {
   return MPPD_OK;
 }
 
+int mppd_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(MPPD_ERR_ARG, "null argument");
+  MPP_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return MPPD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // RVE
 // ---------------------------------------------------------------------------

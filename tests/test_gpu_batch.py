@@ -190,3 +190,27 @@ def test_batch_rejects_bad_input(mp):
     with pytest.raises(mp.MppdError):
         mp.Batch([dict(size=5, micro_type=1, geo=(0.2, 0, 0, 0), materials=[mp.material(0, 1e6, 0.3)] * 3)],
                  graph_chunk=3)
+
+
+def test_batch_newton_stops_unconverged_at_the_cap(mp):
+    """The reference's Newton tests the residual only before an iteration it will run: at nr_max_its it reports
+    non-convergence even when the last residual passes (src/solve.cpp:43-78).  A yielding J2 RVE whose third
+    residual passes 1e-6 must come out unconverged and subiterated, as a micropp<3> instance reports it."""
+    mats = [(1, 1e6, 0.3, 1e5, 2e3), (0, 1e7, 0.25), (0, 1e7, 0.25)]
+    kw = dict(lin_stress=0, subiterations=1, nsubiterations=3, nr_max_its=3, nr_rel_tol=1e-6, cg_rel_tol=1e-10,
+              cg_max_its=3000)
+    eps = np.random.default_rng(7).uniform(-1, 1, (4, 6))[3] * 6e-3 / 3
+    rv = dict(size=6, micro_type=mp.MIC_SPHERE, geo=(0.3, 0, 0, 0), materials=[mp.material(*s) for s in mats],
+              coupling=mp.FE_FULL)
+    b = mp.Batch([rv, dict(rv, coupling=mp.FE_ONE_WAY)], **kw)
+    b.set_strains(np.array([eps, eps]))
+    b.homogenize()
+    m = mp.Micropp3(6, mp.MIC_SPHERE, (0.3, 0, 0, 0), rv["materials"], ngp=1, coupling=[mp.FE_FULL], **kw)
+    m.set_strain(0, eps)
+    m.homogenize()
+    info = b.info()
+    print("batch info", info.tolist(), "instance", m.get_cost(0), m.has_converged(0), m.has_subiterated(0), flush=True)
+    assert np.array_equal(b.stress()[0], m.get_stress(0))
+    assert (int(info[0][1]), int(info[0][2]), int(info[0][3])) == (m.get_cost(0), int(m.has_converged(0)),
+                                                                   int(m.has_subiterated(0)))
+    assert (int(info[0][2]), int(info[0][3])) == (0, 1)  # the case: subiterated, last Newton stopped at the cap
