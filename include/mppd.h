@@ -107,6 +107,10 @@ int mppd_cg(mppd_solver *s, const mppd_solver *A0, const double *b_d, double *x_
 int mppd_ave_stress(mppd_solver *s, const double *u_d, const double *vars_d, double sig[6]);
 /* calc_vars_new (src/update.cpp:32-56); *nonlinear = OR of the evolute flags */
 int mppd_vars_new(mppd_solver *s, const double *u_d, const double *vars_old_d, double *vars_new_d, int *nonlinear);
+/* calc_fields (src/average.cpp:84-112): per element, the mean strain and
+ * stress over its Gauss points, nelem x 6 each, element-major (host arrays);
+ * the cell data of write_vtu (src/output.cpp:67-214). */
+int mppd_elem_fields(mppd_solver *s, const double *u_d, const double *vars_d, double *strain_h, double *stress_h);
 
 /* newton_raphson (src/solve.cpp:28-82) */
 typedef struct {

@@ -134,6 +134,7 @@ SIGNATURES = {
     "mppd_cg": (_I, [_VP, _VP, _VP, _VP, _D, _D, _I, _PI, _PD]),
     "mppd_ave_stress": (_I, [_VP, _VP, _VP, _PD]),
     "mppd_vars_new": (_I, [_VP, _VP, _VP, _VP, _PI]),
+    "mppd_elem_fields": (_I, [_VP, _VP, _VP, _PD, _PD]),
     "mppd_newton": (_I, [_VP, _VP, _PD, _VP, _VP, C.POINTER(NewtonParams), _PI]),
     "mppd_cg_profile": (_I, [_VP, _VP, _I, _PD, _PD]),
     "mppd_msolver_create": (_I, [C.POINTER(C.c_void_p), _VP, _I]),
@@ -317,6 +318,14 @@ class Micropp3:
     def has_subiterated(self, gp):
         return bool(load().micropp3_has_subiterated(C.byref(self.h), gp))
 
+    def output(self, gp, filename):
+        """write_vtu of GP gp into <filename>.vtu."""
+        load().micropp3_output(C.byref(self.h), gp, str(filename).encode())
+
+    def output2(self, gp, elem_global, time_step):
+        """micropp-<elem_global>-<time_step>.vtu in the working directory."""
+        load().micropp3_output2(C.byref(self.h), gp, elem_global, time_step)
+
     def write_restart(self, restart_id):
         load().micropp3_write_restart(C.byref(self.h), restart_id)
 
@@ -489,6 +498,13 @@ class Solver:
         nl = C.c_int()
         _chk(load().mppd_vars_new(self.h, u.ptr, _ptr(vars_old), vars_new.ptr, C.byref(nl)), "mppd_vars_new")
         return nl.value
+
+    def elem_fields(self, u: DeviceArray, vars_: DeviceArray | None = None):
+        """calc_fields: (nelem x 6 strain, nelem x 6 stress)."""
+        ne = self.rve.nelem
+        es, ss = np.zeros(ne * 6), np.zeros(ne * 6)
+        _chk(load().mppd_elem_fields(self.h, u.ptr, _ptr(vars_), _dp(es), _dp(ss)), "mppd_elem_fields")
+        return es.reshape(ne, 6), ss.reshape(ne, 6)
 
     def newton(self, eps, u: DeviceArray, vars_old: DeviceArray | None = None, A0=None, nr_max_its=4,
                nr_max_tol=1e-10, nr_rel_tol=1e-3, cg_max_its=1000, cg_abs_tol=1e-50, cg_rel_tol=1e-5,

@@ -95,6 +95,12 @@ __global__ void __launch_bounds__(256) k_vars_new(Grid g, const double *bmat, co
   vars_new_body(g, blockIdx.x, bmat, mats, et, u, vars_old, vars_new, &red->flag);
 }
 
+__global__ void __launch_bounds__(256) k_elem_fields(Grid g, const double *bmat, const material_base *mats,
+                                                     const int *et, const double *u, const double *vars,
+                                                     double *strain, double *stress) {
+  elem_fields_body(g, blockIdx.x, bmat, mats, et, u, vars, strain, stress);
+}
+
 // ---------------------------------------------------------------------------
 static unsigned nblk(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -145,6 +151,13 @@ hipError_t launch_ave_stress(const Grid &g, const double *bmat, const material_b
   const unsigned nb = nblk(g.nelem, 256);
   hipLaunchKernelGGL(k_ave_stress, dim3(nb), dim3(256), 0, st, g, bmat, mats_d, et, u, vars, partials);
   hipLaunchKernelGGL(k_ave_finish, dim3(1), dim3(256), 0, st, partials, (int)nb, g.vol_tot, red);
+  return hipGetLastError();
+}
+
+hipError_t launch_elem_fields(const Grid &g, const double *bmat, const material_base *mats_d, const int *et,
+                              const double *u, const double *vars, double *strain, double *stress, hipStream_t st) {
+  hipLaunchKernelGGL(k_elem_fields, dim3(nblk(g.nelem, 256)), dim3(256), 0, st, g, bmat, mats_d, et, u, vars, strain,
+                     stress);
   return hipGetLastError();
 }
 

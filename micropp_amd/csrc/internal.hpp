@@ -117,6 +117,9 @@ hipError_t launch_ave_stress(const Grid &g, const double *bmat, const material_b
                              const double *u, const double *vars, double *partials, RedState *red, hipStream_t st);
 hipError_t launch_vars_new(const Grid &g, const double *bmat, const material_base *mats, const int *et, const double *u,
                            const double *vars_old, double *vars_new, RedState *red, hipStream_t st);
+// calc_fields: strain / stress nelem x 6 each
+hipError_t launch_elem_fields(const Grid &g, const double *bmat, const material_base *mats, const int *et,
+                              const double *u, const double *vars, double *strain, double *stress, hipStream_t st);
 
 // ---------------- launchers (solver.hip) ----------------
 // CG state is a pair: iteration k reads state[k & 1] and writes

@@ -323,6 +323,42 @@ __device__ __forceinline__ void ave_stress_body(const Grid &g, int lb, const dou
 }
 
 // ---------------------------------------------------------------------------
+// calc_fields (src/average.cpp:84-112): per element, the Gauss-point strain
+// and stress summed with weight wg, times ivol = 1 / (8 wg)
+// (src/micropp.cpp:59); element-major [e * 6 + v].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void elem_fields_body(const Grid &g, int lb, const double *bmat_g,
+                                                 const material_base *mats, const int *et, const double *u,
+                                                 const double *vars, double *strain, double *stress) {
+#pragma clang fp contract(off)
+  __shared__ double bm[8 * 144];
+  stage_bmat(bmat_g, bm);
+  __syncthreads();
+  const int e = lb * 256 + threadIdx.x;
+  if (e >= g.nelem) return;
+  int ex, ey, ez;
+  elem_coords(g, e, ex, ey, ez);
+  double ue[24];
+  gather_ue(g, ex, ey, ez, u, ue);
+  const material_base &m = mats[et[e]];
+  double ea[6] = {0, 0, 0, 0, 0, 0}, sa[6] = {0, 0, 0, 0, 0, 0};
+  for (int gp = 0; gp < 8; ++gp) {
+    double eps[6], sig[6];
+    strain_at(&bm[gp * 144], ue, eps);
+    mat_stress(m, eps, vars ? vars + ((size_t)e * 8 + gp) * NVAR : nullptr, sig);
+    for (int v = 0; v < 6; ++v) {
+      ea[v] += eps[v] * g.wg;
+      sa[v] += sig[v] * g.wg;
+    }
+  }
+  const double ivol = 1.0 / (g.wg * 8);
+  for (int v = 0; v < 6; ++v) {
+    strain[(size_t)e * 6 + v] = ea[v] * ivol;
+    stress[(size_t)e * 6 + v] = sa[v] * ivol;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // calc_vars_new (src/update.cpp:32-56): evolute per Gauss point; any
 // nonlinear point sets *flag.
 // ---------------------------------------------------------------------------

@@ -269,6 +269,7 @@ struct micropp<tdim>::impl {
   }
 
   void chamis(double c[36]) const;
+  void write_vtu(const gp_state &q, const std::string &name) const;
 };
 
 // calc_ctan_lin_mix_rule_Chamis (src/micropp.cpp:286-331) and invert_3x3
@@ -537,16 +538,116 @@ void micropp<tdim>::read_restart(const int restart_id) const {
   }
 }
 
+// write_vtu (src/output.cpp:67-214) of one GP: its u_k and vars_n, with the
+// element strain and stress of calc_fields computed on the GPU.  The file is
+// the reference's ASCII UnstructuredGrid, reals printed as %e (the
+// reference's std::scientific at the default precision of 6).
 template <int tdim>
-void micropp<tdim>::output(int gp_id, const char *filename) {
-  (void)gp_id;
-  cerr << "micropp_amd: VTU output is not available (" << filename << ")" << endl;
+void micropp<tdim>::impl::write_vtu(const gp_state &q, const std::string &name) const {
+  if (!q.u_k) throw std::runtime_error("micropp_amd: output: the GP has no displacement field (its coupling is "
+                                       "FE_LINEAR or MIX_RULE_CHAMIS)");
+  const int ne = nelem;
+  std::vector<double> u(nndim), es(6 * (size_t)ne), ss(6 * (size_t)ne), vars;
+  std::vector<int> et(ne);
+  const double *vars_d = q.allocated ? q.vars_n : nullptr;
+  check(mppd_memcpy_d2h(u.data(), q.u_k, sizeof(double) * nndim), "output copy");
+  if (vars_d) {
+    vars.resize(nvars);
+    check(mppd_memcpy_d2h(vars.data(), vars_d, sizeof(double) * nvars), "output copy");
+  }
+  check(mppd_elem_fields(sol, q.u_k, vars_d, es.data(), ss.data()), "calc_fields");
+  check(mppd_rve_elem_types(rve, et.data()), "element types");
+  const std::string path = name + ".vtu";
+  FILE *f = fopen(path.c_str(), "w");
+  if (!f) throw std::runtime_error("micropp_amd: output: cannot open " + path);
+  const double dx = 1.0 / (nx - 1), dy = 1.0 / (ny - 1), dz = 1.0 / (nz - 1);
+  fprintf(f,
+          "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
+          "byte_order=\"LittleEndian\">\n<UnstructuredGrid>\n<Piece NumberOfPoints=\"%d\" NumberOfCells=\"%d\">\n"
+          "<Points>\n<DataArray type=\"Float64\" Name=\"Position\" NumberOfComponents=\"3\" format=\"ascii\">\n",
+          nn, ne);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) fprintf(f, "%e %e %e \n", i * dx, j * dy, k * dz);
+  fprintf(f, "</DataArray>\n</Points>\n<Cells>\n\n"
+             "<DataArray type=\"Int32\" Name=\"connectivity\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+  const int nxny = nx * ny;
+  for (int ez = 0; ez < nz - 1; ++ez)
+    for (int ey = 0; ey < ny - 1; ++ey)
+      for (int ex = 0; ex < nx - 1; ++ex) {
+        // get_elem_nodes (src/common.cpp:28-41)
+        const int n0 = ez * nxny + ey * nx + ex;
+        fprintf(f, "%d %d %d %d %d %d %d %d \n", n0, n0 + 1, n0 + nx + 1, n0 + nx, n0 + nxny, n0 + nxny + 1,
+                n0 + nxny + nx + 1, n0 + nxny + nx);
+      }
+  fprintf(f, "</DataArray>\n<DataArray type=\"Int32\" Name=\"offsets\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+  for (int e = 0; e < ne; ++e) fprintf(f, "%d ", 8 * (e + 1));
+  fprintf(f, "\n</DataArray>\n<DataArray type=\"UInt8\"  Name=\"types\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+  for (int e = 0; e < ne; ++e) fputs("12 ", f);
+  fprintf(f, "\n</DataArray>\n</Cells>\n<PointData Vectors=\"displ\" >\n"
+             "<DataArray type=\"Float64\" Name=\"displ\" NumberOfComponents=\"3\" format=\"ascii\" >\n");
+  for (int n = 0; n < nn; ++n) fprintf(f, "%e %e %e \n", u[3 * n], u[3 * n + 1], u[3 * n + 2]);
+  fprintf(f, "</DataArray>\n</PointData>\n<CellData>\n");
+  for (const auto &arr : {std::make_pair("strain", &es), std::make_pair("stress", &ss)}) {
+    fprintf(f, "<DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"6\" format=\"ascii\">\n", arr.first);
+    const double *a = arr.second->data();
+    for (int e = 0; e < ne; ++e)
+      fprintf(f, "%e %e %e %e %e %e \n", a[6 * e], a[6 * e + 1], a[6 * e + 2], a[6 * e + 3], a[6 * e + 4],
+              a[6 * e + 5]);
+    fprintf(f, "</DataArray>\n");
+  }
+  fprintf(f, "<DataArray type=\"Int32\" Name=\"elem_type\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+  for (int e = 0; e < ne; ++e) fprintf(f, "%d ", et[e]);
+  fprintf(f, "\n</DataArray>\n");
+  // history summaries per element, means over the 8 Gauss points of history
+  // slot 0..5 squared (plus the square root of that sum, as the reference
+  // adds it), slot 0, slot 1 and slot 6; zero without history
+  const char *hname[4] = {"plasticity", "damage_e", "damage_D", "hardening"};
+  const int hslot[4] = {-1, 0, 1, 6};
+  for (int h = 0; h < 4; ++h) {
+    fprintf(f, "<DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"1\" format=\"ascii\">\n", hname[h]);
+    for (int e = 0; e < ne; ++e) {
+      double acc = 0.0;
+      if (!vars.empty())
+        for (int gp = 0; gp < 8; ++gp) {
+          const double *v = &vars[((size_t)e * 8 + gp) * NUM_VAR_GP];
+          if (hslot[h] >= 0) {
+            acc += v[hslot[h]];
+          } else {
+            for (int c = 0; c < 6; ++c) acc += v[c] * v[c];
+          }
+        }
+      if (hslot[h] < 0) acc += sqrt(acc);
+      fprintf(f, "%e ", acc / 8);
+    }
+    fprintf(f, "\n</DataArray>\n");
+  }
+  fprintf(f, "</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
+  if (fclose(f) != 0) throw std::runtime_error("micropp_amd: output: write error on " + path);
 }
 
+// output (src/output.cpp:29-37): <filename>.vtu of GP gp_id
+template <int tdim>
+void micropp<tdim>::output(int gp_id, const char *filename) {
+  assert(gp_id >= 0 && gp_id < p_->ngp);
+  try {
+    p_->write_vtu(p_->gp[gp_id], filename);
+  } catch (const std::exception &ex) {
+    cerr << ex.what() << endl;
+  }
+}
+
+// output2 (src/output.cpp:39-65): micropp-<elem_global>-<time_step>.vtu
 template <int tdim>
 void micropp<tdim>::output2(const int gp_id, const int elem_global, const int time_step) {
-  (void)gp_id;
-  cerr << "micropp_amd: VTU output is not available (micropp-" << elem_global << "-" << time_step << ")" << endl;
+  assert(gp_id >= 0 && gp_id < p_->ngp);
+  std::stringstream fn;
+  fn << "micropp-" << elem_global << "-" << time_step;
+  try {
+    p_->write_vtu(p_->gp[gp_id], fn.str());
+  } catch (const std::exception &ex) {
+    cerr << ex.what() << endl;
+  }
 }
 
 // print_info (src/micropp.cpp:546-597)

@@ -459,6 +459,21 @@ int mppd_ave_stress(mppd_solver *s, const double *u_d, const double *vars_d, dou
   return MPPD_OK;
 }
 
+int mppd_elem_fields(mppd_solver *s, const double *u_d, const double *vars_d, double *strain_h, double *stress_h) {
+  if (!s || !u_d || !strain_h || !stress_h) return fail(MPPD_ERR_ARG, "null argument");
+  const mppd_rve *r = s->r;
+  const size_t n = (size_t)r->g.nelem * 6;
+  double *buf = nullptr;
+  int rc = dalloc(&buf, 2 * n, false);
+  if (rc) return rc;
+  hipError_t e = launch_elem_fields(r->g, r->bmat, r->mats_d, r->et, u_d, vars_d, buf, buf + n, s->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(strain_h, buf, sizeof(double) * n, hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(stress_h, buf + n, sizeof(double) * n, hipMemcpyDeviceToHost, s->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+  dfree(buf);
+  return e == hipSuccess ? MPPD_OK : hip_fail(e, "calc_fields");
+}
+
 int mppd_vars_new(mppd_solver *s, const double *u_d, const double *vars_old_d, double *vars_new_d, int *nonlinear) {
   const mppd_rve *r = s->r;
   MPP_HIP(hipMemsetAsync(&s->red->flag, 0, sizeof(int), s->st));

@@ -303,6 +303,9 @@ class Oracle(_Lib):
         L.orc_ave_stress.argtypes = [_VP, _D, PD, _D]
         L.orc_vars_new.argtypes = [_VP, _D, PD, _D]
         L.orc_vars_new.restype = C.c_int
+        L.orc_calc_fields.argtypes = [_VP, _D, PD, _D, _D]
+        L.orc_write_vtu.argtypes = [_VP, _D, PD, C.c_char_p]
+        L.orc_write_vtu.restype = C.c_int
         L.orc_newton.argtypes = [_VP, _D, PD, _D, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double,
                                  C.c_double, PD, C.c_int, _I]
 
@@ -385,6 +388,18 @@ class OracleRVE:
         nl = self.o.lib.orc_vars_new(self.h, np.ascontiguousarray(u), _dptr(vars_old), vn)
         return vn, nl
 
+    def calc_fields(self, u, vars_=None):
+        es, ss = np.zeros(self.nelem * 6), np.zeros(self.nelem * 6)
+        self.o.lib.orc_calc_fields(self.h, np.ascontiguousarray(u), _dptr(vars_), es, ss)
+        return es.reshape(-1, 6), ss.reshape(-1, 6)
+
+    def write_vtu(self, u, vars_, filename):
+        """Writes <filename>.vtu; returns its path."""
+        rc = self.o.lib.orc_write_vtu(self.h, np.ascontiguousarray(u), _dptr(vars_), str(filename).encode())
+        if rc:
+            raise OSError(f"cannot write {filename}.vtu")
+        return f"{filename}.vtu"
+
     def newton(self, eps, u, vars_old=None, nr_max_its=4, nr_max_tol=1e-10, nr_rel_tol=1e-3, cg_max_its=1000,
                cg_abs_tol=1e-50, cg_rel_tol=1e-5, A0=None, its_with_A0=1):
         out = np.zeros(3, dtype=np.int32)
@@ -412,6 +427,7 @@ class Ref(_Lib):
             ("ave_stress", None, [_VP, _D, PD, _D]),
             ("vars_new", C.c_int, [_VP, _D, PD, _D]),
             ("newton", None, [_VP, _D, PD, _D, C.c_double, C.c_double, C.c_int, _I]),
+            ("write_vtu", C.c_int, [_VP, _D, PD, C.c_char_p]),
         ]:
             f = getattr(L, "ref_" + n)
             f.restype = res
@@ -462,6 +478,11 @@ class RefMicro(Micro):
         vn = np.zeros(self.nvars)
         nl = self.lib.lib.ref_vars_new(self.h, np.ascontiguousarray(u), _dptr(vars_old), vn)
         return vn, nl
+
+    def write_vtu(self, u, vars_, filename):
+        """calc_fields + write_vtu of the reference; returns the file's path."""
+        self.lib.lib.ref_write_vtu(self.h, np.ascontiguousarray(u), _dptr(vars_), str(filename).encode())
+        return f"{filename}.vtu"
 
     def newton(self, eps, u, vars_old=None, cg_abs_tol=1e-50, cg_rel_tol=1e-5, cg_max_its=1000):
         out = np.zeros(3, dtype=np.int32)

@@ -26,6 +26,7 @@
 
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -867,6 +868,117 @@ int orc_vars_new(const orc_rve *r, const double *u, const double *vars_old, doub
         }
       }
   return nl;
+}
+
+/* calc_fields (src/average.cpp:84-112): per element, the Gauss-point strain
+ * and stress summed with weight wg, times ivol = 1 / (wg * 8) (src/micropp.cpp:59). */
+void orc_calc_fields(const orc_rve *r, const double *u, const double *vars, double *elem_strain, double *elem_stress) {
+  const double ivol = 1.0 / (r->wg * 8);
+  for (int ez = 0; ez < r->nez; ++ez)
+    for (int ey = 0; ey < r->ney; ++ey)
+      for (int ex = 0; ex < r->nex; ++ex) {
+        double ea[6] = {0, 0, 0, 0, 0, 0}, sa[6] = {0, 0, 0, 0, 0, 0};
+        const int e = elem_id(r, ex, ey, ez);
+        for (int gp = 0; gp < 8; ++gp) {
+          double eps[6], sig[6];
+          gp_strain(r, u, gp, ex, ey, ez, eps);
+          gp_stress(r, vars, e, gp, eps, sig);
+          for (int v = 0; v < 6; ++v) {
+            ea[v] += eps[v] * r->wg;
+            sa[v] += sig[v] * r->wg;
+          }
+        }
+        for (int v = 0; v < 6; ++v) {
+          elem_strain[e * 6 + v] = ea[v] * ivol;
+          elem_stress[e * 6 + v] = sa[v] * ivol;
+        }
+      }
+}
+
+/* One cell array of write_vtu: the mean over the 8 Gauss points of history
+ * slot v (or, kind 1, of |eps_p|^2 plus its square root, as the reference's
+ * "plasticity" array computes it, src/output.cpp:163-176). */
+static double vtu_cell_mean(const double *vars, int e, int kind, int v) {
+  double s = 0.0;
+  if (kind == 1) {
+    if (vars)
+      for (int gp = 0; gp < 8; ++gp)
+        for (int c = 0; c < 6; ++c) {
+          const double x = vars[((size_t)e * 8 + gp) * ORC_NVAR + c];
+          s += x * x;
+        }
+    s += sqrt(s);
+  } else if (vars) {
+    for (int gp = 0; gp < 8; ++gp) s += vars[((size_t)e * 8 + gp) * ORC_NVAR + v];
+  }
+  return s / 8;
+}
+
+/* write_vtu (src/output.cpp:67-214) after calc_fields: the ASCII
+ * UnstructuredGrid file <filename>.vtu with positions, hexahedra (VTK 12),
+ * displacements, element strain and stress, phase ids and the history
+ * summaries.  Reals are printed as "%e" (the reference's std::scientific at
+ * the default precision).  Returns 0, or -1 if the file cannot be opened. */
+int orc_write_vtu(const orc_rve *r, const double *u, const double *vars, const char *filename) {
+  char path[4096];
+  snprintf(path, sizeof(path), "%s.vtu", filename);
+  FILE *f = fopen(path, "w");
+  if (!f) return -1;
+  const int ne = r->nelem;
+  double *es = malloc(sizeof(double) * 6 * (size_t)ne), *ss = malloc(sizeof(double) * 6 * (size_t)ne);
+  orc_calc_fields(r, u, vars, es, ss);
+  fprintf(f, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
+             "byte_order=\"LittleEndian\">\n<UnstructuredGrid>\n");
+  fprintf(f, "<Piece NumberOfPoints=\"%d\" NumberOfCells=\"%d\">\n<Points>\n", r->nn, ne);
+  fprintf(f, "<DataArray type=\"Float64\" Name=\"Position\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+  for (int k = 0; k < r->nz; ++k)
+    for (int j = 0; j < r->ny; ++j)
+      for (int i = 0; i < r->nx; ++i) fprintf(f, "%e %e %e \n", i * r->dx, j * r->dy, k * r->dz);
+  fprintf(f, "</DataArray>\n</Points>\n<Cells>\n\n");
+  fprintf(f, "<DataArray type=\"Int32\" Name=\"connectivity\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+  for (int ez = 0; ez < r->nez; ++ez)
+    for (int ey = 0; ey < r->ney; ++ey)
+      for (int ex = 0; ex < r->nex; ++ex) {
+        int n[8];
+        orc_elem_nodes(r->nx, r->ny, ex, ey, ez, n);
+        for (int a = 0; a < 8; ++a) fprintf(f, "%d ", n[a]);
+        fprintf(f, "\n");
+      }
+  fprintf(f, "</DataArray>\n");
+  fprintf(f, "<DataArray type=\"Int32\" Name=\"offsets\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+  for (int e = 0; e < ne; ++e) fprintf(f, "%d ", 8 * (e + 1));
+  fprintf(f, "\n</DataArray>\n");
+  fprintf(f, "<DataArray type=\"UInt8\"  Name=\"types\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+  for (int e = 0; e < ne; ++e) fprintf(f, "12 ");
+  fprintf(f, "\n</DataArray>\n</Cells>\n");
+  fprintf(f, "<PointData Vectors=\"displ\" >\n");
+  fprintf(f, "<DataArray type=\"Float64\" Name=\"displ\" NumberOfComponents=\"3\" format=\"ascii\" >\n");
+  for (int n = 0; n < r->nn; ++n) fprintf(f, "%e %e %e \n", u[n * 3], u[n * 3 + 1], u[n * 3 + 2]);
+  fprintf(f, "</DataArray>\n</PointData>\n<CellData>\n");
+  const char *names[2] = {"strain", "stress"};
+  const double *arr[2] = {es, ss};
+  for (int q = 0; q < 2; ++q) {
+    fprintf(f, "<DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"6\" format=\"ascii\">\n", names[q]);
+    for (int e = 0; e < ne; ++e) {
+      for (int v = 0; v < 6; ++v) fprintf(f, "%e ", arr[q][e * 6 + v]);
+      fprintf(f, "\n");
+    }
+    fprintf(f, "</DataArray>\n");
+  }
+  fprintf(f, "<DataArray type=\"Int32\" Name=\"elem_type\" NumberOfComponents=\"1\" format=\"ascii\">\n");
+  for (int e = 0; e < ne; ++e) fprintf(f, "%d ", r->et[e]);
+  fprintf(f, "\n</DataArray>\n");
+  const char *hn[4] = {"plasticity", "damage_e", "damage_D", "hardening"};
+  const int kind[4] = {1, 0, 0, 0}, slot[4] = {0, 0, 1, 6};
+  for (int q = 0; q < 4; ++q) {
+    fprintf(f, "<DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"1\" format=\"ascii\">\n", hn[q]);
+    for (int e = 0; e < ne; ++e) fprintf(f, "%e ", vtu_cell_mean(vars, e, kind[q], slot[q]));
+    fprintf(f, "\n</DataArray>\n");
+  }
+  fprintf(f, "</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
+  free(es);
+  free(ss);
+  return fclose(f) == 0 ? 0 : -1;
 }
 
 /* newton_raphson (src/solve.cpp:28-82).  The convergence test sits at the top

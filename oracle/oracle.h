@@ -84,6 +84,10 @@ double orc_assembly_rhs(const orc_rve *r, const double *u, const double *vars, d
 void orc_assembly_mat(const orc_rve *r, const double *u, const double *vars, double *vals);
 void orc_ave_stress(const orc_rve *r, const double *u, const double *vars, double sig[6]);
 int orc_vars_new(const orc_rve *r, const double *u, const double *vars_old, double *vars_new);
+/* calc_fields (src/average.cpp:84-112): nelem x 6 element strain and stress */
+void orc_calc_fields(const orc_rve *r, const double *u, const double *vars, double *elem_strain, double *elem_stress);
+/* calc_fields + write_vtu (src/output.cpp:67-214) into <filename>.vtu; 0 or -1 */
+int orc_write_vtu(const orc_rve *r, const double *u, const double *vars, const char *filename);
 /* Newton-Raphson (src/solve.cpp:28-82); out = {its, solver_its, converged}.
  * A0 may be NULL; then use_A0 is ignored. */
 void orc_newton(const orc_rve *r, const double eps[6], const double *vars_old, double *u, int nr_max_its,

@@ -72,6 +72,10 @@ class probe : public micropp<3> {
     free(du);
     ell_free(&A);
   }
+  void vtu(const double *u, const double *v, const char *fn) {
+    calc_fields(const_cast<double *>(u), const_cast<double *>(v));
+    write_vtu(const_cast<double *>(u), const_cast<double *>(v), fn);
+  }
   void ctan_lin(double *o) const { std::memcpy(o, ctan_lin_fe, sizeof(ctan_lin_fe)); }
   void vfrac(double *o) const {
     o[0] = Vm;
@@ -193,6 +197,10 @@ void ref_assembly_mat(void *m, const double *u, const double *v, double *vals) {
 }
 void ref_ave_stress(void *m, const double *u, const double *v, double sig[6]) {
   static_cast<probe *>(m)->ave(u, v, sig);
+}
+int ref_write_vtu(void *m, const double *u, const double *v, const char *fn) {
+  static_cast<probe *>(m)->vtu(u, v, fn);
+  return 0;
 }
 int ref_vars_new(void *m, const double *u, const double *vo, double *vn) {
   return static_cast<probe *>(m)->vnew(u, vo, vn);
