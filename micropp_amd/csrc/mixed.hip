@@ -420,17 +420,45 @@ __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const doubl
 // k_m_spmv: AP = A P for every field and the partials of (p, Ap).  One block
 // per slice of 64 nodes; wave g handles field group g (fields 4g .. 4g+3), so
 // a block has K/4 waves.  Partial table part1[field * nsl + slice].
+//
+// The slice's matrix streams HBM -> LDS by direct loads (global_load_lds,
+// 16 bytes per lane, no VGPRs held) in chunks of 9 quads = 4 neighbours,
+// two chunks ahead in a 3-stage ring; the block's waves share each chunk
+// (each wave moves a third of it), so the matrix crosses L1 once per block
+// instead of once per wave.  The P gathers of a chunk go to VGPRs.
+//
+// Neighbour addresses are n + offset clamped into [0, NP): every neighbour
+// of an interior node lies in the grid, and the rows of boundary and padding
+// nodes are zero (with P finite everywhere), so no in-grid test is needed.
 // ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void;
+typedef const __attribute__((address_space(1))) void glob_void;
+constexpr int kChunkQuads = 9, kChunks = 7, kStages = 3;
+
 template <int K>
 __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per, const float4 *A, const float4 *P,
                                                 float4 *AP, double *part1, const MState *cur) {
   constexpr int KG = K / 4;
-  if (cur->all_done) return;
+  __shared__ float4 stg[kStages][kChunkQuads][64];
+  if (cur->all_done) return;  // whole grid
   const int slice = xcd_block(per);
-  if (slice >= nsl) return;
+  if (slice >= nsl) return;   // whole block
   const int grp = threadIdx.x >> 6, lane = threadIdx.x & 63, r0 = grp * 4;
-  if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole wave
-  const int n = slice * 64 + lane;
+  // a wave whose four fields have stopped still moves its share of the matrix
+  // and meets every barrier, but gathers and computes nothing
+  const bool live = !(cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]);
+  const int n = slice * 64 + lane, nxny = g.nx * g.ny;
+  const float4 *ablk = A + (size_t)slice * kNodeQuads * 64;
+  const float4 *pg = P + (size_t)grp * NP;
+  auto issue = [&](int ch) {
+#pragma unroll
+    for (int u = 0; u < (kChunkQuads + KG - 1) / KG; ++u) {
+      const int q = grp + KG * u;
+      if (q < kChunkQuads && ch * kChunkQuads + q < kNodeQuads)
+        __builtin_amdgcn_global_load_lds((glob_void *)(ablk + (size_t)(ch * kChunkQuads + q) * 64 + lane),
+                                         (lds_void *)&stg[ch % kStages][q][0], 16, 0, 0);
+    }
+  };
   float acc[3][4];
   float4 pc[3];  // p at the node itself (neighbour 13)
 #pragma unroll
@@ -439,54 +467,64 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[d][r] = 0.0f;
   }
-  if (n < g.nn) {
-    const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
-    const int nxny = g.nx * g.ny;
-    const f4v *arow = reinterpret_cast<const f4v *>(A) + (size_t)slice * kNodeQuads * 64 + lane;
-    const float4 *pg = P + (size_t)grp * NP;
-    // 7 chunks of up to 4 neighbours = 9 quads each (the last: 3 neighbours,
-    // 7 quads); the chunk loop stays rolled to bound the live registers
+  issue(0);
+  issue(1);
 #pragma unroll 1
-    for (int ch = 0; ch < 7; ++ch) {
-      float a[36];
-#pragma unroll
-      for (int q = 0; q < 9; ++q)
-        if (ch * 9 + q < kNodeQuads) {
-          const f4v v = arow[(size_t)(ch * 9 + q) * 64];
-          a[q * 4 + 0] = v.x;
-          a[q * 4 + 1] = v.y;
-          a[q * 4 + 2] = v.z;
-          a[q * 4 + 3] = v.w;
-        }
-      float4 pv[4][3];
+  for (int ch = 0; ch < kChunks; ++ch) {
+    float4 pv[4][3];
+    if (live) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int nb = ch * 4 + t;
-        const int dz = nb / 9 - 1, dy = (nb / 3) % 3 - 1, dx = nb % 3 - 1;
-        const bool in = nb < 27 && (i + dx >= 0 && i + dx < g.nx) && (j + dy >= 0 && j + dy < g.ny) &&
-                        (k + dz >= 0 && k + dz < g.nz);
-        const int m = in ? n + dz * nxny + dy * g.nx + dx : n;  // outside: zero block, any valid node
+        const int off = (nb / 9 - 1) * nxny + ((nb / 3) % 3 - 1) * g.nx + (nb % 3 - 1);
+        const int m = min(max(n + off, 0), NP - 1);
+        if (nb < 27)
 #pragma unroll
-        for (int fj = 0; fj < 3; ++fj) pv[t][fj] = pg[(size_t)fj * KG * NP + m];
+          for (int fj = 0; fj < 3; ++fj) pv[t][fj] = pg[(size_t)fj * KG * NP + m];
       }
-      if (ch == 3)
-#pragma unroll
-        for (int fj = 0; fj < 3; ++fj) pc[fj] = pv[1][fj];
-#pragma unroll
-      for (int t = 0; t < 4; ++t)
-        if (ch * 4 + t < 27)
-#pragma unroll
-          for (int fj = 0; fj < 3; ++fj)
-#pragma unroll
-            for (int fi = 0; fi < 3; ++fi) {
-              const float c = a[t * 9 + fi * 3 + fj];
-              acc[fi][0] += c * pv[t][fj].x;
-              acc[fi][1] += c * pv[t][fj].y;
-              acc[fi][2] += c * pv[t][fj].z;
-              acc[fi][3] += c * pv[t][fj].w;
-            }
+      // Vector-memory loads complete in issue order: once no more than this
+      // chunk's P gathers (12; the last chunk 9) are outstanding, this wave's
+      // copies of chunks ch and ch + 1 have landed in LDS.
+      if (ch < kChunks - 1)
+        __builtin_amdgcn_s_waitcnt(0xF7C);  // vmcnt(12)
+      else
+        __builtin_amdgcn_s_waitcnt(0xF79);  // vmcnt(9)
+    } else {
+      __builtin_amdgcn_s_waitcnt(0);
     }
+    __builtin_amdgcn_s_barrier();  // every wave's share of chunk ch is in LDS
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    float a[36];
+#pragma unroll
+    for (int q = 0; q < kChunkQuads; ++q) {
+      const float4 v = stg[ch % kStages][q][lane];
+      a[q * 4 + 0] = v.x;
+      a[q * 4 + 1] = v.y;
+      a[q * 4 + 2] = v.z;
+      a[q * 4 + 3] = v.w;
+    }
+    // the stage of chunk ch + 2 held chunk ch - 1, which every wave has
+    // consumed before reaching this iteration's barrier
+    if (ch + 2 < kChunks) issue(ch + 2);
+    if (!live) continue;
+    if (ch == 3)
+#pragma unroll
+      for (int fj = 0; fj < 3; ++fj) pc[fj] = pv[1][fj];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      if (ch * 4 + t < 27)
+#pragma unroll
+        for (int fj = 0; fj < 3; ++fj)
+#pragma unroll
+          for (int fi = 0; fi < 3; ++fi) {
+            const float c = a[t * 9 + fi * 3 + fj];
+            acc[fi][0] += c * pv[t][fj].x;
+            acc[fi][1] += c * pv[t][fj].y;
+            acc[fi][2] += c * pv[t][fj].z;
+            acc[fi][3] += c * pv[t][fj].w;
+          }
   }
+  if (!live) return;
   double pap[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
