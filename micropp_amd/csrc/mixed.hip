@@ -30,7 +30,8 @@
 // c * NP + n.
 //
 // Reductions are fence-free: blocks write FP64 partials in a [row][block]
-// table and a one-block kernel re-sums each row in a fixed order.  Iteration
+// table, and each row is re-summed in a fixed order, by a one-block kernel or
+// identically by every block of the consumer kernel.  Iteration
 // k of the inner CG reads state[k & 1] and writes state[(k+1) & 1].  The
 // SpMV maps slices to XCDs so that each XCD's L2 holds a contiguous slab of
 // nodes.
@@ -415,7 +416,7 @@ __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const doubl
 
 // ---------------------------------------------------------------------------
 // Inner CG (the reference's Jacobi-PCG recurrence, src/ell.cpp:80-117), one
-// iteration = k_m_spmv, k_m_alpha, k_m_upd, k_m_beta, k_m_pupd.
+// iteration = k_m_spmv, k_m_alpha, k_m_upd, k_m_pupd.
 //
 // k_m_spmv: AP = A P for every field and the partials of (p, Ap).  One block
 // per slice of 64 nodes; wave g handles field group g (fields 4g .. 4g+3), so
@@ -550,16 +551,17 @@ __global__ void __launch_bounds__(kRedThreads) k_m_alpha(int nsl, const double *
   if (f < K && !cur->done[f]) cur->alpha[f] = cur->rz[f] / pap;
 }
 
-// k_m_upd: X += alpha P; R -= alpha AP; partials of (z, z) and (r, z) with
-// z = kinv R.  Grid (blocks of kVecNodes nodes, 3 K/4 components
-// c = d * K/4 + g), each thread kVecNodes / 256 nodes; partial tables pz / pr
+// k_m_upd: R -= alpha AP; partials of (z, z) and (r, z) with z = kinv R.
+// Grid (blocks of kVecNodes nodes, 3 K/4 components c = d * K/4 + g), each
+// thread kVecNodes / 256 nodes; partial tables pz / pr
 // [(field * 3 + d) * gridDim.x + block] (the 3 gridDim.x partials of a field
-// contiguous; pr = pz + 3 K gridDim.x).
+// contiguous; pr = pz + 3 K gridDim.x).  X is advanced in k_m_pupd, which
+// reads P anyway.
 template <int K>
-__global__ void __launch_bounds__(256) k_m_upd(int NP, const float *kinv, const float4 *P, const float4 *AP,
-                                               float4 *X, float4 *R, double *pz, double *pr, const MState *cur) {
-  constexpr int KG = K / 4;
+__global__ void __launch_bounds__(256) k_m_upd(int NP, const float *kinv, const float4 *AP, float4 *R, double *pz,
+                                               double *pr, const MState *cur) {
   __shared__ double lds[4 * 8];
+  constexpr int KG = K / 4;
   const int c = blockIdx.y, grp = c % KG, d = c / KG, r0 = grp * 4;
   if (cur->all_done) return;
   if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole block
@@ -574,17 +576,12 @@ __global__ void __launch_bounds__(256) k_m_upd(int NP, const float *kinv, const 
     if (n >= NP) break;
     const size_t o = (size_t)c * NP + n;
     const float kv = kinv[(size_t)d * NP + n];
-    const float4 p = P[o], ap = AP[o];
-    float4 x = X[o], r = R[o];
-    x.x += al.x * p.x;
-    x.y += al.y * p.y;
-    x.z += al.z * p.z;
-    x.w += al.w * p.w;
+    const float4 ap = AP[o];
+    float4 r = R[o];
     r.x -= al.x * ap.x;
     r.y -= al.y * ap.y;
     r.z -= al.z * ap.z;
     r.w -= al.w * ap.w;
-    X[o] = x;
     R[o] = r;
     const float rv[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
@@ -612,65 +609,90 @@ __global__ void __launch_bounds__(256) k_m_upd(int NP, const float *kinv, const 
   }
 }
 
-// k_m_beta (one block): beta, the stopping test on ||z|| (src/ell.cpp:95),
-// the next state
-template <int K>
-__global__ void __launch_bounds__(kRedThreads) k_m_beta(int nbx, const double *pz, const MState *cur, MState *nxt) {
-  __shared__ double zz_s[kMaxRhs], rz_s[kMaxRhs];
-  if (cur->all_done) {
-    const int *s = reinterpret_cast<const int *>(cur);
-    int *t = reinterpret_cast<int *>(nxt);
-    for (int q = threadIdx.x; q < (int)(sizeof(MState) / sizeof(int)); q += kRedThreads) t[q] = s[q];
-    return;
-  }
-  // rows 0..K-1: z.z of each field; rows K..2K-1: r.z (the pr table follows pz)
-  const double s = rows_sum<2 * K>(pz, (size_t)3 * nbx, 3 * nbx);
-  if (threadIdx.x < K)
-    zz_s[threadIdx.x] = s;
-  else if (threadIdx.x < 2 * K)
-    rz_s[threadIdx.x - K] = s;
-  __syncthreads();
-  // one thread per field; every read of the current state comes before any write
-  __shared__ int dn_s[kMaxRhs];
-  const int r = threadIdx.x;
-  const int its = cur->its + 1, max_its = cur->max_its;
-  if (r < K) {
-    const int was = cur->done[r];
-    const double rz_old = cur->rz[r], pn_old = cur->pn[r], pn0 = cur->pn0[r], tau = cur->tau[r];
-    const double alpha = cur->alpha[r];
-    const double pn = was ? pn_old : sqrt(zz_s[r]);
-    const int dn = was || pn < tau * pn0 || !(pn == pn);
-    dn_s[r] = dn;
-    nxt->rz[r] = was ? rz_old : rz_s[r];
-    nxt->beta[r] = was ? 0.0 : rz_s[r] / rz_old;
-    nxt->alpha[r] = alpha;
-    nxt->pn[r] = pn;
-    nxt->pn0[r] = pn0;
-    nxt->tau[r] = tau;
-    nxt->done[r] = dn;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int all = 1;
-    for (int q = 0; q < K; ++q) all &= dn_s[q];
-    nxt->its = its;
-    nxt->max_its = max_its;
-    nxt->all_done = all || its >= max_its;
-  }
+// The reference's CG step after the residual update (src/ell.cpp:95-116) for
+// field r: ||z|| from (z, z), the stopping test, beta = (r, z)_new / (r, z)_old.
+struct MStep {
+  double pn, beta, rz;
+  int done;
+};
+__device__ __forceinline__ MStep m_step(const MState *cur, int r, double zz, double rz) {
+  MStep o;
+  const int was = cur->done[r];
+  o.pn = was ? cur->pn[r] : sqrt(zz);
+  o.done = was || o.pn < cur->tau[r] * cur->pn0[r] || !(o.pn == o.pn);
+  o.rz = was ? cur->rz[r] : rz;
+  o.beta = was ? 0.0 : rz / cur->rz[r];
+  return o;
 }
 
-// k_m_pupd: P = kinv R + beta P for the field groups with a live field
-// (stopped fields keep P).  Grid as k_m_upd.
+// k_m_pupd: X += alpha P; P = kinv R + beta P for the fields still running;
+// the next CG state.  Every block re-sums the (z, z) and (r, z) partials of
+// its four fields in the same fixed order (so all blocks agree bit for bit),
+// and block (0, 0) sums all fields and writes the state of the next iteration
+// (beta, ||z||, stop flags, iteration count).  Grid as k_m_upd.
 template <int K>
-__global__ void __launch_bounds__(256) k_m_pupd(int NP, const float *kinv, const float4 *R, float4 *P,
-                                                const MState *nxt) {
+__global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kinv, const float4 *R, float4 *P,
+                                                        float4 *X, const double *pz, const MState *cur,
+                                                        MState *nxt) {
   constexpr int KG = K / 4;
+  __shared__ double beta_s[kMaxRhs];
+  __shared__ int live_s[kMaxRhs], dn_s[kMaxRhs];
   const int c = blockIdx.y, grp = c % KG, d = c / KG, r0 = grp * 4;
-  if (nxt->all_done) return;
-  if (nxt->done[r0] && nxt->done[r0 + 1] && nxt->done[r0 + 2] && nxt->done[r0 + 3]) return;  // whole block
-  const float be[4] = {(float)nxt->beta[r0], (float)nxt->beta[r0 + 1], (float)nxt->beta[r0 + 2],
-                       (float)nxt->beta[r0 + 3]};
-  const int dn[4] = {nxt->done[r0], nxt->done[r0 + 1], nxt->done[r0 + 2], nxt->done[r0 + 3]};
+  const bool head = blockIdx.x == 0 && blockIdx.y == 0;
+  if (cur->all_done) {
+    if (head) {  // keep the finished state for the iterations queued after it
+      const int *s = reinterpret_cast<const int *>(cur);
+      int *t = reinterpret_cast<int *>(nxt);
+      for (int q = threadIdx.x; q < (int)(sizeof(MState) / sizeof(int)); q += kRedThreads) t[q] = s[q];
+    }
+    return;
+  }
+  const size_t row = (size_t)3 * gridDim.x;  // partials per field
+  const double *pr = pz + (size_t)K * row;
+  const int its = cur->its + 1;
+  if (head) {
+    const double zz = rows_sum<K>(pz, row, (int)row);
+    const double rz = rows_sum<K>(pr, row, (int)row);
+    const int r = threadIdx.x;
+    if (r < K) {
+      const MStep o = m_step(cur, r, zz, rz);
+      nxt->rz[r] = o.rz;
+      nxt->beta[r] = o.beta;
+      nxt->alpha[r] = cur->alpha[r];
+      nxt->pn[r] = o.pn;
+      nxt->pn0[r] = cur->pn0[r];
+      nxt->tau[r] = cur->tau[r];
+      nxt->done[r] = o.done;
+      dn_s[r] = o.done;
+      if (r >= r0 && r < r0 + 4) {
+        beta_s[r - r0] = o.beta;
+        live_s[r - r0] = !o.done && its < cur->max_its;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int all = 1;
+      for (int q = 0; q < K; ++q) all &= dn_s[q];
+      nxt->its = its;
+      nxt->max_its = cur->max_its;
+      nxt->all_done = all || its >= cur->max_its;
+    }
+  } else {
+    const double zz = rows_sum<4>(pz + r0 * row, row, (int)row);
+    const double rz = rows_sum<4>(pr + r0 * row, row, (int)row);
+    if (threadIdx.x < 4) {
+      const MStep o = m_step(cur, r0 + threadIdx.x, zz, rz);
+      beta_s[threadIdx.x] = o.beta;
+      live_s[threadIdx.x] = !o.done && its < cur->max_its;
+    }
+  }
+  __syncthreads();
+  const bool was[4] = {cur->done[r0] != 0, cur->done[r0 + 1] != 0, cur->done[r0 + 2] != 0, cur->done[r0 + 3] != 0};
+  if (was[0] && was[1] && was[2] && was[3]) return;  // the group stopped before this iteration
+  const float4 al = make_float4(was[0] ? 0.0f : (float)cur->alpha[r0], was[1] ? 0.0f : (float)cur->alpha[r0 + 1],
+                                was[2] ? 0.0f : (float)cur->alpha[r0 + 2], was[3] ? 0.0f : (float)cur->alpha[r0 + 3]);
+  const float be[4] = {(float)beta_s[0], (float)beta_s[1], (float)beta_s[2], (float)beta_s[3]};
+  const int lv[4] = {live_s[0], live_s[1], live_s[2], live_s[3]};
 #pragma unroll
   for (int jn = 0; jn < kVecNodes / 256; ++jn) {
     const int n = blockIdx.x * kVecNodes + jn * 256 + threadIdx.x;
@@ -678,11 +700,16 @@ __global__ void __launch_bounds__(256) k_m_pupd(int NP, const float *kinv, const
     const size_t o = (size_t)c * NP + n;
     const float kv = kinv[(size_t)d * NP + n];
     const float4 r = R[o];
-    float4 p = P[o];
-    if (!dn[0]) p.x = kv * r.x + be[0] * p.x;
-    if (!dn[1]) p.y = kv * r.y + be[1] * p.y;
-    if (!dn[2]) p.z = kv * r.z + be[2] * p.z;
-    if (!dn[3]) p.w = kv * r.w + be[3] * p.w;
+    float4 p = P[o], x = X[o];
+    x.x += al.x * p.x;
+    x.y += al.y * p.y;
+    x.z += al.z * p.z;
+    x.w += al.w * p.w;
+    X[o] = x;
+    if (lv[0]) p.x = kv * r.x + be[0] * p.x;
+    if (lv[1]) p.y = kv * r.y + be[1] * p.y;
+    if (lv[2]) p.z = kv * r.z + be[2] * p.z;
+    if (lv[3]) p.w = kv * r.w + be[3] * p.w;
     P[o] = p;
   }
 }
@@ -1034,12 +1061,11 @@ hipError_t launch_m_iter(const Grid &g, int K, const float *A, const float *kinv
                        reinterpret_cast<const float4 *>(A), reinterpret_cast<const float4 *>(P),
                        reinterpret_cast<float4 *>(AP), part1, cur);
     hipLaunchKernelGGL(k_m_alpha<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nsl, part1, cur);
-    hipLaunchKernelGGL(k_m_upd<KK>, vgrid, dim3(256), 0, st, s.NP, kinv, reinterpret_cast<const float4 *>(P),
-                       reinterpret_cast<const float4 *>(AP), reinterpret_cast<float4 *>(X),
+    hipLaunchKernelGGL(k_m_upd<KK>, vgrid, dim3(256), 0, st, s.NP, kinv, reinterpret_cast<const float4 *>(AP),
                        reinterpret_cast<float4 *>(R), pz, pr, cur);
-    hipLaunchKernelGGL(k_m_beta<KK>, dim3(1), dim3(kRedThreads), 0, st, nbx, pz, cur, nxt);
-    hipLaunchKernelGGL(k_m_pupd<KK>, vgrid, dim3(256), 0, st, s.NP, kinv, reinterpret_cast<const float4 *>(R),
-                       reinterpret_cast<float4 *>(P), nxt);
+    hipLaunchKernelGGL(k_m_pupd<KK>, vgrid, dim3(kRedThreads), 0, st, s.NP, kinv,
+                       reinterpret_cast<const float4 *>(R), reinterpret_cast<float4 *>(P),
+                       reinterpret_cast<float4 *>(X), pz, cur, nxt);
   });
   return hipGetLastError();
 }

@@ -113,3 +113,21 @@ def test_mixed_full_size(mp):
         s64 = s.ave_stress(u)
         print(f"50^3 strain {k}: rel {rel(sig[k], s64):.2e}; stats {stats}", flush=True)
         assert rel(sig[k], s64) < 1e-6
+
+
+@pytest.mark.parametrize("K", [4, 12, 16])
+def test_mixed_solve_is_deterministic(mp, K):
+    """Repeated solves agree bit for bit: the SpMV's LDS staging (direct global-to-LDS loads shared by the
+    field-group waves) and every reduction run in a fixed order, so any lost synchronisation would show as a
+    difference between runs.  Several fields stop early, so waves with stopped fields are exercised too."""
+    r = mp.Rve(31, mp.MIC3D_RANDOM_SPHERES, (77.0, 6.0, 0.05, 0.2), elastic(mp))
+    rng = np.random.default_rng(5)
+    eps = rng.uniform(-5e-3, 5e-3, (K, 6))
+    eps[::3] *= 1e-3  # small fields reach the absolute floor at other iterations
+    ms = mp.MSolver(r, K)
+    first, st0, _, ok0 = ms.solve(eps)
+    assert ok0
+    for rep in range(3):
+        sig, st, _, ok = mp.MSolver(r, K).solve(eps) if rep == 1 else ms.solve(eps)
+        assert ok and st == st0
+        assert np.array_equal(sig, first), f"repeat {rep}: max diff {np.abs(sig - first).max():.3e}"
