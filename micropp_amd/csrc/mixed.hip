@@ -416,7 +416,7 @@ __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const doubl
 
 // ---------------------------------------------------------------------------
 // Inner CG (the reference's Jacobi-PCG recurrence, src/ell.cpp:80-117), one
-// iteration = k_m_spmv, k_m_alpha, k_m_upd, k_m_pupd.
+// iteration = k_m_spmv, k_m_upd, k_m_pupd.
 //
 // k_m_spmv: AP = A P for every field and the partials of (p, Ap).  One block
 // per slice of 64 nodes; wave g handles field group g (fields 4g .. 4g+3), so
@@ -542,33 +542,34 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
   }
 }
 
-// k_m_alpha (one block): alpha = (r, z) / (p, Ap) of every live field
-template <int K>
-__global__ void __launch_bounds__(kRedThreads) k_m_alpha(int nsl, const double *part1, MState *cur) {
-  if (cur->all_done) return;
-  const double pap = rows_sum<K>(part1, nsl, nsl);
-  const int f = threadIdx.x;
-  if (f < K && !cur->done[f]) cur->alpha[f] = cur->rz[f] / pap;
-}
-
-// k_m_upd: R -= alpha AP; partials of (z, z) and (r, z) with z = kinv R.
-// Grid (blocks of kVecNodes nodes, 3 K/4 components c = d * K/4 + g), each
-// thread kVecNodes / 256 nodes; partial tables pz / pr
+// k_m_upd: alpha = (r, z) / (p, Ap); R -= alpha AP; partials of (z, z) and
+// (r, z) with z = kinv R.  Every block re-sums the (p, Ap) partials of its
+// four fields (part1[field * nsl + slice]) in the same fixed order, so all
+// blocks agree bit for bit; block (0, g) records alpha of group g in the
+// state for k_m_pupd.  Grid (blocks of kVecNodes nodes, 3 K/4 components
+// c = d * K/4 + g), each thread kVecNodes / 256 nodes; partial tables pz / pr
 // [(field * 3 + d) * gridDim.x + block] (the 3 gridDim.x partials of a field
 // contiguous; pr = pz + 3 K gridDim.x).  X is advanced in k_m_pupd, which
 // reads P anyway.
 template <int K>
-__global__ void __launch_bounds__(256) k_m_upd(int NP, const float *kinv, const float4 *AP, float4 *R, double *pz,
-                                               double *pr, const MState *cur) {
+__global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const double *part1, const float *kinv,
+                                                       const float4 *AP, float4 *R, double *pz, double *pr,
+                                                       MState *cur) {
   __shared__ double lds[4 * 8];
+  __shared__ double al_s[4];
   constexpr int KG = K / 4;
   const int c = blockIdx.y, grp = c % KG, d = c / KG, r0 = grp * 4;
   if (cur->all_done) return;
   if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole block
-  const float4 al = make_float4(cur->done[r0] ? 0.0f : (float)cur->alpha[r0],
-                                cur->done[r0 + 1] ? 0.0f : (float)cur->alpha[r0 + 1],
-                                cur->done[r0 + 2] ? 0.0f : (float)cur->alpha[r0 + 2],
-                                cur->done[r0 + 3] ? 0.0f : (float)cur->alpha[r0 + 3]);
+  const double pap = rows_sum<4>(part1 + (size_t)r0 * nsl, nsl, nsl);
+  if (threadIdx.x < 4) {
+    const int r = r0 + threadIdx.x;
+    const double a = cur->done[r] ? 0.0 : cur->rz[r] / pap;
+    al_s[threadIdx.x] = a;
+    if (blockIdx.x == 0 && d == 0 && !cur->done[r]) cur->alpha[r] = a;
+  }
+  __syncthreads();
+  const float4 al = make_float4((float)al_s[0], (float)al_s[1], (float)al_s[2], (float)al_s[3]);
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int jn = 0; jn < kVecNodes / 256; ++jn) {
@@ -1060,9 +1061,8 @@ hipError_t launch_m_iter(const Grid &g, int K, const float *A, const float *kinv
     hipLaunchKernelGGL(k_m_spmv<KK>, dim3(8 * s.pers), dim3(16 * KK), 0, st, g, s.NP, s.nsl, s.pers,
                        reinterpret_cast<const float4 *>(A), reinterpret_cast<const float4 *>(P),
                        reinterpret_cast<float4 *>(AP), part1, cur);
-    hipLaunchKernelGGL(k_m_alpha<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nsl, part1, cur);
-    hipLaunchKernelGGL(k_m_upd<KK>, vgrid, dim3(256), 0, st, s.NP, kinv, reinterpret_cast<const float4 *>(AP),
-                       reinterpret_cast<float4 *>(R), pz, pr, cur);
+    hipLaunchKernelGGL(k_m_upd<KK>, vgrid, dim3(kRedThreads), 0, st, s.NP, s.nsl, part1, kinv,
+                       reinterpret_cast<const float4 *>(AP), reinterpret_cast<float4 *>(R), pz, pr, cur);
     hipLaunchKernelGGL(k_m_pupd<KK>, vgrid, dim3(kRedThreads), 0, st, s.NP, kinv,
                        reinterpret_cast<const float4 *>(R), reinterpret_cast<float4 *>(P),
                        reinterpret_cast<float4 *>(X), pz, cur, nxt);
