@@ -88,6 +88,11 @@ typedef struct {
   int its_with_A0 = 1;
   bool lin_stress = true;
   bool write_log = false;
+  // additive (no reference counterpart): Newton linear solves as FP64 defect
+  // correction around FP32 Jacobi-PCG (mppd_newton_mixed); the residual,
+  // return mapping, damage state, u and the averages stay FP64 (use_A0 is
+  // not applied to these solves)
+  bool mixed_precision = false;
 
   void print() {
     cout << "ngp  : " << ngp << endl;
@@ -105,6 +110,7 @@ typedef struct {
     cout << "its_with_A0 : " << its_with_A0 << endl;
     cout << "lin_stress : " << lin_stress << endl;
     cout << "write_log : " << write_log << endl;
+    cout << "mixed_precision : " << mixed_precision << endl;
   }
 } micropp_params_t;
 

@@ -170,6 +170,23 @@ int mppd_msolver_adjoint(mppd_msolver *m, int n, const double *eps_h, const mppd
 /* timing aid: average ms of the FP32 multi-RHS SpMV kernel and of a whole
  * inner iteration, over iters launches, events on the solver's stream */
 int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *iter_ms);
+/* newton_raphson (src/solve.cpp:28-82) with mixed-precision linear solves,
+ * for an RVE of any material law (a plastic or damaged RVE's Newton tangent
+ * included).  The Newton control flow, the residual (assembly_rhs with the
+ * return mapping / damage update), u and the norms are FP64 as in
+ * mppd_newton; each linear solve A du = b (the FP64 tangent of assembly_mat)
+ * runs as FP64 defect correction around FP32 Jacobi-PCG on an FP32 copy of
+ * the matrix's interior block, and stops on the reference CG's metric
+ * ||D^-1 (b - A du)|| < max(cg_abs_tol, cg_rel_tol ||D^-1 b||) evaluated in
+ * FP64.  mp supplies inner_tol, max_inner_its and max_outer (rel_tol /
+ * abs_tol unused; NULL: 1e-5, cg_max_its, 10).  A linear solve whose FP32
+ * inner CG breaks down (non-finite: the tangent of a softening law can be
+ * indefinite) is redone by the FP64 Jacobi-PCG of mppd_newton.  out as
+ * mppd_newton, with solver_its = inner (and any FP64 fallback) iterations;
+ * stats (may be NULL) = {outer steps, iterations, FP64 fallbacks, linear
+ * solves stopped by max_outer short of the tolerance}. */
+int mppd_newton_mixed(mppd_solver *s, const double eps[6], const double *vars_old_d, double *u_d,
+                      const mppd_newton_params *p, const mppd_mixed_params *mp, int out[3], int stats[4]);
 
 /* Timing aid (no reference counterpart; for the benchmark and rocprof
  * cross-checks): runs `iters` CG iterations on the solver's matrix with b_d

@@ -166,6 +166,16 @@ hipError_t launch_m_iter(const Grid &g, int K, const float *A, const float *kinv
 hipError_t launch_m_spmv_only(const Grid &g, int K, const float *A, const float *P, float *AP, double *part,
                               const void *state, hipStream_t st);
 hipError_t launch_m_accum(const Grid &g, int K, const float *X, double *U, hipStream_t st);
+// mixed-precision solves of an assembled FP64 ELL system (field 0 of K = 4):
+// FP32 node-block copy of the interior block + Jacobi weights; FP64 residual
+// b - A du with the outer test and inner start state; du += X
+hipError_t launch_m_from_ell(const Grid &g, const double *vals, float *A, double *kinv64, float *kinv,
+                             hipStream_t st);
+hipError_t launch_m_resid_ell(const Grid &g, int K, const int *cols, const double *vals, const double *b,
+                              const double *du, const double *kinv64, const float *kinv, float *R, float *X,
+                              float *P, double *part, int first, double rel, double abs_tol, double tau_min,
+                              int max_inner, void *state, hipStream_t st);
+hipError_t launch_m_accum_ell(const Grid &g, int K, const float *X, double *du, hipStream_t st);
 hipError_t launch_m_ave(const Grid &g, int K, const double *bmat, const material_base *mats_d, const int *et,
                         const double *U, double *part, double *out, hipStream_t st);
 }  // namespace mpp

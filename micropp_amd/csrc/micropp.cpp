@@ -52,7 +52,7 @@ struct micropp<tdim>::impl {
   int nsubiterations, mpi_rank;
   bool subiterations;
   mppd_newton_params np;
-  bool calc_ctan_lin_flag, use_A0, lin_stress, write_log_flag;
+  bool calc_ctan_lin_flag, use_A0, lin_stress, write_log_flag, mixed;
   int log_id = 0;
   ofstream log;
   double Vm = 0, Vf = 0;
@@ -81,7 +81,10 @@ struct micropp<tdim>::impl {
 
   newton_t newton(const double eps[6], const double *vars_old) {
     int out[3] = {0, 0, 0};
-    check(mppd_newton(sol, use_A0 ? A0 : nullptr, eps, vars_old, u, &np, out), "newton_raphson");
+    if (mixed)
+      check(mppd_newton_mixed(sol, eps, vars_old, u, &np, nullptr, out, nullptr), "newton_raphson (mixed)");
+    else
+      check(mppd_newton(sol, use_A0 ? A0 : nullptr, eps, vars_old, u, &np, out), "newton_raphson");
     newton_t n;
     n.its = out[0];
     n.solver_its = out[1];
@@ -176,12 +179,13 @@ struct micropp<tdim>::impl {
   // The batch is made at the first homogenize, sized to half the free device
   // memory; MPPD_GP_BATCH=0 in the environment keeps the one-GP-at-a-time
   // path, MPPD_GP_BATCH=n caps the slots at n.  With A0 (Newton on the
-  // initial matrix) or fewer than two FE GPs the one-GP path runs.
+  // initial matrix), with mixed precision (the batch solves in FP64) or with
+  // fewer than two FE GPs the one-GP path runs.
   bool batch_ready() {
     if (batch) return true;
     if (batch_tried) return false;
     batch_tried = true;
-    if (use_A0 || fe_ids.size() < 2) return false;
+    if (use_A0 || mixed || fe_ids.size() < 2) return false;
     const char *env = getenv("MPPD_GP_BATCH");
     const long cap = env ? strtol(env, nullptr, 10) : -1;
     if (cap == 0) return false;
@@ -334,6 +338,7 @@ micropp<tdim>::micropp(const micropp_params_t &params) : p_(new impl()) {
   s.use_A0 = params.use_A0;
   s.lin_stress = params.lin_stress;
   s.write_log_flag = params.write_log;
+  s.mixed = params.mixed_precision;
   try {
     if (s.ngp < 0) throw std::invalid_argument("micropp_amd: ngp must be >= 0");
     if (s.micro_type < 0 || s.micro_type > MIC3D_RANDOM_SPHERES)

@@ -36,6 +36,7 @@
 // SpMV maps slices to XCDs so that each XCD's L2 holds a contiguous slab of
 // nodes.
 #include "internal.hpp"
+#include "kernels.hpp"
 #include "physics.hpp"
 #include "reduce.hpp"
 
@@ -960,6 +961,105 @@ __global__ void __launch_bounds__(256) k_m_rows(const double *part, int n, doubl
 }
 
 // ---------------------------------------------------------------------------
+// Mixed-precision solves of an assembled FP64 ELL system (any material law:
+// the Newton tangent of a plastic or damaged RVE).  The inner FP32 CG runs on
+// the node-block copy of the matrix's interior block; the defect-correction
+// residual b - A du is taken from the FP64 ELL matrix itself and du is kept
+// in FP64.  Field 0 of a 4-field system carries the solve; fields 1-3 start
+// from a zero residual and stop at once.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t sell_ix(int row, int slot) {
+  return ((size_t)(row >> 6) * kSlots + slot) * 64 + (row & 63);
+}
+
+// FP32 node blocks of the interior block of the ELL matrix (ELL slot nb*3+fj
+// of row 3n+fi is node-block slot nb*9+fi*3+fj; columns of Dirichlet DOFs and
+// the rows of boundary and padding nodes are zero), and the Jacobi weights
+// 1 / A_nn(d, d) (FP64 and FP32, zero off the interior)
+__global__ void __launch_bounds__(256) k_m_from_ell(Grid g, int NP, const double *vals, float *A, double *kinv64,
+                                                    float *kinv) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= NP) return;
+  float *base = A + ((size_t)(n >> 6) * kNodeQuads * 64 + (n & 63)) * 4;
+  auto put = [&](int s, float v) { base[(size_t)(s >> 2) * 256 + (s & 3)] = v; };
+  int i = 0, j = 0, k = 0;
+  bool interior = false;
+  if (n < g.nn) {
+    i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
+    interior = !node_on_boundary(g, i, j, k);
+  }
+  for (int nb = 0; nb < 27; ++nb) {
+    const int dz = nb / 9 - 1, dy = (nb / 3) % 3 - 1, dx = nb % 3 - 1;
+    const bool keep = interior && !node_on_boundary(g, i + dx, j + dy, k + dz);
+    for (int fi = 0; fi < 3; ++fi)
+      for (int fj = 0; fj < 3; ++fj)
+        put(nb * 9 + fi * 3 + fj, keep ? (float)vals[sell_ix(3 * n + fi, nb * 3 + fj)] : 0.0f);
+  }
+  put(kNodeQuads * 4 - 1, 0.0f);
+  for (int d = 0; d < 3; ++d) {
+    const double w = interior ? 1.0 / vals[sell_ix(3 * n + d, 13 * 3 + d)] : 0.0;
+    kinv64[(size_t)d * NP + n] = w;
+    kinv[(size_t)d * NP + n] = (float)w;
+  }
+}
+
+// r = b - A du on the interior rows (FP64, the ELL matrix; du is zero on the
+// boundary), one thread per row; the FP32 CG start R = r, X = 0, P = kinv R
+// in field 0, zero in the other fields; partials as k_m_resid
+// (part[q * gridDim.x + block]: q < K the FP64 ||kinv r||^2 of field q,
+// q >= K the (R, P) of field q - K)
+template <int K>
+__global__ void __launch_bounds__(256) k_m_resid_ell(Grid g, int NP, const int *cols, const double *vals,
+                                                     const double *b, const double *du, const double *kinv64,
+                                                     const float *kinv, float4 *R, float4 *X, float4 *P,
+                                                     double *part) {
+  constexpr int KG = K / 4;
+  __shared__ double lds[4 * 2 * K];
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  double zz = 0.0, rz = 0.0;
+  if (row < g.nrow) {
+    const int n = row / 3, d = row % 3;
+    const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
+    float r32 = 0.0f, p32 = 0.0f;
+    if (!node_on_boundary(g, i, j, k)) {
+      const double r64 = b[row] - row_dot(cols, vals, du, row);
+      r32 = (float)r64;
+      p32 = kinv[(size_t)d * NP + n] * r32;
+      const double z64 = kinv64[(size_t)d * NP + n] * r64;
+      zz = z64 * z64;
+      rz = (double)r32 * (double)p32;
+    }
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int gq = 0; gq < KG; ++gq) {
+      const size_t o = (size_t)(d * KG + gq) * NP + n;
+      R[o] = gq ? zero : make_float4(r32, 0.0f, 0.0f, 0.0f);
+      X[o] = zero;
+      P[o] = gq ? zero : make_float4(p32, 0.0f, 0.0f, 0.0f);
+    }
+  }
+  double v[2 * K];
+#pragma unroll
+  for (int q = 0; q < 2 * K; ++q) v[q] = 0.0;
+  v[0] = zz;
+  v[K] = rz;
+  block_sum<2 * K, 256>(v, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < 2 * K; ++q) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
+}
+
+// du += X (field 0), FP64, one thread per row
+template <int K>
+__global__ void __launch_bounds__(256) k_m_accum_ell(Grid g, int NP, const float4 *X, double *du) {
+  constexpr int KG = K / 4;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= g.nrow) return;
+  const int n = row / 3, d = row % 3;
+  du[row] += (double)X[(size_t)(d * KG) * NP + n].x;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static unsigned nblk(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -986,7 +1086,7 @@ size_t mixed_matrix_floats(const Grid &g) { return (size_t)mshape(g).NP * kNodeQ
 size_t mixed_vector_len(const Grid &g, int K) { return (size_t)mshape(g).NP * 3 * K; }
 size_t mixed_partials(const Grid &g, int K) {
   const MLaunch s = mshape(g);
-  const size_t resid = (size_t)s.nb * 2 * K;
+  const size_t resid = std::max((size_t)s.nb, (size_t)g.nrow_pad / 256) * 2 * K;  // k_m_resid / k_m_resid_ell
   const size_t ave = (size_t)nblk(g.nelem, 256) * 6 * K;
   const size_t iter = (size_t)K * s.nsl + 2 * (size_t)3 * K * (s.NP / 256);
   return std::max(std::max(resid, ave), iter);
@@ -1077,6 +1177,38 @@ hipError_t launch_m_spmv_only(const Grid &g, int K, const float *A, const float 
   MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_spmv<KK>, dim3(8 * s.pers), dim3(16 * KK), 0, st, g, s.NP, s.nsl, s.pers,
                                        reinterpret_cast<const float4 *>(A), reinterpret_cast<const float4 *>(P),
                                        reinterpret_cast<float4 *>(AP), part, cur));
+  return hipGetLastError();
+}
+
+hipError_t launch_m_from_ell(const Grid &g, const double *vals, float *A, double *kinv64, float *kinv,
+                             hipStream_t st) {
+  const MLaunch s = mshape(g);
+  hipLaunchKernelGGL(k_m_from_ell, dim3(s.NP / 256), dim3(256), 0, st, g, s.NP, vals, A, kinv64, kinv);
+  return hipGetLastError();
+}
+
+hipError_t launch_m_resid_ell(const Grid &g, int K, const int *cols, const double *vals, const double *b,
+                              const double *du, const double *kinv64, const float *kinv, float *R, float *X,
+                              float *P, double *part, int first, double rel, double abs_tol, double tau_min,
+                              int max_inner, void *state, hipStream_t st) {
+  const MLaunch s = mshape(g);
+  MState *ms = static_cast<MState *>(state);
+  MOuter *mo = reinterpret_cast<MOuter *>(ms + 2);
+  const int nbr = g.nrow_pad / 256;
+  MPP_K_DISPATCH(K, {
+    hipLaunchKernelGGL(k_m_resid_ell<KK>, dim3(nbr), dim3(256), 0, st, g, s.NP, cols, vals, b, du, kinv64, kinv,
+                       reinterpret_cast<float4 *>(R), reinterpret_cast<float4 *>(X), reinterpret_cast<float4 *>(P),
+                       part);
+    hipLaunchKernelGGL(k_m_outer<KK>, dim3(1), dim3(kRedThreads), 0, st, nbr, part, first, rel, abs_tol, tau_min,
+                       max_inner, mo, ms);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_m_accum_ell(const Grid &g, int K, const float *X, double *du, hipStream_t st) {
+  const MLaunch s = mshape(g);
+  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_accum_ell<KK>, dim3(g.nrow_pad / 256), dim3(256), 0, st, g, s.NP,
+                                       reinterpret_cast<const float4 *>(X), du));
   return hipGetLastError();
 }
 

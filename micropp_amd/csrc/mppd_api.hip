@@ -63,7 +63,26 @@ struct mppd_solver {
   double *partials = nullptr;
   CgState *cg = nullptr, *cg_h = nullptr;
   RedState *red = nullptr, *red_h = nullptr;
+  struct MixedLin *ml = nullptr;  // mppd_newton_mixed buffers, made on first use
 };
+
+// FP32 inner-solve buffers of mppd_newton_mixed: a 4-field system (field 0
+// carries the Newton correction) in the layout of mixed.hip
+struct MixedLin {
+  static constexpr int K = 4;
+  float *A = nullptr, *kinv = nullptr, *R = nullptr, *X = nullptr, *P = nullptr, *AP = nullptr;
+  double *kinv64 = nullptr, *part = nullptr;
+  void *state = nullptr, *state_h = nullptr;
+};
+
+static void mixed_lin_free(MixedLin *m) {
+  if (!m) return;
+  for (void *p : {(void *)m->A, (void *)m->kinv, (void *)m->R, (void *)m->X, (void *)m->P, (void *)m->AP,
+                  (void *)m->kinv64, (void *)m->part, m->state})
+    dfree(p);
+  if (m->state_h) (void)hipHostFree(m->state_h);
+  delete m;
+}
 
 extern "C" {
 
@@ -306,6 +325,7 @@ int mppd_solver_destroy(mppd_solver *s) {
     dfree(p);
   if (s->cg_h) (void)hipHostFree(s->cg_h);
   if (s->red_h) (void)hipHostFree(s->red_h);
+  mixed_lin_free(s->ml);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
   return MPPD_OK;
@@ -640,6 +660,60 @@ int mppd_msolver_destroy(mppd_msolver *m) {
 // a_v / V of field v), then n forward fields with the macro strains eps_h,
 // then idle fields.  FP64 defect correction with FP32 inner CG until every
 // field converges; the solution is left in m->U.
+// Inner FP32 CG of a K-field system, from the start state the outer test has
+// written (state_h holds it), queued in chunks; after q iterations the state
+// is state[q & 1].  The next chunk is sized from each live field's observed
+// convergence rate (iterations still needed, + 2), so few iterations are
+// queued past the end.  *its = iterations run; *finite (may be NULL) = every
+// field's ||z|| is finite at the end (no breakdown).
+static int mixed_inner(const Grid &g, int K, const float *A, const float *kinv, float *R, float *X, float *P,
+                       float *AP, double *part, void *state, void *state_h, int max_inner, hipStream_t st,
+                       int *its_out, bool *finite = nullptr) {
+  int queued = 0, its = 0, done = 0, its_prev = 0;
+  int chunk = 32;
+  std::vector<double> pn_prev(K, 0.0);
+  for (int k = 0; k < K; ++k) {
+    double target;
+    (void)mixed_state_field(state_h, 0, k, &pn_prev[k], &target);
+  }
+  while (!done) {
+    const int nq = std::min(chunk, max_inner - queued);
+    if (nq <= 0) break;
+    for (int i = 0; i < nq; ++i) MPP_HIP(launch_m_iter(g, K, A, kinv, R, X, P, AP, part, state, queued + i, st));
+    queued += nq;
+    MPP_HIP(hipMemcpyAsync(state_h, state, mixed_state_bytes(), hipMemcpyDeviceToHost, st));
+    MPP_HIP(hipStreamSynchronize(st));
+    mixed_state_view(state_h, queued, &its, &done);
+    if (done) break;
+    int next = 0;
+    const int dit = its - its_prev;
+    for (int k = 0; k < K; ++k) {
+      double pnk, target;
+      if (mixed_state_field(state_h, queued, k, &pnk, &target)) continue;
+      int need = chunk * 2;
+      if (dit > 0 && pn_prev[k] > 0 && pnk > 0 && pnk < pn_prev[k]) {
+        const double rate = std::log(pnk / pn_prev[k]) / dit;
+        const double est = std::log(target / pnk) / rate;
+        if (std::isfinite(est)) need = (int)std::ceil(est) + 2;
+      }
+      next = std::max(next, need);
+      pn_prev[k] = pnk;
+    }
+    its_prev = its;
+    chunk = std::max(4, std::min(256, next));
+  }
+  *its_out = its;
+  if (finite) {
+    *finite = true;
+    for (int k = 0; k < K; ++k) {
+      double pnk, target;
+      (void)mixed_state_field(state_h, queued, k, &pnk, &target);
+      if (!std::isfinite(pnk)) *finite = false;
+    }
+  }
+  return MPPD_OK;
+}
+
 static int msolver_run(mppd_msolver *m, int nadj, int n, const double *eps_h, const mppd_mixed_params *p,
                        int stats[2], std::vector<double> &pn, std::vector<double> &pn0, int *all_conv_out) {
   const Grid &g = m->r->g;
@@ -666,43 +740,9 @@ static int msolver_run(mppd_msolver *m, int nadj, int n, const double *eps_h, co
     MPP_HIP(hipStreamSynchronize(m->st));
     mixed_outer_view(m->state_h, K, &all_conv, pn.data(), pn0.data());
     if (all_conv || outer >= p->max_outer) break;
-    // inner FP32 CG, queued in chunks; after q iterations the state is state[q & 1].
-    // The next chunk is sized from each live field's observed convergence rate
-    // (iterations still needed, + 2), so few iterations are queued past the end.
-    int queued = 0, its = 0, done = 0, its_prev = 0;
-    int chunk = 32;
-    std::vector<double> pn_prev(K, 0.0);
-    for (int k = 0; k < K; ++k) {  // the start state read with the outer test
-      double target;
-      (void)mixed_state_field(m->state_h, 0, k, &pn_prev[k], &target);
-    }
-    while (!done) {
-      const int nq = std::min(chunk, p->max_inner_its - queued);
-      if (nq <= 0) break;
-      for (int i = 0; i < nq; ++i)
-        MPP_HIP(launch_m_iter(g, K, m->A, m->kinv, m->R, m->X, m->P, m->AP, m->part, m->state, queued + i, m->st));
-      queued += nq;
-      MPP_HIP(hipMemcpyAsync(m->state_h, m->state, mixed_state_bytes(), hipMemcpyDeviceToHost, m->st));
-      MPP_HIP(hipStreamSynchronize(m->st));
-      mixed_state_view(m->state_h, queued, &its, &done);
-      if (done) break;
-      int next = 0;
-      const int dit = its - its_prev;
-      for (int k = 0; k < K; ++k) {
-        double pnk, target;
-        if (mixed_state_field(m->state_h, queued, k, &pnk, &target)) continue;
-        int need = chunk * 2;
-        if (dit > 0 && pn_prev[k] > 0 && pnk > 0 && pnk < pn_prev[k]) {
-          const double rate = std::log(pnk / pn_prev[k]) / dit;
-          const double est = std::log(target / pnk) / rate;
-          if (std::isfinite(est)) need = (int)std::ceil(est) + 2;
-        }
-        next = std::max(next, need);
-        pn_prev[k] = pnk;
-      }
-      its_prev = its;
-      chunk = std::max(4, std::min(256, next));
-    }
+    int its = 0, rc = mixed_inner(g, K, m->A, m->kinv, m->R, m->X, m->P, m->AP, m->part, m->state, m->state_h,
+                                  p->max_inner_its, m->st, &its);
+    if (rc) return rc;
     inner_total += its;
     MPP_HIP(launch_m_accum(g, K, m->X, m->U, m->st));
     ++outer;
@@ -777,6 +817,117 @@ int mppd_msolver_get_u(mppd_msolver *m, int k, double *u_h) {
   const size_t np = U.size() / (3 * (size_t)m->K);
   for (int n = 0; n < g.nn; ++n)
     for (int d = 0; d < 3; ++d) u_h[(size_t)n * 3 + d] = U[((size_t)d * m->K + k) * np + n];
+  return MPPD_OK;
+}
+
+// newton_raphson (src/solve.cpp:28-82) with mixed-precision linear solves.
+// Each Newton step assembles the FP64 tangent (assembly_mat: any law), copies
+// its interior block to FP32 node blocks, and solves A du = b by FP64 defect
+// correction around the FP32 Jacobi-PCG of mixed.hip; the defect-correction
+// residual comes from the FP64 ELL matrix and du stays FP64.  The Newton
+// residual (assembly_rhs, with the return mapping or damage update of every
+// Gauss point), u and the norms stay FP64, and the Newton control flow is the
+// reference's.  The linear solve stops on the reference CG's metric,
+// ||D^-1 (b - A du)|| < max(cg_abs_tol, cg_rel_tol ||D^-1 b||), in FP64.
+// Should the FP32 inner CG break down (a non-finite ||z||: the tangent of a
+// softening law can be indefinite), that linear solve is redone by the FP64
+// Jacobi-PCG of mppd_newton, so the result is never worse than the FP64 path.
+int mppd_newton_mixed(mppd_solver *s, const double eps[6], const double *vars_old_d, double *u_d,
+                      const mppd_newton_params *p, const mppd_mixed_params *mp, int out[3], int stats[4]) {
+  if (!s || !eps || !u_d || !p || !out) return fail(MPPD_ERR_ARG, "null argument");
+  const Grid &g = s->r->g;
+  constexpr int K = MixedLin::K;
+  const double inner_tol = mp ? mp->inner_tol : 1e-5;
+  const int max_inner = mp ? mp->max_inner_its : p->cg_max_its;
+  const int max_outer = mp ? mp->max_outer : 10;
+  if (max_inner < 1 || max_outer < 1) return fail(MPPD_ERR_ARG, "bad mixed parameters");
+  int rc = MPPD_OK;
+  if (!s->ml) {
+    MixedLin *ml = new MixedLin();
+    s->ml = ml;
+    const size_t nv = mixed_vector_len(g, K), NP = nv / (3 * K);
+    if ((rc = dalloc(&ml->A, mixed_matrix_floats(g)))) return rc;
+    for (float **v : {&ml->R, &ml->X, &ml->P, &ml->AP})
+      if ((rc = dalloc(v, nv))) return rc;
+    if ((rc = dalloc(&ml->kinv, 3 * NP))) return rc;
+    if ((rc = dalloc(&ml->kinv64, 3 * NP))) return rc;
+    if ((rc = dalloc(&ml->part, mixed_partials(g, K)))) return rc;
+    char *st = nullptr;
+    if ((rc = dalloc(&st, mixed_state_bytes()))) return rc;
+    ml->state = st;
+    MPP_HIP(hipHostMalloc(&ml->state_h, mixed_state_bytes(), hipHostMallocDefault));
+  }
+  MixedLin *ml = s->ml;
+  int its = 0, solver_its = 0, conv = 0, outers = 0, fallbacks = 0, short_solves = 0;
+  double pn[K], pn0[K];
+  MPP_HIP(launch_set_displ_bc(g, eps, u_d, s->st));
+  MPP_HIP(rhs_async(s, u_d, vars_old_d, s->b));
+  MPP_HIP(read_red(s));
+  double norm = s->red_h->v[0];
+  const double norm0 = norm;
+  while (its < p->nr_max_its) {
+    if (norm < p->nr_max_tol || norm < norm0 * p->nr_rel_tol) {
+      conv = 1;
+      break;
+    }
+    MPP_HIP(mat_async(s, u_d, vars_old_d));
+    MPP_HIP(launch_m_from_ell(g, s->vals, ml->A, ml->kinv64, ml->kinv, s->st));
+    MPP_HIP(hipMemsetAsync(s->du, 0, sizeof(double) * g.nrow_pad, s->st));
+    bool broke = false;
+    for (int outer = 0;; ++outer) {
+      MPP_HIP(launch_m_resid_ell(g, K, s->cols, s->vals, s->b, s->du, ml->kinv64, ml->kinv, ml->R, ml->X, ml->P,
+                                 ml->part, outer == 0, p->cg_rel_tol, p->cg_abs_tol, inner_tol, max_inner,
+                                 ml->state, s->st));
+      MPP_HIP(hipMemcpyAsync(ml->state_h, ml->state, mixed_state_bytes(), hipMemcpyDeviceToHost, s->st));
+      MPP_HIP(hipStreamSynchronize(s->st));
+      int all_conv = 0;
+      mixed_outer_view(ml->state_h, K, &all_conv, pn, pn0);
+      if (!std::isfinite(pn[0])) {
+        broke = true;
+        break;
+      }
+      if (all_conv) break;
+      if (outer >= max_outer) {
+        ++short_solves;
+        break;
+      }
+      int inner = 0;
+      bool finite = true;
+      if ((rc = mixed_inner(g, K, ml->A, ml->kinv, ml->R, ml->X, ml->P, ml->AP, ml->part, ml->state, ml->state_h,
+                            max_inner, s->st, &inner, &finite)))
+        return rc;
+      solver_its += inner;
+      ++outers;
+      if (!finite) {
+        broke = true;
+        break;
+      }
+      MPP_HIP(launch_m_accum_ell(g, K, ml->X, s->du, s->st));
+    }
+    if (broke) {  // the FP64 Jacobi-PCG of mppd_newton on the same system, from 0
+      int cg_its = 0;
+      double err = 0;
+      if ((rc = cg_run(s, s, s->b, p->cg_abs_tol, p->cg_rel_tol, p->cg_max_its, &cg_its, &err))) return rc;
+      solver_its += cg_its;
+      ++fallbacks;
+      MPP_HIP(launch_axpy(g.nrow, 1.0, s->x, u_d, s->st));
+    } else {
+      MPP_HIP(launch_axpy(g.nrow, 1.0, s->du, u_d, s->st));
+    }
+    MPP_HIP(rhs_async(s, u_d, vars_old_d, s->b));
+    MPP_HIP(read_red(s));
+    norm = s->red_h->v[0];
+    its++;
+  }
+  out[0] = its;
+  out[1] = solver_its;
+  out[2] = conv;
+  if (stats) {
+    stats[0] = outers;
+    stats[1] = solver_its;
+    stats[2] = fallbacks;
+    stats[3] = short_solves;
+  }
   return MPPD_OK;
 }
 

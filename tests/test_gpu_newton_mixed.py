@@ -1,0 +1,114 @@
+"""GPU: Newton with mixed-precision linear solves for plastic and damaged RVEs.
+
+mppd_newton_mixed keeps the reference's Newton loop, residual (assembly_rhs
+with the return mapping or damage update), u and norms in FP64 and solves
+each tangent system by FP64 defect correction around FP32 Jacobi-PCG, to the
+reference CG's metric evaluated in FP64.
+
+Bars:
+- J2 plasticity: sigma within 1e-6 relative of the FP64 path and of the
+  oracle, ctan within 1e-4 (ctan is a forward difference over a 1e-8 strain
+  step, so it multiplies solve-level differences by ~1e8 / |sigma|; the FP64
+  path is held to 1e-6 by its own parity tests).
+- Damage: sigma within 1e-4, ctan within 1e-2.  With the reference's damage
+  law the FD tangent of a damaged RVE is nonsymmetric and ill-conditioned and
+  the reference Newton stops at nr_max_its without converging, so the
+  returned state carries the linear solves' error (about the condition
+  number times the CG tolerance: up to ~4e-5 on this load path, ~1e-13 at
+  the points that stay well conditioned), which two correct solvers with
+  different rounding do not share; the FP64 path agrees with the oracle more
+  closely only because both use the same arithmetic order.  The load path
+  keeps every reference CG below its iteration cap (checked on the oracle).
+"""
+import numpy as np
+import pytest
+
+from oracle import bindings as ob
+
+pytestmark = pytest.mark.gpu
+
+LAWS = {
+    "plastic": [(1, 1e6, 0.3, 1e5, 2e3), (0, 1e7, 0.25), (0, 1e7, 0.25)],
+    "damage": [(2, 1e6, 0.3, 0, 0, 1e4), (0, 1e7, 0.25), (0, 1e7, 0.25)],  # threshold Xt / sqrt(E) = 10
+}
+AMP = {"plastic": 4e-3, "damage": 0.015}
+KW = dict(lin_stress=0, nr_max_its=6, nr_rel_tol=1e-8, cg_rel_tol=1e-12, cg_max_its=4000)
+
+
+@pytest.fixture(scope="module")
+def mp():
+    import micropp_amd
+
+    micropp_amd.load()
+    if not micropp_amd.gpu_available():
+        pytest.skip("no HIP device")
+    return micropp_amd
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("kind", list(LAWS))
+def test_micropp_mixed_newton(mp, oracle, kind):
+    cp = [mp.FE_FULL, mp.FE_ONE_WAY]
+    mats = [mp.material(*s) for s in LAWS[kind]]
+    size = (9, 10, 11)
+    mx = mp.Micropp3(size, mp.MIC_SPHERE, (0.3, 0, 0, 0), mats, ngp=2, coupling=cp, mixed_precision=1, **KW)
+    fp = mp.Micropp3(size, mp.MIC_SPHERE, (0.3, 0, 0, 0), mats, ngp=2, coupling=cp, **KW)
+    p, k = ob.make_params(size, mp.MIC_SPHERE, (0.3, 0, 0, 0), [ob.material(*s) for s in LAWS[kind]], ngp=2,
+                          coupling=cp, **KW)
+    orc = oracle.micro(p, k)
+    rng = np.random.default_rng(3)
+    d = [rng.uniform(-1, 1, 6) * AMP[kind], rng.uniform(-1, 1, 6) * AMP[kind] * 0.7]
+    steps, nl = (3, 0) if kind == "plastic" else (2, 0)
+    tol_s, tol_c = (1e-6, 1e-4) if kind == "plastic" else (1e-4, 1e-2)
+    for t in range(steps):
+        for m in (mx, fp, orc):
+            for g in range(2):
+                m.set_strain(g, d[g] * (t + 1) / 3)
+            m.homogenize()
+        for g in range(2):
+            s_mx, s_fp, s_or = mx.get_stress(g), fp.get_stress(g), orc.get_stress(g)
+            c_mx, c_fp = mx.get_ctan(g), fp.get_ctan(g)
+            print(f"{kind} step {t} gp {g}: nl {mx.is_non_linear(g)} cost mixed {mx.get_cost(g)} fp64 "
+                  f"{fp.get_cost(g)}; sigma mixed-fp64 {rel(s_mx, s_fp):.2e} mixed-oracle {rel(s_mx, s_or):.2e}; "
+                  f"ctan mixed-fp64 {rel(c_mx, c_fp):.2e}", flush=True)
+            assert rel(s_mx, s_fp) < tol_s and rel(s_mx, s_or) < tol_s
+            assert rel(c_mx, c_fp) < tol_c
+            assert mx.is_non_linear(g) == fp.is_non_linear(g)
+            nl += mx.is_non_linear(g)
+        for m in (mx, fp, orc):
+            m.update_vars()
+    assert nl > 0, "the case must reach the nonlinear range"
+
+
+def test_solver_newton_mixed_matches_fp64(mp):
+    """mppd_newton_mixed against mppd_newton on a yielding J2 RVE at 21^3 (history from zero): the same Newton
+    iteration count and convergence flag, sigma within 1e-6, and no FP64 fallback or short solve."""
+    mats = [mp.material(*s) for s in LAWS["plastic"]]
+    r = mp.Rve(21, mp.MIC3D_RANDOM_SPHERES, (11.0, 6.0, 0.05, 0.2), mats)
+    s = mp.Solver(r)
+    eps = np.array([4e-3, -1e-3, 5e-4, 2e-3, 0.0, -1e-3])
+    u64, u32 = mp.DeviceArray(r.nrow), mp.DeviceArray(r.nrow)
+    kw = dict(nr_max_its=6, nr_rel_tol=1e-8, cg_rel_tol=1e-12, cg_max_its=4000)
+    out64 = s.newton(eps, u64, **kw)
+    out32, stats = s.newton_mixed(eps, u32, **kw)
+    sig64, sig32 = s.ave_stress(u64), s.ave_stress(u32)
+    vn = mp.DeviceArray(r.nvars)
+    nl = s.vars_new(u64, None, vn)
+    print(f"fp64 {out64}, mixed {out32}, stats (outer, its, fallbacks, short) {stats}; nonlinear {nl}; "
+          f"sigma rel {rel(sig32, sig64):.2e}", flush=True)
+    assert nl == 1
+    assert out32[0] == out64[0] and out32[2] == out64[2]
+    assert stats[2] == 0 and stats[3] == 0
+    assert rel(sig32, sig64) < 1e-6
+
+
+def test_newton_mixed_rejects_bad_arguments(mp):
+    r = mp.Rve(5, mp.MIC_SPHERE, (0.2, 0, 0, 0), [mp.material(0, 1e6, 0.3)] * 3)
+    s = mp.Solver(r)
+    u = mp.DeviceArray(r.nrow)
+    with pytest.raises(mp.MppdError):
+        s.newton_mixed(np.zeros(6), u, max_outer=0)
