@@ -252,6 +252,41 @@ def adjoint_workload(nrve, _steps):
             "dmat_rel_err_vs_central_fd": err_fd}
 
 
+def newton_mixed_workload(nrve, steps):
+    """The 50^3 random-sphere RVE with a J2 matrix (E=1e6, nu=0.3, Sy=1e4, Ka=1e5) and elastic spheres, one
+    micropp<3> Gauss point (FE_ONE_WAY, sigma from calc_ave_stress, CG 1e-8): load steps of a yielding strain
+    ramp, FP64 Newton (the reference algorithm) vs Newton with mixed-precision linear solves."""
+    import micropp_amd as mp
+
+    steps = steps or 2
+    geo, strains = workload()
+    mats = [mp.material(1, 1e6, 0.3, 1e5, 1e4), mp.material(0, 1e7, 0.25), mp.material(0, 1e7, 0.25)]
+    kw = dict(lin_stress=0, calc_ctan_lin=0, nr_max_its=4, nr_rel_tol=1e-6, cg_rel_tol=1e-8, cg_max_its=4000)
+    d = strains[0] / np.abs(strains[0]).max() * 2e-2  # peak component 2e-2: the matrix yields
+    res = {}
+    for mixed in (0, 1):
+        m = mp.Micropp3(51, mp.MIC3D_RANDOM_SPHERES, geo, mats, ngp=1, coupling=[mp.FE_ONE_WAY],
+                        mixed_precision=mixed, **kw)
+        sig, cost, t = [], 0, 0.0
+        for k in range(steps):
+            m.set_strain(0, d * (k + 1) / steps)
+            mp.load().mppd_synchronize()
+            t0 = time.perf_counter()
+            m.homogenize()
+            t += time.perf_counter() - t0
+            sig.append(m.get_stress(0))
+            cost += m.get_cost(0)
+            m.update_vars()
+            print(f"newton_mixed: mixed={mixed} step {k} done", file=sys.stderr, flush=True)
+        res[mixed] = (np.array(sig), cost, t, m.is_non_linear(0))
+    return {"metric": "J2 Newton solves/s at 50^3 (FP64 vs mixed-precision linear solves)",
+            "value": steps / res[1][2], "unit": "Newton solves/s (one Gauss point, mixed precision)",
+            "workload": f"50^3 random-sphere RVE, J2 matrix, {steps} load steps to peak strain 2e-2",
+            "fp64_solves_per_s": steps / res[0][2], "speedup_vs_fp64": res[0][2] / res[1][2],
+            "iterations_fp64": res[0][1], "iterations_mixed": res[1][1], "nonlinear": int(res[1][3]),
+            "sigma_rel_err_vs_fp64": rel(res[1][0], res[0][0])}
+
+
 def pmc_traffic(summary):
     """HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE runs whose summary (commands, copy-probe
     scaling) is profiles/<summary>; None when the summary is absent."""
@@ -294,7 +329,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of COPY_PROBE_BYTES (scales HBM counter readings)")
     ap.add_argument("--workload", default="mixed50",
-                    choices=["mixed50", "batch_j2", "batch_damage", "adjoint"],
+                    choices=["mixed50", "batch_j2", "batch_damage", "adjoint", "newton_mixed"],
                     help="mixed50 (default, the headline); the others measure the batched and adjoint paths")
     ap.add_argument("--rves", type=int, default=0, help="RVEs of a batch / adjoint workload (0: its default)")
     ap.add_argument("--load-steps", type=int, default=0, help="load steps of a batch workload (0: its default)")
@@ -307,7 +342,8 @@ def main():
         copy_probe()
         return
     if args.workload != "mixed50":
-        fn = {"batch_j2": batch_j2, "batch_damage": batch_damage, "adjoint": adjoint_workload}[args.workload]
+        fn = {"batch_j2": batch_j2, "batch_damage": batch_damage, "adjoint": adjoint_workload,
+              "newton_mixed": newton_mixed_workload}[args.workload]
         print(json.dumps(fn(args.rves, args.load_steps)), flush=True)
         return
 

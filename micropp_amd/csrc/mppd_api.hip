@@ -668,7 +668,7 @@ int mppd_msolver_destroy(mppd_msolver *m) {
 // field's ||z|| is finite at the end (no breakdown).
 static int mixed_inner(const Grid &g, int K, const float *A, const float *kinv, float *R, float *X, float *P,
                        float *AP, double *part, void *state, void *state_h, int max_inner, hipStream_t st,
-                       int *its_out, bool *finite = nullptr) {
+                       int *its_out, bool *finite = nullptr, bool stop_on_stall = false) {
   int queued = 0, its = 0, done = 0, its_prev = 0;
   int chunk = 32;
   std::vector<double> pn_prev(K, 0.0);
@@ -687,9 +687,11 @@ static int mixed_inner(const Grid &g, int K, const float *A, const float *kinv, 
     if (done) break;
     int next = 0;
     const int dit = its - its_prev;
+    bool stalled = dit >= 32;
     for (int k = 0; k < K; ++k) {
       double pnk, target;
       if (mixed_state_field(state_h, queued, k, &pnk, &target)) continue;
+      if (pnk < 0.9 * pn_prev[k]) stalled = false;
       int need = chunk * 2;
       if (dit > 0 && pn_prev[k] > 0 && pnk > 0 && pnk < pn_prev[k]) {
         const double rate = std::log(pnk / pn_prev[k]) / dit;
@@ -701,6 +703,10 @@ static int mixed_inner(const Grid &g, int K, const float *A, const float *kinv, 
     }
     its_prev = its;
     chunk = std::max(4, std::min(256, next));
+    // FP32 CG that no longer reduces ||z|| by 10% over a chunk of >= 32
+    // iterations (rounding, or a nonsymmetric Newton tangent) is stopped: the
+    // caller's FP64 defect correction restarts it from the true residual
+    if (stalled && stop_on_stall) break;
   }
   *its_out = its;
   if (finite) {
@@ -894,7 +900,7 @@ int mppd_newton_mixed(mppd_solver *s, const double eps[6], const double *vars_ol
       int inner = 0;
       bool finite = true;
       if ((rc = mixed_inner(g, K, ml->A, ml->kinv, ml->R, ml->X, ml->P, ml->AP, ml->part, ml->state, ml->state_h,
-                            max_inner, s->st, &inner, &finite)))
+                            max_inner, s->st, &inner, &finite, true)))
         return rc;
       solver_its += inner;
       ++outers;
