@@ -136,6 +136,7 @@ def _batch_run(mp, descs, ramps, params, label):
     dt_b = time.perf_counter() - t0
     info = b.info()
     stats = b.stats()
+    _, spmv_slots, spmv_xcd = b.tuning()
     sig_b = b.stress()
     del b
     nseq = min(nrve, 4)
@@ -158,7 +159,8 @@ def _batch_run(mp, descs, ramps, params, label):
             "speedup_vs_one_by_one": (nrve * steps / dt_b) / (steps / dt_s),
             "batched_equals_one_by_one": bool(np.array_equal(np.array(sig_s), sig_b[:nseq])),
             "nonlinear_rves": int(info[:, 0].sum()), "converged_rves": int(info[:, 2].sum()),
-            "graph_chunk": int(stats[1]), "cg_iterations_queued": int(stats[0])}
+            "graph_chunk": int(stats[1]), "spmv_slots_per_round": spmv_slots, "spmv_blocks_per_xcd": spmv_xcd,
+            "cg_iterations_queued": int(stats[0])}
 
 
 def batch_j2(nrve, steps):

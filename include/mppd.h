@@ -223,8 +223,11 @@ typedef struct {
   int subiterations;         /* subiterate a non-converged Newton */
   int nsubiterations;
   int graph_chunk;           /* CG iterations per captured graph; 0 = choose by timing */
+  int spmv_chunk;            /* ELL slots per load round of the SpMV: 9, 27 or 81; 0 = choose by timing */
+  int block_order;           /* SpMV blocks: 0 in order, 1 grouped per XCD; -1 = choose by timing */
 } mppd_batch_params;
-/* the micropp_params_t defaults (include/types.hpp:43-83) with graph_chunk 0 */
+/* the micropp_params_t defaults (include/types.hpp:43-83); graph_chunk,
+ * spmv_chunk and block_order chosen by timing at mppd_batch_create */
 void mppd_batch_params_default(mppd_batch_params *p);
 int mppd_batch_create(mppd_batch **out, int nrve, const mppd_batch_rve *rves, const mppd_batch_params *p);
 int mppd_batch_destroy(mppd_batch *b);
@@ -239,6 +242,8 @@ int mppd_batch_get_info(const mppd_batch *b, int *info);
 int mppd_batch_update_vars(mppd_batch *b);
 /* stats: {CG iterations queued, graph chunk length, graph launches, device ms of the last homogenize} */
 int mppd_batch_stats(const mppd_batch *b, double stats[4]);
+/* the tuned shape: {graph chunk length, SpMV slots per load round, SpMV block order (1 = per XCD)} */
+int mppd_batch_tuning(const mppd_batch *b, int out[3]);
 
 /* ---- a batch as working slots for GPs whose state the caller keeps ----
  * micropp<3>::homogenize (src/homogenize.cpp:68-99, OpenMP over the GPs of

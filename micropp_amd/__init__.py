@@ -156,6 +156,7 @@ SIGNATURES = {
     "mppd_batch_update_vars": (_I, [_VP]),
     "mppd_batch_stats": (_I, [_VP, _PD]),
     "mppd_batch_slot_bytes": (_I, [_VP, C.POINTER(C.c_size_t)]),
+    "mppd_batch_tuning": (_I, [_VP, _PI]),
     "mppd_batch_load": (_I, [_VP, _I, _VP, _VP, _VP]),
     "mppd_batch_store": (_I, [_VP, _I, _VP, _VP, _VP]),
 }
@@ -172,7 +173,8 @@ class BatchParams(C.Structure):
     """``mppd_batch_params`` (include/mppd.h)."""
 
     _fields_ = [("newton", NewtonParams), ("lin_stress", C.c_int), ("calc_ctan_lin", C.c_int),
-                ("subiterations", C.c_int), ("nsubiterations", C.c_int), ("graph_chunk", C.c_int)]
+                ("subiterations", C.c_int), ("nsubiterations", C.c_int), ("graph_chunk", C.c_int),
+                ("spmv_chunk", C.c_int), ("block_order", C.c_int)]
 
 
 class MixedParams(C.Structure):
@@ -618,7 +620,7 @@ class Batch:
     def __init__(self, rves, **params):
         """rves: list of dicts with size, micro_type, geo, materials (3), coupling (FE_* id).
         params: newton fields (nr_max_its, nr_max_tol, nr_rel_tol, cg_max_its, cg_abs_tol, cg_rel_tol),
-        lin_stress, calc_ctan_lin, subiterations, nsubiterations, graph_chunk."""
+        lin_stress, calc_ctan_lin, subiterations, nsubiterations, graph_chunk, spmv_chunk, block_order."""
         p = BatchParams()
         load().mppd_batch_params_default(C.byref(p))
         for k, v in params.items():
@@ -684,6 +686,12 @@ class Batch:
         out = np.zeros(4)
         _chk(load().mppd_batch_stats(self.h, _dp(out)), "mppd_batch_stats")
         return out
+
+    def tuning(self):
+        """(graph chunk length, SpMV slots per load round, SpMV blocks grouped per XCD)."""
+        out = np.zeros(3, dtype=np.int32)
+        _chk(load().mppd_batch_tuning(self.h, _ip(out)), "mppd_batch_tuning")
+        return tuple(int(v) for v in out)
 
 
 def cg_profile(solver: "Solver", b: DeviceArray, iters: int):

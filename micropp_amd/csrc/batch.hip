@@ -154,10 +154,19 @@ __global__ void __launch_bounds__(256) kb_cg_init_state(const BRve *rv, const in
   cg_init_state_body(R.part, R.nb_row, min_err, rel_err, max_its, R.cg);
 }
 
-__global__ void __launch_bounds__(256) kb_cg_step1(const BRve *rv, const int *map, const int *blk0, int parity) {
-  const int rid = map[blockIdx.x];
+// The SpMV of the batch in one of its tuned shapes: CH slots of a row in
+// flight per load round; XCD: launch block b takes logical block
+// (b & 7) * per + (b >> 3), so each XCD streams a contiguous run of blocks
+// (blocks are dispatched round-robin over the 8 XCDs).  The arithmetic is
+// the same in every shape.
+template <int CH, bool XCD>
+__global__ void __launch_bounds__(256) kb_cg_step1(const BRve *rv, const int *map, const int *blk0, int nb, int per,
+                                                   int parity) {
+  const int b = XCD ? (int)((blockIdx.x & 7) * per + (blockIdx.x >> 3)) : (int)blockIdx.x;
+  if (b >= nb) return;
+  const int rid = map[b];
   const BRve &R = rv[rid];
-  cg_step1_body(blockIdx.x - blk0[rid], R.cols, R.vals, R.p, R.Ap, R.part, R.cg + (parity & 1));
+  cg_step1_body<CH>(b - blk0[rid], R.cols, R.vals, R.p, R.Ap, R.part, R.cg + (parity & 1));
 }
 
 __global__ void __launch_bounds__(256) kb_cg_step2(const BRve *rv, const int *map, const int *blk0, int parity) {
@@ -223,8 +232,24 @@ hipError_t launch_b_cg_init(const BLaunch &L, double min_err, double rel_err, in
   hipLaunchKernelGGL(kb_cg_init_state, dim3(L.nrve), dim3(256), 0, st, L.rv, L.act, min_err, rel_err, max_its);
   return hipGetLastError();
 }
+hipError_t launch_b_spmv(const BLaunch &L, int shape, int parity, hipStream_t st) {
+  const int nb = L.row.nb, per = (nb + 7) / 8;
+  const dim3 plain(nb), xcd(8 * per);
+  switch (shape) {
+    case 0: hipLaunchKernelGGL((kb_cg_step1<9, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 1: hipLaunchKernelGGL((kb_cg_step1<27, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 2: hipLaunchKernelGGL((kb_cg_step1<81, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 3: hipLaunchKernelGGL((kb_cg_step1<9, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 4: hipLaunchKernelGGL((kb_cg_step1<27, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 5: hipLaunchKernelGGL((kb_cg_step1<81, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_b_cg_iter(const BLaunch &L, int parity, hipStream_t st) {
-  hipLaunchKernelGGL(kb_cg_step1, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
+  hipError_t e = launch_b_spmv(L, L.spmv_shape, parity, st);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kb_cg_step2, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
   hipLaunchKernelGGL(kb_cg_step3, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
   return hipGetLastError();

@@ -6,6 +6,8 @@
 
 namespace mpp {
 
+constexpr int kSpmvShapes = 6, kSpmvDefault = 4;  // default: 27 slots, XCD-grouped blocks
+
 // One RVE of a batch: its geometry and every device array the FP64 path
 // touches.  Row-space arrays hold nrow_pad entries, element arrays nelem
 // entries; the history arrays are null for an all-elastic RVE.
@@ -41,6 +43,7 @@ struct BLaunch {
   const BRve *rv;
   const int *act;  // per-RVE activity mask of this launch
   BShape row, elem, ke;
+  int spmv_shape = kSpmvDefault;  // see launch_b_spmv
 };
 
 hipError_t launch_b_set_bc(const BLaunch &L, const double *eps, hipStream_t st);
@@ -59,6 +62,10 @@ hipError_t launch_b_vars_new(const BLaunch &L, int *flags, hipStream_t st);
 hipError_t launch_b_cg_init(const BLaunch &L, double min_err, double rel_err, int max_its, hipStream_t st);
 // one CG iteration of every RVE (inactive / converged RVEs return at once)
 hipError_t launch_b_cg_iter(const BLaunch &L, int parity, hipStream_t st);
+// the CG SpMV alone in one of kSpmvShapes shapes: shape = 3 * xcd + c, with
+// c = 0, 1, 2 for 9, 27, 81 slots in flight and xcd = 1 for the XCD-grouped
+// block order (timing of the autotuner; same results in every shape)
+hipError_t launch_b_spmv(const BLaunch &L, int shape, int parity, hipStream_t st);
 // uw += x
 hipError_t launch_b_axpy(const BLaunch &L, hipStream_t st);
 

@@ -63,6 +63,7 @@ struct mppd_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int gchunk = 0;
+  int spmv_shape = kSpmvDefault;  // tuned SpMV shape (launch_b_spmv)
   double cg_queued = 0, graph_launches = 0, last_ms = 0;
 
   BLaunch launch(int mask) const {
@@ -73,6 +74,7 @@ struct mppd_batch {
     L.row = {nb[0], map_d[0], blk0_d[0]};
     L.elem = {nb[1], map_d[1], blk0_d[1]};
     L.ke = {nb[2], map_d[2], blk0_d[2]};
+    L.spmv_shape = spmv_shape;
     return L;
   }
 };
@@ -326,6 +328,60 @@ int time_chunk(mppd_batch *B, int g, double *ms_per_it) {
   return MPPD_OK;
 }
 
+// SpMV shape: every candidate (slots in flight x block order, restricted by
+// the parameters) timed on this batch's own matrices with every RVE's CG
+// state live; the fastest is kept.  The shapes differ only in load
+// scheduling and block order, so the results do not depend on the choice.
+int tune_spmv(mppd_batch *B) {
+  const int chunks[3] = {9, 27, 81};
+  std::vector<int> cand;
+  for (int x = 0; x < 2; ++x)
+    for (int c = 0; c < 3; ++c)
+      if ((B->prm.spmv_chunk == 0 || B->prm.spmv_chunk == chunks[c]) &&
+          (B->prm.block_order < 0 || B->prm.block_order == x))
+        cand.push_back(3 * x + c);
+  if (cand.size() == 1) {
+    B->spmv_shape = cand[0];
+    return MPPD_OK;
+  }
+  int rc = sync(B);
+  if (rc) return rc;
+  std::vector<CgState> live(2 * (size_t)B->n);
+  for (auto &c : live) {
+    c = CgState{};
+    c.max_its = 1;
+  }
+  MPP_HIP(hipMemcpy(B->cg, live.data(), sizeof(CgState) * live.size(), hipMemcpyHostToDevice));
+  hipEvent_t e0, e1;
+  MPP_HIP(hipEventCreate(&e0));
+  MPP_HIP(hipEventCreate(&e1));
+  const BLaunch L = B->launch(M_MAIN);
+  float best = 0.f;
+  hipError_t e = hipSuccess;
+  for (int shape : cand) {
+    constexpr int kReps = 10;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_b_spmv(L, shape, 0, B->st);  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, B->st);
+    for (int i = 0; i < kReps && e == hipSuccess; ++i) e = launch_b_spmv(L, shape, 0, B->st);
+    if (e == hipSuccess) e = hipEventRecord(e1, B->st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) break;
+    if (shape == cand[0] || ms < best) {
+      best = ms;
+      B->spmv_shape = shape;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return hip_fail(e, "batch SpMV tuning");
+  // every CG state (both parities) back to stopped
+  for (auto &c : live) c.done = 1;
+  MPP_HIP(hipMemcpyAsync(B->cg, live.data(), sizeof(CgState) * live.size(), hipMemcpyHostToDevice, B->st));
+  return sync(B);
+}
+
 // Graph chunk length: the shortest whose per-iteration launch cost is within
 // 10% of the longest candidate's (short chunks waste fewer iterations past
 // convergence, long ones amortise the graph launch).
@@ -489,7 +545,9 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
   if ((rc = set_mask(B, M_KE, ph))) return rc;
   if ((rc = set_mask(B, M_GATHER, std::vector<int>(n, 0)))) return rc;
   MPP_HIP(launch_b_mat(B->launch(M_MAIN), B->mask_d[M_KE], B->mask_d[M_GATHER], B->st));
-  // the CG graph (its kernels read the table and masks through fixed pointers)
+  // the SpMV shape, then the CG graph (its kernels read the table and masks
+  // through fixed pointers)
+  if ((rc = tune_spmv(B))) return rc;
   if ((rc = set_mask(B, M_MAIN, std::vector<int>(n, 0)))) return rc;
   MPP_HIP(launch_b_cg_init(B->launch(M_MAIN), 0.0, 0.0, 0, B->st));  // every CG done: timing runs exit early
   if (B->prm.graph_chunk > 0)
@@ -525,6 +583,8 @@ void mppd_batch_params_default(mppd_batch_params *p) {
   p->subiterations = 0;
   p->nsubiterations = 10;
   p->graph_chunk = 0;
+  p->spmv_chunk = 0;
+  p->block_order = -1;
 }
 
 int mppd_batch_create(mppd_batch **out, int nrve, const mppd_batch_rve *rves, const mppd_batch_params *p) {
@@ -533,6 +593,9 @@ int mppd_batch_create(mppd_batch **out, int nrve, const mppd_batch_rve *rves, co
   if (nrve < 1 || !rves || !p) return fail(MPPD_ERR_ARG, "bad batch arguments");
   if (p->graph_chunk < 0 || p->graph_chunk > 256 || (p->graph_chunk & 1))
     return fail(MPPD_ERR_ARG, "graph_chunk must be 0 or an even number <= 256");
+  if (p->spmv_chunk != 0 && p->spmv_chunk != 9 && p->spmv_chunk != 27 && p->spmv_chunk != 81)
+    return fail(MPPD_ERR_ARG, "spmv_chunk must be 0, 9, 27 or 81");
+  if (p->block_order < -1 || p->block_order > 1) return fail(MPPD_ERR_ARG, "block_order must be -1, 0 or 1");
   if (p->subiterations && p->nsubiterations < 1) return fail(MPPD_ERR_ARG, "nsubiterations must be >= 1");
   mppd_batch *B = new mppd_batch();
   B->n = nrve;
@@ -744,6 +807,15 @@ int mppd_batch_stats(const mppd_batch *b, double stats[4]) {
   stats[1] = b->gchunk;
   stats[2] = b->graph_launches;
   stats[3] = b->last_ms;
+  return MPPD_OK;
+}
+
+int mppd_batch_tuning(const mppd_batch *b, int out[3]) {
+  if (!b || !out) return fail(MPPD_ERR_ARG, "null argument");
+  const int chunks[3] = {9, 27, 81};
+  out[0] = b->gchunk;
+  out[1] = chunks[b->spmv_shape % 3];
+  out[2] = b->spmv_shape / 3;
   return MPPD_OK;
 }
 

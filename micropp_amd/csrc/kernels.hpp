@@ -387,31 +387,34 @@ __device__ __forceinline__ void vars_new_body(const Grid &g, int lb, const doubl
 }
 
 // ---------------------------------------------------------------------------
-// ell_mvp (src/ell.cpp:35-44) row: the 81 slots in three chunks of 27: the
-// chunk's columns and values first (non-temporal, streamed once), then the
-// 27 x gathers, then the products added in slot order without contraction,
-// so y is bit-identical to the reference SpMV.
+// ell_mvp (src/ell.cpp:35-44) row: the 81 slots in chunks of CH (9, 27 or
+// 81 slots in flight): the chunk's columns and values first (non-temporal,
+// streamed once), then its x gathers, then the products added in slot order
+// without contraction, so y is bit-identical to the reference SpMV whatever
+// the chunk width.
 // ---------------------------------------------------------------------------
 constexpr int kChunk = 27;
 
+template <int CH = kChunk>
 __device__ __forceinline__ double row_dot(const int *cols, const double *vals, const double *x, int row) {
 #pragma clang fp contract(off)
+  static_assert(kSlots % CH == 0, "chunks tile the row");
   const size_t base = (size_t)(row >> 6) * kSlots * 64 + (row & 63);
   const int *c = cols + base;
   const double *v = vals + base;
   double acc = 0;
 #pragma unroll
-  for (int ch = 0; ch < kSlots / kChunk; ++ch) {
-    int ci[kChunk];
-    double vi[kChunk], xv[kChunk];
+  for (int ch = 0; ch < kSlots / CH; ++ch) {
+    int ci[CH];
+    double vi[CH], xv[CH];
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) ci[k] = __builtin_nontemporal_load(&c[(ch * kChunk + k) * 64]);
+    for (int k = 0; k < CH; ++k) ci[k] = __builtin_nontemporal_load(&c[(ch * CH + k) * 64]);
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) vi[k] = __builtin_nontemporal_load(&v[(ch * kChunk + k) * 64]);
+    for (int k = 0; k < CH; ++k) vi[k] = __builtin_nontemporal_load(&v[(ch * CH + k) * 64]);
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) xv[k] = x[ci[k]];
+    for (int k = 0; k < CH; ++k) xv[k] = x[ci[k]];
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) acc += vi[k] * xv[k];
+    for (int k = 0; k < CH; ++k) acc += vi[k] * xv[k];
   }
   return acc;
 }
@@ -480,13 +483,14 @@ __device__ __forceinline__ void cg_init_state_body(const double *part, int npart
   }
 }
 
+template <int CH = kChunk>
 __device__ __forceinline__ void cg_step1_body(int lb, const int *cols, const double *vals, const double *p, double *Ap,
                                               double *part1, const CgState *cur) {
 #pragma clang fp contract(off)
   __shared__ double lds[4];
   if (cur->done) return;
   const int row = lb * 256 + threadIdx.x;
-  const double y = row_dot(cols, vals, p, row);
+  const double y = row_dot<CH>(cols, vals, p, row);
   Ap[row] = y;
   double v[1] = {p[row] * y};
   block_sum<1, 256>(v, lds);

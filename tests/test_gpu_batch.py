@@ -190,6 +190,9 @@ def test_batch_rejects_bad_input(mp):
     with pytest.raises(mp.MppdError):
         mp.Batch([dict(size=5, micro_type=1, geo=(0.2, 0, 0, 0), materials=[mp.material(0, 1e6, 0.3)] * 3)],
                  graph_chunk=3)
+    with pytest.raises(mp.MppdError):
+        mp.Batch([dict(size=5, micro_type=1, geo=(0.2, 0, 0, 0), materials=[mp.material(0, 1e6, 0.3)] * 3)],
+                 spmv_chunk=10)
 
 
 def test_batch_newton_stops_unconverged_at_the_cap(mp):
@@ -214,3 +217,20 @@ def test_batch_newton_stops_unconverged_at_the_cap(mp):
     assert (int(info[0][1]), int(info[0][2]), int(info[0][3])) == (m.get_cost(0), int(m.has_converged(0)),
                                                                    int(m.has_subiterated(0)))
     assert (int(info[0][2]), int(info[0][3])) == (0, 1)  # the case: subiterated, last Newton stopped at the cap
+
+
+@pytest.mark.parametrize("chunk,order", [(9, 0), (81, 1), (27, 0)])
+def test_spmv_shape_does_not_change_results(mp, chunk, order):
+    """The autotuned SpMV shapes (ELL slots per load round, block order) change only load scheduling: a batch
+    with a fixed shape gives bit-identical results to the tuned one."""
+    descs = mixed_batch(mp)[:4]
+    a, _ = run_batch(mp, descs, 2, spmv_chunk=chunk, block_order=order, **KW)
+    rves = [dict(size=d["size"], micro_type=d["micro_type"], geo=d["geo"],
+                 materials=[mp.material(*s) for s in SPECS[d["kind"]]], coupling=d["coupling"]) for d in descs]
+    tuned = mp.Batch(rves, **KW)
+    g, c, x = tuned.tuning()
+    print(f"tuned: graph chunk {g}, SpMV slots {c}, XCD order {x}", flush=True)
+    assert g in (2, 4, 8, 16, 32) and c in (9, 27, 81) and x in (0, 1)
+    b, _ = run_batch(mp, descs, 2, **KW)
+    for (sa, ca, ia), (sb, cb, ib) in zip(a, b):
+        assert np.array_equal(sa, sb) and np.array_equal(ca, cb) and np.array_equal(ia, ib)
