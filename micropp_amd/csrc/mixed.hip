@@ -45,7 +45,7 @@ namespace mpp {
 constexpr int kMaxRhs = 16;
 constexpr int kNodeQuads = 61;  // 244 floats per node (243 coefficients + 1 pad)
 constexpr int kRedThreads = 256;  // threads of the one-block reducers
-constexpr int kVecNodes = 1024;   // nodes per block of the vector kernels
+constexpr int kVecNodes = 2048;   // nodes per block of the vector kernels
 
 // inner CG scalars per field: rz = (r, z), alpha / beta of the iteration, pn
 // the current ||z||, pn0 the start value, tau the relative stop
