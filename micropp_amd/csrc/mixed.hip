@@ -436,82 +436,145 @@ __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const doubl
 typedef __attribute__((address_space(3))) void lds_void;
 typedef const __attribute__((address_space(1))) void glob_void;
 constexpr int kChunkQuads = 9, kChunks = 7, kStages = 3;
+// Buffer resource word 3 for raw 32-bit-element buffers (DATA_FORMAT 32).
+constexpr int kBufWord3 = 0x00020000;
+static_assert(kChunks * kChunkQuads >= kNodeQuads && (kChunks - 1) * kChunkQuads < kNodeQuads, "chunking");
+static_assert(4 * (kChunks - 1) < 27 && 4 * kChunks >= 27, "4 neighbours (36 coefficients) per chunk");
+
+// s_waitcnt immediate for vmcnt <= n (gfx9 encoding: vmcnt in bits [3:0]
+// and [15:14]; expcnt and lgkmcnt at their maxima, i.e. not waited for)
+constexpr int vm_wait(int n) { return (n & 15) | ((n >> 4) << 14) | 0x70 | 0xF00; }
+
+// The matrix part of chunk ch is in LDS (this wave's loads) once at most
+// `after` vector loads younger than them are outstanding; vector-memory
+// loads complete in issue order.
+template <int after>
+__device__ __forceinline__ void wait_loads() {
+  static_assert(after >= 0 && after < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt(vm_wait(after));
+}
+
+// Reads the 9 quads of one stage (this lane's column) with one asm block
+// that ends with its own lgkmcnt(0).  The compiler cannot tell which stage a
+// direct load wrote, so before an LDS read of its own it waits for every
+// outstanding vector load, the P gathers in flight included; here the waits
+// for the direct loads are the explicit wait_loads + barrier.  Volatile with a
+// memory clobber, the block stays between the barrier and the next loads.
+__device__ __forceinline__ void read_stage(const float4 *stage, float a[36]) {
+  static_assert(kChunkQuads == 9, "the asm block reads 9 quads 1024 bytes apart");
+  const unsigned la = (unsigned)(size_t)(lds_void *)stage;
+  f4v m0, m1, m2, m3, m4, m5, m6, m7, m8;
+  asm volatile(
+      "ds_read_b128 %0, %9\n\t"
+      "ds_read_b128 %1, %9 offset:1024\n\t"
+      "ds_read_b128 %2, %9 offset:2048\n\t"
+      "ds_read_b128 %3, %9 offset:3072\n\t"
+      "ds_read_b128 %4, %9 offset:4096\n\t"
+      "ds_read_b128 %5, %9 offset:5120\n\t"
+      "ds_read_b128 %6, %9 offset:6144\n\t"
+      "ds_read_b128 %7, %9 offset:7168\n\t"
+      "ds_read_b128 %8, %9 offset:8192\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(m0), "=&v"(m1), "=&v"(m2), "=&v"(m3), "=&v"(m4), "=&v"(m5), "=&v"(m6), "=&v"(m7), "=&v"(m8)
+      : "v"(la)
+      : "memory");
+  const f4v m[9] = {m0, m1, m2, m3, m4, m5, m6, m7, m8};
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    a[q * 4 + 0] = m[q].x;
+    a[q * 4 + 1] = m[q].y;
+    a[q * 4 + 2] = m[q].z;
+    a[q * 4 + 3] = m[q].w;
+  }
+}
 
 template <int K>
 __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per, const float4 *A, const float4 *P,
                                                 float4 *AP, double *part1, const MState *cur) {
   constexpr int KG = K / 4;
+  constexpr int kDma = (kChunkQuads + KG - 1) / KG;  // matrix loads per wave per chunk
   __shared__ float4 stg[kStages][kChunkQuads][64];
+  static_assert(sizeof(stg[0][0]) == 1024, "one quad of a 64-node slice");
   if (cur->all_done) return;  // whole grid
   const int slice = xcd_block(per);
   if (slice >= nsl) return;   // whole block
   const int grp = threadIdx.x >> 6, lane = threadIdx.x & 63, r0 = grp * 4;
-  // a wave whose four fields have stopped still moves its share of the matrix
-  // and meets every barrier, but gathers and computes nothing
-  const bool live = !(cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]);
   const int n = slice * 64 + lane, nxny = g.nx * g.ny;
   const float4 *ablk = A + (size_t)slice * kNodeQuads * 64;
   const float4 *pg = P + (size_t)grp * NP;
+  // Every wave issues kDma loads per chunk so that the number of loads in
+  // flight is known at compile time.  Quads past the chunk (q >= 9) or past
+  // the node's 61 quads repeat a quad the block already loads into the same
+  // LDS slot (the same bytes) or go to the unused tail of the last stage.
+  // The slice's matrix as a buffer (raw, 16-byte loads straight to LDS,
+  // bounds-checked).  Buffer (MUBUF) rather than global (FLAT) direct loads:
+  // the compiler treats an outstanding FLAT load that writes LDS as possibly
+  // out of order and waits for every vector load before using any gathered p.
+  const __amdgpu_buffer_rsrc_t arsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)ablk, 0, kNodeQuads * 64 * 16, kBufWord3);
   auto issue = [&](int ch) {
 #pragma unroll
-    for (int u = 0; u < (kChunkQuads + KG - 1) / KG; ++u) {
-      const int q = grp + KG * u;
-      if (q < kChunkQuads && ch * kChunkQuads + q < kNodeQuads)
-        __builtin_amdgcn_global_load_lds((glob_void *)(ablk + (size_t)(ch * kChunkQuads + q) * 64 + lane),
-                                         (lds_void *)&stg[ch % kStages][q][0], 16, 0, 0);
+    for (int u = 0; u < kDma; ++u) {
+      const int q = min(grp + KG * u, kChunkQuads - 1);
+      const int gq = min(ch * kChunkQuads + q, kNodeQuads - 1);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(arsrc, (lds_void *)&stg[ch % kStages][q][0], 16,
+                                               (gq * 64 + lane) * 16, 0, 0, 0);
     }
   };
-  float acc[3][4];
-  float4 pc[3];  // p at the node itself (neighbour 13)
+  // A wave whose four fields have stopped still moves its share of the
+  // matrix and meets every barrier, but gathers and computes nothing.
+  if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) {
+    issue(0);
+    issue(1);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    pc[d] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int ch = 0; ch < kChunks; ++ch) {
+      __builtin_amdgcn_s_waitcnt(vm_wait(0));
+      __builtin_amdgcn_s_barrier();
+      if (ch + 2 < kChunks) issue(ch + 2);
+    }
+    return;
+  }
+  float acc[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[d][r] = 0.0f;
-  }
   issue(0);
+  asm volatile("" ::: "memory");  // no gather of chunk 0 ahead of chunk 0's matrix loads
   issue(1);
-#pragma unroll 1
+  // Chunk ch: gather p of its neighbours (12 loads, the last chunk 9), wait
+  // for the matrix part of chunk ch (the loads of chunk ch + 1 and the
+  // gathers issued after it may stay in flight), barrier, read the stage,
+  // start chunk ch + 2 into the stage chunk ch - 1 used (every wave has read
+  // it before this barrier), then the FMAs.  The wait counts hold because no
+  // load moves across the asm blocks: whatever is younger than chunk ch's
+  // matrix loads includes chunk ch + 1's loads and chunk ch's gathers.
+#pragma unroll
   for (int ch = 0; ch < kChunks; ++ch) {
     float4 pv[4][3];
-    if (live) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int nb = ch * 4 + t;
-        const int off = (nb / 9 - 1) * nxny + ((nb / 3) % 3 - 1) * g.nx + (nb % 3 - 1);
-        const int m = min(max(n + off, 0), NP - 1);
-        if (nb < 27)
+    for (int t = 0; t < 4; ++t) {
+      const int nb = ch * 4 + t;
+      const int off = (nb / 9 - 1) * nxny + ((nb / 3) % 3 - 1) * g.nx + (nb % 3 - 1);
+      const int m = min(max(n + off, 0), NP - 1);
+      if (nb < 27)
 #pragma unroll
-          for (int fj = 0; fj < 3; ++fj) pv[t][fj] = pg[(size_t)fj * KG * NP + m];
-      }
-      // Vector-memory loads complete in issue order: once no more than this
-      // chunk's P gathers (12; the last chunk 9) are outstanding, this wave's
-      // copies of chunks ch and ch + 1 have landed in LDS.
-      if (ch < kChunks - 1)
-        __builtin_amdgcn_s_waitcnt(0xF7C);  // vmcnt(12)
-      else
-        __builtin_amdgcn_s_waitcnt(0xF79);  // vmcnt(9)
-    } else {
-      __builtin_amdgcn_s_waitcnt(0);
+        for (int fj = 0; fj < 3; ++fj) pv[t][fj] = pg[(size_t)fj * KG * NP + m];
+    }
+    asm volatile("" ::: "memory");  // every gather of chunk ch is issued before the wait
+    switch (ch) {  // folds: the loop is unrolled
+      case 0: wait_loads<12 + kDma>(); break;
+      case 1: wait_loads<12 + kDma>(); break;
+      case 2: wait_loads<12 + kDma>(); break;
+      case 3: wait_loads<12 + kDma>(); break;
+      case 4: wait_loads<12 + kDma>(); break;
+      case 5: wait_loads<12 + kDma>(); break;
+      default: wait_loads<9>(); break;
     }
     __builtin_amdgcn_s_barrier();  // every wave's share of chunk ch is in LDS
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     float a[36];
-#pragma unroll
-    for (int q = 0; q < kChunkQuads; ++q) {
-      const float4 v = stg[ch % kStages][q][lane];
-      a[q * 4 + 0] = v.x;
-      a[q * 4 + 1] = v.y;
-      a[q * 4 + 2] = v.z;
-      a[q * 4 + 3] = v.w;
-    }
-    // the stage of chunk ch + 2 held chunk ch - 1, which every wave has
-    // consumed before reaching this iteration's barrier
+    read_stage(&stg[ch % kStages][0][lane], a);
     if (ch + 2 < kChunks) issue(ch + 2);
-    if (!live) continue;
-    if (ch == 3)
-#pragma unroll
-      for (int fj = 0; fj < 3; ++fj) pc[fj] = pv[1][fj];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
       if (ch * 4 + t < 27)
@@ -526,15 +589,17 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
             acc[fi][3] += c * pv[t][fj].w;
           }
   }
-  if (!live) return;
+  // p at the node itself is read again (an L2 hit) rather than kept in
+  // registers from chunk 3, which would cost the fourth wave per SIMD
   double pap[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
+    const float4 pc = pg[(size_t)d * KG * NP + n];
     AP[(size_t)(d * KG + grp) * NP + n] = make_float4(acc[d][0], acc[d][1], acc[d][2], acc[d][3]);
-    pap[0] += (double)pc[d].x * acc[d][0];
-    pap[1] += (double)pc[d].y * acc[d][1];
-    pap[2] += (double)pc[d].z * acc[d][2];
-    pap[3] += (double)pc[d].w * acc[d][3];
+    pap[0] += (double)pc.x * acc[d][0];
+    pap[1] += (double)pc.y * acc[d][1];
+    pap[2] += (double)pc.z * acc[d][2];
+    pap[3] += (double)pc.w * acc[d][3];
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
