@@ -80,7 +80,7 @@ __device__ __forceinline__ int xcd_block(int per) {
 // wave adds its lanes with DPP moves, and thread t < NR adds the waves' sums
 // of row t.  Fixed order throughout.  Thread t < NR returns the sum of row t
 // (others return 0).
-template <int NR>
+template <int NR, int UNROLL = 1>
 __device__ __forceinline__ double rows_sum(const double *part, size_t stride, int n) {
   static_assert(NR <= kRedThreads, "one row per thread at the end");
   constexpr int kW = kRedThreads / 64;
@@ -89,6 +89,9 @@ __device__ __forceinline__ double rows_sum(const double *part, size_t stride, in
   double s[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) s[q] = 0.0;
+  // UNROLL > 1 keeps several rounds of loads in flight (long rows); each s[q]
+  // still adds its entries in index order
+#pragma unroll UNROLL
   for (int j = t; j < n; j += kRedThreads)
 #pragma unroll
     for (int q = 0; q < NR; ++q) s[q] += part[(size_t)q * stride + j];
@@ -423,8 +426,8 @@ __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const doubl
 // per slice of 64 nodes; wave g handles field group g (fields 4g .. 4g+3), so
 // a block has K/4 waves.  Partial table part1[field * nsl + slice].
 //
-// The slice's matrix streams HBM -> LDS by direct loads (global_load_lds,
-// 16 bytes per lane, no VGPRs held) in chunks of 9 quads = 4 neighbours,
+// The slice's matrix streams HBM -> LDS by direct buffer loads (16 bytes
+// per lane, no VGPRs held) in chunks of 9 quads = 4 neighbours,
 // two chunks ahead in a 3-stage ring; the block's waves share each chunk
 // (each wave moves a third of it), so the matrix crosses L1 once per block
 // instead of once per wave.  The P gathers of a chunk go to VGPRs.
@@ -434,7 +437,6 @@ __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const doubl
 // nodes are zero (with P finite everywhere), so no in-grid test is needed.
 // ---------------------------------------------------------------------------
 typedef __attribute__((address_space(3))) void lds_void;
-typedef const __attribute__((address_space(1))) void glob_void;
 constexpr int kChunkQuads = 9, kChunks = 7, kStages = 3;
 // Buffer resource word 3 for raw 32-bit-element buffers (DATA_FORMAT 32).
 constexpr int kBufWord3 = 0x00020000;
@@ -627,7 +629,9 @@ __global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const do
   const int c = blockIdx.y, grp = c % KG, d = c / KG, r0 = grp * 4;
   if (cur->all_done) return;
   if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole block
-  const double pap = rows_sum<4>(part1 + (size_t)r0 * nsl, nsl, nsl);
+  constexpr int J = kVecNodes / 256;
+  const int n0 = blockIdx.x * kVecNodes + threadIdx.x;
+  const double pap = rows_sum<4, 8>(part1 + (size_t)r0 * nsl, nsl, nsl);  // one partial per slice
   if (threadIdx.x < 4) {
     const int r = r0 + threadIdx.x;
     const double a = cur->done[r] ? 0.0 : cur->rz[r] / pap;
@@ -638,8 +642,8 @@ __global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const do
   const float4 al = make_float4((float)al_s[0], (float)al_s[1], (float)al_s[2], (float)al_s[3]);
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int jn = 0; jn < kVecNodes / 256; ++jn) {
-    const int n = blockIdx.x * kVecNodes + jn * 256 + threadIdx.x;
+  for (int jn = 0; jn < J; ++jn) {
+    const int n = n0 + jn * 256;
     if (n >= NP) break;
     const size_t o = (size_t)c * NP + n;
     const float kv = kinv[(size_t)d * NP + n];
@@ -714,6 +718,23 @@ __global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kin
     }
     return;
   }
+  // This block's vectors are loaded first (rows clamped; the update skips
+  // n >= NP), so their latency overlaps the re-sums below.  (In k_m_upd the
+  // same order measured slower: its re-sum of one partial per SpMV slice is
+  // long enough that its loads queue behind the vector loads.)
+  constexpr int J = kVecNodes / 256;
+  float kvv[J];
+  float4 rvv[J], pvv[J], xvv[J];
+  const int n0 = blockIdx.x * kVecNodes + threadIdx.x;
+#pragma unroll
+  for (int jn = 0; jn < J; ++jn) {
+    const int n = min(n0 + jn * 256, NP - 1);
+    const size_t o = (size_t)c * NP + n;
+    kvv[jn] = kinv[(size_t)d * NP + n];
+    rvv[jn] = R[o];
+    pvv[jn] = P[o];
+    xvv[jn] = X[o];
+  }
   const size_t row = (size_t)3 * gridDim.x;  // partials per field
   const double *pr = pz + (size_t)K * row;
   const int its = cur->its + 1;
@@ -761,13 +782,13 @@ __global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kin
   const float be[4] = {(float)beta_s[0], (float)beta_s[1], (float)beta_s[2], (float)beta_s[3]};
   const int lv[4] = {live_s[0], live_s[1], live_s[2], live_s[3]};
 #pragma unroll
-  for (int jn = 0; jn < kVecNodes / 256; ++jn) {
-    const int n = blockIdx.x * kVecNodes + jn * 256 + threadIdx.x;
+  for (int jn = 0; jn < J; ++jn) {
+    const int n = n0 + jn * 256;
     if (n >= NP) break;
     const size_t o = (size_t)c * NP + n;
-    const float kv = kinv[(size_t)d * NP + n];
-    const float4 r = R[o];
-    float4 p = P[o], x = X[o];
+    const float kv = kvv[jn];
+    const float4 r = rvv[jn];
+    float4 p = pvv[jn], x = xvv[jn];
     x.x += al.x * p.x;
     x.y += al.y * p.y;
     x.z += al.z * p.z;
