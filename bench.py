@@ -289,6 +289,46 @@ def newton_mixed_workload(nrve, steps):
             "sigma_rel_err_vs_fp64": rel(res[1][0], res[0][0])}
 
 
+def gp_concurrent_workload(nrve, steps):
+    """One micropp<3> instance with mixed-precision Newton (which does not run as a batch): nrve FE_ONE_WAY
+    Gauss points of a 30^3 z-fibre RVE with a J2 matrix (E=1e6, nu=0.3, Sy=1e4, Ka=1e5), each with its own
+    seeded strain ramp (peak components U[-3e-3, 3e-3]).  The same instance type with MPPD_GP_WORKERS=1 (one
+    solver, GPs one after another) and =4 (four solvers on their own streams, GPs concurrently); the results
+    must be equal bit for bit."""
+    import micropp_amd as mp
+
+    nrve = nrve or 16
+    steps = steps or 2
+    mats = [mp.material(1, 1e6, 0.3, 1e5, 1e4), mp.material(0, 1e7, 0.25), mp.material(0, 1e7, 0.25)]
+    kw = dict(lin_stress=0, calc_ctan_lin=0, nr_max_its=4, nr_rel_tol=1e-6, cg_rel_tol=1e-8, cg_max_its=4000,
+              mixed_precision=1)
+    rng = np.random.default_rng(SEED)
+    d = rng.uniform(-3e-3, 3e-3, (nrve, 6))
+    res = {}
+    for workers in (1, 4):
+        os.environ["MPPD_GP_WORKERS"] = str(workers)
+        m = mp.Micropp3(31, mp.MIC_CILI_FIB_Z, (0.2, 0, 0, 0), mats, ngp=nrve, coupling=[mp.FE_ONE_WAY] * nrve,
+                        **kw)
+        sig, t = [], 0.0
+        for k in range(steps):
+            for g in range(nrve):
+                m.set_strain(g, d[g] * (k + 1) / steps)
+            mp.load().mppd_synchronize()
+            t0 = time.perf_counter()
+            m.homogenize()
+            t += time.perf_counter() - t0
+            sig.append([m.get_stress(g) for g in range(nrve)])
+            m.update_vars()
+            print(f"gp_concurrent: workers={workers} step {k} done", file=sys.stderr, flush=True)
+        res[workers] = (np.array(sig), t, sum(m.is_non_linear(g) for g in range(nrve)))
+    os.environ.pop("MPPD_GP_WORKERS")
+    return {"metric": "Gauss-point solves/s, mixed-precision Newton at 30^3 (one vs four concurrent solvers)",
+            "value": nrve * steps / res[4][1], "unit": "GP solves/s (4 solvers)",
+            "workload": f"{nrve} GPs of a 30^3 z-fibre RVE, J2 matrix, {steps} load steps",
+            "one_solver_gp_solves_per_s": nrve * steps / res[1][1], "speedup": res[1][1] / res[4][1],
+            "nonlinear_gps": int(res[4][2]), "bit_equal": bool(np.array_equal(res[1][0], res[4][0]))}
+
+
 def pmc_traffic(summary):
     """HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE runs whose summary (commands, copy-probe
     scaling) is profiles/<summary>; None when the summary is absent."""
@@ -331,7 +371,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of COPY_PROBE_BYTES (scales HBM counter readings)")
     ap.add_argument("--workload", default="mixed50",
-                    choices=["mixed50", "batch_j2", "batch_damage", "adjoint", "newton_mixed"],
+                    choices=["mixed50", "batch_j2", "batch_damage", "adjoint", "newton_mixed", "gp_concurrent"],
                     help="mixed50 (default, the headline); the others measure the batched and adjoint paths")
     ap.add_argument("--rves", type=int, default=0, help="RVEs of a batch / adjoint workload (0: its default)")
     ap.add_argument("--load-steps", type=int, default=0, help="load steps of a batch workload (0: its default)")
@@ -345,7 +385,7 @@ def main():
         return
     if args.workload != "mixed50":
         fn = {"batch_j2": batch_j2, "batch_damage": batch_damage, "adjoint": adjoint_workload,
-              "newton_mixed": newton_mixed_workload}[args.workload]
+              "newton_mixed": newton_mixed_workload, "gp_concurrent": gp_concurrent_workload}[args.workload]
         print(json.dumps(fn(args.rves, args.load_steps)), flush=True)
         return
 

@@ -83,6 +83,13 @@ int mppd_material_eval(const struct material_base *m, int n, const double *eps_d
  * out-of-grid neighbours point at column fj with a zero value. */
 int mppd_solver_create(mppd_solver **out, mppd_rve *r);
 int mppd_solver_destroy(mppd_solver *s);
+/* Device-to-device copy and byte fill ordered on the solver's own stream,
+ * complete on return.  Unlike mppd_memcpy_d2d / mppd_memset they do not wait
+ * for other solvers' work, so solvers driven from other host threads keep
+ * running (no counterpart in the reference, which keeps the GP state in host
+ * memory: memcpy in src/homogenize.cpp:119-187). */
+int mppd_solver_copy(mppd_solver *s, void *dst_d, const void *src_d, size_t bytes);
+int mppd_solver_fill(mppd_solver *s, void *ptr_d, int value, size_t bytes);
 /* The matrix in the reference layout: cols_h/vals_h are [nrow][81]
  * row-major (either may be NULL). */
 int mppd_ell_download(mppd_solver *s, int *cols_h, double *vals_h);

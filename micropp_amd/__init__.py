@@ -116,6 +116,8 @@ SIGNATURES = {
     "mppd_memcpy_d2d": (_I, [_VP, _VP, C.c_size_t]),
     "mppd_memset": (_I, [_VP, _I, C.c_size_t]),
     "mppd_synchronize": (_I, []),
+    "mppd_solver_copy": (_I, [_VP, _VP, _VP, C.c_size_t]),
+    "mppd_solver_fill": (_I, [_VP, _VP, _I, C.c_size_t]),
     "mppd_mem_info": (_I, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "mppd_rve_create": (_I, [C.POINTER(C.c_void_p), _PI, _I, _PD, C.POINTER(MaterialBase)]),
     "mppd_rve_destroy": (_I, [_VP]),

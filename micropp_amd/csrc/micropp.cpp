@@ -7,9 +7,13 @@
 // control flow: the Newton loop, subiterations, the FE_FULL perturbation loop,
 // cost bookkeeping, and the 6x6 products of homogenize_linear.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 #include <stdexcept>
+#include <system_error>
+#include <thread>
 
 #include "micropp.hpp"
 #include "mppd.h"
@@ -26,6 +30,25 @@ double *dzeros(size_t n) {
   check(mppd_memset(p, 0, n * sizeof(double)), "device memset");
   return static_cast<double *>(p);
 }
+
+// zeroed on the solver's own stream (other solvers keep running)
+double *dzeros_on(mppd_solver *s, size_t n) {
+  void *p = nullptr;
+  check(mppd_malloc(&p, n * sizeof(double)), "device allocation");
+  if (mppd_solver_fill(s, p, 0, n * sizeof(double)) != MPPD_OK) {
+    mppd_free(p);
+    check(MPPD_ERR_HIP, "device fill");
+  }
+  return static_cast<double *>(p);
+}
+
+// One FE solve context: a solver (matrix, CG workspace and its own stream)
+// with the working displacement and a scratch history array.
+struct worker {
+  mppd_solver *sol = nullptr;
+  double *u = nullptr;         // working displacement (device)
+  double *vars_aux = nullptr;  // vars_new when the GP has no history yet (device)
+};
 
 struct gp_state {
   double strain_old[6] = {0, 0, 0, 0, 0, 0};
@@ -67,9 +90,17 @@ struct micropp<tdim>::impl {
   mppd_batch *batch = nullptr;  // working slots for those GPs (homogenize)
   int batch_slots = 0;
   bool batch_tried = false;
+  std::vector<worker> wk;  // wk[0] = {sol, u, vars_aux}; more for concurrent GPs
+  bool wk_tried = false;
+  std::mutex err_mu;
 
   ~impl() {
     mppd_batch_destroy(batch);
+    for (size_t k = 1; k < wk.size(); ++k) {
+      mppd_free(wk[k].u);
+      mppd_free(wk[k].vars_aux);
+      mppd_solver_destroy(wk[k].sol);
+    }
     for (auto &q : gp)
       for (double *p : {q.vars_n, q.vars_k, q.u_n, q.u_k}) mppd_free(p);
     mppd_free(u);
@@ -79,12 +110,12 @@ struct micropp<tdim>::impl {
     mppd_rve_destroy(rve);
   }
 
-  newton_t newton(const double eps[6], const double *vars_old) {
+  newton_t newton(worker &w, const double eps[6], const double *vars_old) {
     int out[3] = {0, 0, 0};
     if (mixed)
-      check(mppd_newton_mixed(sol, eps, vars_old, u, &np, nullptr, out, nullptr), "newton_raphson (mixed)");
+      check(mppd_newton_mixed(w.sol, eps, vars_old, w.u, &np, nullptr, out, nullptr), "newton_raphson (mixed)");
     else
-      check(mppd_newton(sol, use_A0 ? A0 : nullptr, eps, vars_old, u, &np, out), "newton_raphson");
+      check(mppd_newton(w.sol, use_A0 ? A0 : nullptr, eps, vars_old, w.u, &np, out), "newton_raphson");
     newton_t n;
     n.its = out[0];
     n.solver_its = out[1];
@@ -92,22 +123,23 @@ struct micropp<tdim>::impl {
     return n;
   }
 
-  void ave_stress(const double *ud, const double *vars, double sig[6]) {
-    check(mppd_ave_stress(sol, ud, vars, sig), "calc_ave_stress");
+  void ave_stress(worker &w, const double *ud, const double *vars, double sig[6]) {
+    check(mppd_ave_stress(w.sol, ud, vars, sig), "calc_ave_stress");
   }
 
-  void copy(double *dst, const double *src) {
-    check(mppd_memcpy_d2d(dst, src, sizeof(double) * nndim), "device copy");
+  void copy(worker &w, double *dst, const double *src) {
+    check(mppd_solver_copy(w.sol, dst, src, sizeof(double) * nndim), "device copy");
   }
 
   // calc_ctan_lin_fe_models (src/micropp.cpp:255-284)
   void ctan_lin_fe_models() {
+    worker &w = wk[0];
     for (int i = 0; i < 6; ++i) {
-      check(mppd_memset(u, 0, sizeof(double) * nndim), "device memset");
+      check(mppd_solver_fill(w.sol, w.u, 0, sizeof(double) * nndim), "device fill");
       double eps[6] = {0, 0, 0, 0, 0, 0}, sig[6];
       eps[i] += D_EPS_CTAN_AVE;
-      newton(eps, nullptr);
-      ave_stress(u, nullptr, sig);
+      newton(w, eps, nullptr);
+      ave_stress(w, w.u, nullptr, sig);
       for (int v = 0; v < 6; ++v) ctan_lin_fe[v * 6 + i] = sig[v] / D_EPS_CTAN_AVE;
     }
   }
@@ -120,52 +152,52 @@ struct micropp<tdim>::impl {
   }
 
   // homogenize_fe_one_way / homogenize_fe_full (src/homogenize.cpp:111-282)
-  void fe(gp_state &q, bool full) {
-    check(mppd_memset(vars_aux, 0, sizeof(double) * nvars), "device memset");
-    double *vars_new = q.allocated ? q.vars_k : vars_aux;
+  void fe(worker &w, gp_state &q, bool full) {
+    check(mppd_solver_fill(w.sol, w.vars_aux, 0, sizeof(double) * nvars), "device fill");
+    double *vars_new = q.allocated ? q.vars_k : w.vars_aux;
     q.cost = 0;
     q.subiterated = false;
-    copy(u, q.u_n);
-    newton_t n = newton(q.strain, q.vars_n);
-    copy(q.u_k, u);
+    copy(w, w.u, q.u_n);
+    newton_t n = newton(w, q.strain, q.vars_n);
+    copy(w, q.u_k, w.u);
     q.cost += n.solver_its;
     q.converged = n.converged;
     if (!q.converged && subiterations) {
       double eps_sub[6], deps[6];
-      copy(u, q.u_n);
+      copy(w, w.u, q.u_n);
       for (int i = 0; i < 6; ++i) eps_sub[i] = q.strain_old[i];
       q.subiterated = true;
       for (int i = 0; i < 6; ++i) deps[i] = (q.strain[i] - q.strain_old[i]) / nsubiterations;
       for (int s = 0; s < nsubiterations; ++s) {
         for (int j = 0; j < 6; ++j) eps_sub[j] += deps[j];
-        n = newton(eps_sub, q.vars_n);
+        n = newton(w, eps_sub, q.vars_n);
         q.cost += n.solver_its;
       }
       q.converged = n.converged;
-      copy(q.u_k, u);
+      copy(w, q.u_k, w.u);
     }
     if (lin_stress)
       linear(q);
     else
-      ave_stress(q.u_k, q.vars_n, q.stress);
+      ave_stress(w, q.u_k, q.vars_n, q.stress);
     int nl = 0;
-    check(mppd_vars_new(sol, q.u_k, q.vars_n, vars_new, &nl), "calc_vars_new");
+    check(mppd_vars_new(w.sol, q.u_k, q.vars_n, vars_new, &nl), "calc_vars_new");
     if (nl && !q.allocated) {
-      q.vars_n = dzeros(nvars);
-      q.vars_k = dzeros(nvars);
+      q.vars_n = dzeros_on(w.sol, nvars);
+      q.vars_k = dzeros_on(w.sol, nvars);
       q.allocated = true;
-      check(mppd_memcpy_d2d(q.vars_k, vars_new, sizeof(double) * nvars), "device copy");
+      check(mppd_solver_copy(w.sol, q.vars_k, vars_new, sizeof(double) * nvars), "device copy");
     }
     if (full && q.allocated) {
       double eps1[6], sig0[6], sig1[6];
-      copy(u, q.u_k);
+      copy(w, w.u, q.u_k);
       for (int v = 0; v < 6; ++v) sig0[v] = q.stress[v];
       for (int i = 0; i < 6; ++i) {
         for (int v = 0; v < 6; ++v) eps1[v] = q.strain[v];
         eps1[i] += D_EPS_CTAN_AVE;
-        newton(eps1, q.vars_n);
+        newton(w, eps1, q.vars_n);
         q.cost += n.solver_its;  // the reference adds the sigma solve's count again
-        ave_stress(u, q.vars_n, sig1);
+        ave_stress(w, w.u, q.vars_n, sig1);
         for (int v = 0; v < 6; ++v) q.ctan[v * 6 + i] = (sig1[v] - sig0[v]) / D_EPS_CTAN_AVE;
       }
     }
@@ -272,6 +304,68 @@ struct micropp<tdim>::impl {
     }
   }
 
+  // FE GPs that do not run as a batch (use_A0, mixed precision,
+  // MPPD_GP_BATCH=0, or no batch could be made) run concurrently, in place of
+  // the reference's OpenMP loop over GPs (src/homogenize.cpp:72): up to
+  // MPPD_GP_WORKERS (default 4) solvers, each with its own stream, take GPs
+  // in turn from host threads.  A GP's kernels and their order do not depend
+  // on the solver that runs it, so the results are those of one solver bit
+  // for bit.  The extra solvers are made at the first homogenize, while at
+  // least a quarter of the device memory stays free; MPPD_GP_WORKERS=1 keeps
+  // one solver.
+  bool workers_ready() {
+    if (wk.size() > 1) return true;
+    if (wk_tried) return false;
+    wk_tried = true;
+    const char *env = getenv("MPPD_GP_WORKERS");
+    const long want = std::min<long>(env ? strtol(env, nullptr, 10) : 4, (long)fe_ids.size());
+    for (long k = 1; k < want; ++k) {
+      size_t fr = 0, tot = 0;
+      if (mppd_mem_info(&fr, &tot) != MPPD_OK || fr < tot / 4) break;
+      worker w;
+      try {
+        check(mppd_solver_create(&w.sol, rve), "solver set-up");
+        w.u = dzeros_on(w.sol, nndim);
+        w.vars_aux = dzeros_on(w.sol, nvars);
+      } catch (const std::exception &ex) {
+        cerr << ex.what() << "; " << wk.size() << " concurrent GP solver(s)" << endl;
+        mppd_free(w.u);
+        mppd_free(w.vars_aux);
+        mppd_solver_destroy(w.sol);
+        break;
+      }
+      wk.push_back(w);
+    }
+    return wk.size() > 1;
+  }
+
+  void fe_parallel() {
+    const int nfe = (int)fe_ids.size();
+    std::atomic<int> next{0};
+    auto run = [&](worker *w) {
+      for (int i = next++; i < nfe; i = next++) {
+        gp_state &q = gp[fe_ids[i]];
+        try {
+          fe(*w, q, q.coupling == FE_FULL);
+        } catch (const std::exception &ex) {
+          q.converged = false;
+          std::lock_guard<std::mutex> lk(err_mu);
+          cerr << ex.what() << " (gp " << fe_ids[i] << ")" << endl;
+        }
+      }
+    };
+    std::vector<std::thread> th;
+    for (size_t k = 1; k < wk.size(); ++k) {
+      try {
+        th.emplace_back(run, &wk[k]);
+      } catch (const std::system_error &) {
+        break;  // the threads already started and this one share the GPs
+      }
+    }
+    run(&wk[0]);
+    for (auto &t : th) t.join();
+  }
+
   void chamis(double c[36]) const;
   void write_vtu(const gp_state &q, const std::string &name) const;
 };
@@ -351,6 +445,7 @@ micropp<tdim>::micropp(const micropp_params_t &params) : p_(new impl()) {
     s.Vf = vf[1];
     s.u = dzeros(s.nndim);
     s.vars_aux = dzeros(s.nvars);
+    s.wk.push_back(worker{s.sol, s.u, s.vars_aux});
     s.gp.resize(s.ngp);
     int n_fe = 0;
     for (int g = 0; g < s.ngp; ++g) {
@@ -412,9 +507,10 @@ void micropp<tdim>::get_ctan(const int gp_id, double *ctan) const {
 }
 
 // homogenize (src/homogenize.cpp:68-99).  FE GPs run as a batch (see
-// impl::batch_ready) or one after the other, each solve filling the GPU.  A
-// failed device call marks the GP (or the batch group) as not converged and
-// is reported on stderr; the other GPs still run.
+// impl::batch_ready), else concurrently on several solvers (see
+// impl::workers_ready), else one after the other.  A failed device call marks
+// the GP (or the batch group) as not converged and is reported on stderr; the
+// other GPs still run.
 template <int tdim>
 void micropp<tdim>::homogenize() {
   impl &s = *p_;
@@ -424,18 +520,28 @@ void micropp<tdim>::homogenize() {
   } catch (const std::exception &ex) {
     cerr << ex.what() << "; GPs run one at a time" << endl;
   }
-  if (batched) s.fe_batched();
+  bool parallel = false;
+  if (batched) {
+    s.fe_batched();
+  } else if (s.fe_ids.size() > 1) {
+    try {
+      parallel = s.workers_ready();
+    } catch (const std::exception &ex) {
+      cerr << ex.what() << "; GPs run one at a time" << endl;
+    }
+    if (parallel) s.fe_parallel();
+  }
   for (int g = 0; g < s.ngp; ++g) {
     gp_state &q = s.gp[g];
     try {
       if (q.coupling == FE_LINEAR || q.coupling == MIX_RULE_CHAMIS)
         impl::linear(q);
-      else if (batched)
+      else if (batched || parallel)
         continue;
       else if (q.coupling == FE_ONE_WAY)
-        s.fe(q, false);
+        s.fe(s.wk[0], q, false);
       else if (q.coupling == FE_FULL)
-        s.fe(q, true);
+        s.fe(s.wk[0], q, true);
     } catch (const std::exception &ex) {
       q.converged = false;
       cerr << ex.what() << " (gp " << g << ")" << endl;

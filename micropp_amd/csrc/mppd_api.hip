@@ -331,6 +331,22 @@ int mppd_solver_destroy(mppd_solver *s) {
   return MPPD_OK;
 }
 
+int mppd_solver_copy(mppd_solver *s, void *dst_d, const void *src_d, size_t bytes) {
+  if (!s || ((!dst_d || !src_d) && bytes)) return fail(MPPD_ERR_ARG, "null argument");
+  if (!bytes) return MPPD_OK;
+  MPP_HIP(hipMemcpyAsync(dst_d, src_d, bytes, hipMemcpyDeviceToDevice, s->st));
+  MPP_HIP(hipStreamSynchronize(s->st));
+  return MPPD_OK;
+}
+
+int mppd_solver_fill(mppd_solver *s, void *ptr_d, int value, size_t bytes) {
+  if (!s || (!ptr_d && bytes)) return fail(MPPD_ERR_ARG, "null argument");
+  if (!bytes) return MPPD_OK;
+  MPP_HIP(hipMemsetAsync(ptr_d, value, bytes, s->st));
+  MPP_HIP(hipStreamSynchronize(s->st));
+  return MPPD_OK;
+}
+
 int mppd_ell_download(mppd_solver *s, int *cols_h, double *vals_h) {
   const Grid &g = s->r->g;
   const size_t n = (size_t)g.nrow * kSlots;
