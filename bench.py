@@ -17,7 +17,7 @@ Paths
         another (ELL SpMV with explicit columns).
 roofline: the dominant kernel of the mixed path (the FP32 multi-RHS SpMV),
   timed with HIP events on its own stream; algorithmic bytes per node =
-  244 x 4 (coefficients incl. the pad slot) + 2 x 3K x 4 (p in, Ap out).
+  252 x 4 (coefficients incl. the 9 row pads) + 2 x 3K x 4 (p in, Ap out).
 cpu_baseline: the oracle (C restatement of the reference; OpenMP over the
   solves as the reference's homogenize) on this host, one step.
 """
@@ -42,7 +42,7 @@ CG_TOL, CG_MAX = 1e-8, 2000
 D_EPS = 1e-8
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ROW_SPMV64 = 81 * 12 + 16
-BYTES_PER_NODE_SPMV32 = 244 * 4 + 2 * 3 * K * 4
+BYTES_PER_NODE_SPMV32 = 252 * 4 + 2 * 3 * K * 4
 
 
 def workload(seed=SEED):

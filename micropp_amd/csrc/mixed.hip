@@ -15,12 +15,14 @@
 // diverges), with the stopping test on ||z||, the reference CG's metric.
 //
 // FP32 matrix layout (node-blocked SELL-64): the 27 neighbour blocks of 3x3
-// coefficients of a node (slot s = nb*9 + fi*3 + fj, nb as in the reference
-// ELL) are padded to 244 floats and stored as 61 float4 per node, quad-major
-// inside a slice of 64 nodes: quad q of node n sits at float4 index
-// ((n / 64) * 61 + q) * 64 + n % 64.  Column indices are implicit (structured
-// 27-point stencil).  Rows and columns of Dirichlet nodes are dropped (the CG
-// vectors are zero there), which leaves the symmetric interior block.
+// coefficients of a node (nb as in the reference ELL) in 9 stencil rows
+// (dz, dy) of 3 neighbours (dx = -1, 0, 1); a row's 27 floats plus 1 pad are 7
+// float4, so slot (nb, fi, fj) is float (nb / 3) * 28 + (nb % 3) * 9 + fi * 3
+// + fj and a node has 63 float4, quad-major inside a slice of 64 nodes: quad
+// q of node n sits at float4 index ((n / 64) * 63 + q) * 64 + n % 64.  Column
+// indices are implicit (structured 27-point stencil).  Rows and columns of
+// Dirichlet nodes are dropped (the CG vectors are zero there), which leaves
+// the symmetric interior block.
 //
 // FP32 vectors group 4 fields: float4 (d * K/4 + g) * NP + n holds fields
 // 4g .. 4g+3 of DOF direction d at node n (NP = nodes rounded up to 256).  A
@@ -43,7 +45,9 @@
 namespace mpp {
 
 constexpr int kMaxRhs = 16;
-constexpr int kNodeQuads = 61;  // 244 floats per node (243 coefficients + 1 pad)
+constexpr int kRowQuads = 7, kRows = 9;         // stencil rows (dz, dy) of 27 floats + 1 pad
+constexpr int kNodeQuads = kRowQuads * kRows;  // 63 float4 per node
+__host__ __device__ constexpr int mslot(int nb, int fi, int fj) { return (nb / 3) * 28 + (nb % 3) * 9 + fi * 3 + fj; }
 constexpr int kRedThreads = 256;  // threads of the one-block reducers
 constexpr int kVecNodes = 2048;   // nodes per block of the vector kernels
 
@@ -185,7 +189,7 @@ __global__ void __launch_bounds__(256) k_m_asm32(Grid g, int NP, const int *et, 
     const int dz = nb / 9 - 1, dy = (nb / 3) % 3 - 1, dx = nb % 3 - 1;
     if (node_on_boundary(g, i + dx, j + dy, k + dz)) continue;  // column of a Dirichlet DOF: dropped
     for (int fi = 0; fi < 3; ++fi)
-      for (int fj = 0; fj < 3; ++fj) put(nb * 9 + fi * 3 + fj, (float)B.entry(nb, fi, fj));
+      for (int fj = 0; fj < 3; ++fj) put(mslot(nb, fi, fj), (float)B.entry(nb, fi, fj));
   }
 }
 
@@ -426,29 +430,32 @@ __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const doubl
 // per slice of 64 nodes; wave g handles field group g (fields 4g .. 4g+3), so
 // a block has K/4 waves.  Partial table part1[field * nsl + slice].
 //
-// The slice's matrix streams HBM -> LDS by direct buffer loads (16 bytes
-// per lane, no VGPRs held) in chunks of 9 quads = 4 neighbours,
-// two chunks ahead in a 3-stage ring; the block's waves share each chunk
-// (each wave moves a third of it), so the matrix crosses L1 once per block
-// instead of once per wave.  The P gathers of a chunk go to VGPRs.
+// The slice's matrix streams HBM -> LDS by direct buffer loads (16 bytes per
+// lane, no VGPRs held) one stencil row (7 quads, 3 neighbours) at a time,
+// two rows ahead in a 3-stage ring; the block's waves share each row (each
+// wave moves a third of it), so the matrix crosses L1 once per block instead
+// of once per wave.  For each row (dz, dy) a wave gathers p of the dx = 0
+// neighbours only, 64 consecutive nodes; the dx = -1 and dx = +1 values are
+// those of the adjacent lanes, moved by DPP wave shifts, with lanes 0 and 63
+// taking theirs from a second load.  That is a third of the gathers of
+// loading every neighbour.
 //
 // Neighbour addresses are n + offset clamped into [0, NP): every neighbour
 // of an interior node lies in the grid, and the rows of boundary and padding
 // nodes are zero (with P finite everywhere), so no in-grid test is needed.
 // ---------------------------------------------------------------------------
 typedef __attribute__((address_space(3))) void lds_void;
-constexpr int kChunkQuads = 9, kChunks = 7, kStages = 3;
+constexpr int kStages = 3;
 // Buffer resource word 3 for raw 32-bit-element buffers (DATA_FORMAT 32).
 constexpr int kBufWord3 = 0x00020000;
-static_assert(kChunks * kChunkQuads >= kNodeQuads && (kChunks - 1) * kChunkQuads < kNodeQuads, "chunking");
-static_assert(4 * (kChunks - 1) < 27 && 4 * kChunks >= 27, "4 neighbours (36 coefficients) per chunk");
+static_assert(kRowQuads * 4 == 28 && kRows == 9, "stencil rows of 27 floats + 1 pad");
 
 // s_waitcnt immediate for vmcnt <= n (gfx9 encoding: vmcnt in bits [3:0]
 // and [15:14]; expcnt and lgkmcnt at their maxima, i.e. not waited for)
 constexpr int vm_wait(int n) { return (n & 15) | ((n >> 4) << 14) | 0x70 | 0xF00; }
 
-// The matrix part of chunk ch is in LDS (this wave's loads) once at most
-// `after` vector loads younger than them are outstanding; vector-memory
+// The matrix part of stencil row w is in LDS (this wave's loads) once at
+// most `after` vector loads younger than them are outstanding; vector-memory
 // loads complete in issue order.
 template <int after>
 __device__ __forceinline__ void wait_loads() {
@@ -456,33 +463,31 @@ __device__ __forceinline__ void wait_loads() {
   __builtin_amdgcn_s_waitcnt(vm_wait(after));
 }
 
-// Reads the 9 quads of one stage (this lane's column) with one asm block
+// Reads the 7 quads of one stage (this lane's column) with one asm block
 // that ends with its own lgkmcnt(0).  The compiler cannot tell which stage a
 // direct load wrote, so before an LDS read of its own it waits for every
 // outstanding vector load, the P gathers in flight included; here the waits
 // for the direct loads are the explicit wait_loads + barrier.  Volatile with a
 // memory clobber, the block stays between the barrier and the next loads.
-__device__ __forceinline__ void read_stage(const float4 *stage, float a[36]) {
-  static_assert(kChunkQuads == 9, "the asm block reads 9 quads 1024 bytes apart");
+__device__ __forceinline__ void read_stage(const float4 *stage, float a[28]) {
+  static_assert(kRowQuads == 7, "the asm block reads 7 quads 1024 bytes apart");
   const unsigned la = (unsigned)(size_t)(lds_void *)stage;
-  f4v m0, m1, m2, m3, m4, m5, m6, m7, m8;
+  f4v m0, m1, m2, m3, m4, m5, m6;
   asm volatile(
-      "ds_read_b128 %0, %9\n\t"
-      "ds_read_b128 %1, %9 offset:1024\n\t"
-      "ds_read_b128 %2, %9 offset:2048\n\t"
-      "ds_read_b128 %3, %9 offset:3072\n\t"
-      "ds_read_b128 %4, %9 offset:4096\n\t"
-      "ds_read_b128 %5, %9 offset:5120\n\t"
-      "ds_read_b128 %6, %9 offset:6144\n\t"
-      "ds_read_b128 %7, %9 offset:7168\n\t"
-      "ds_read_b128 %8, %9 offset:8192\n\t"
+      "ds_read_b128 %0, %7\n\t"
+      "ds_read_b128 %1, %7 offset:1024\n\t"
+      "ds_read_b128 %2, %7 offset:2048\n\t"
+      "ds_read_b128 %3, %7 offset:3072\n\t"
+      "ds_read_b128 %4, %7 offset:4096\n\t"
+      "ds_read_b128 %5, %7 offset:5120\n\t"
+      "ds_read_b128 %6, %7 offset:6144\n\t"
       "s_waitcnt lgkmcnt(0)"
-      : "=&v"(m0), "=&v"(m1), "=&v"(m2), "=&v"(m3), "=&v"(m4), "=&v"(m5), "=&v"(m6), "=&v"(m7), "=&v"(m8)
+      : "=&v"(m0), "=&v"(m1), "=&v"(m2), "=&v"(m3), "=&v"(m4), "=&v"(m5), "=&v"(m6)
       : "v"(la)
       : "memory");
-  const f4v m[9] = {m0, m1, m2, m3, m4, m5, m6, m7, m8};
+  const f4v m[7] = {m0, m1, m2, m3, m4, m5, m6};
 #pragma unroll
-  for (int q = 0; q < 9; ++q) {
+  for (int q = 0; q < 7; ++q) {
     a[q * 4 + 0] = m[q].x;
     a[q * 4 + 1] = m[q].y;
     a[q * 4 + 2] = m[q].z;
@@ -490,12 +495,27 @@ __device__ __forceinline__ void read_stage(const float4 *stage, float a[36]) {
   }
 }
 
+// DPP wave shifts: lane i takes lane i - 1's value (lane 0 keeps `old`), or
+// lane i + 1's (lane 63 keeps `old`)
+__device__ __forceinline__ float4 from_left(const float4 &v, const float4 &old) {
+  auto f = [](float x, float o) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(o), __float_as_int(x), 0x138, 0xf, 0xf, false));
+  };
+  return make_float4(f(v.x, old.x), f(v.y, old.y), f(v.z, old.z), f(v.w, old.w));
+}
+__device__ __forceinline__ float4 from_right(const float4 &v, const float4 &old) {
+  auto f = [](float x, float o) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(o), __float_as_int(x), 0x130, 0xf, 0xf, false));
+  };
+  return make_float4(f(v.x, old.x), f(v.y, old.y), f(v.z, old.z), f(v.w, old.w));
+}
+
 template <int K>
 __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per, const float4 *A, const float4 *P,
                                                 float4 *AP, double *part1, const MState *cur) {
   constexpr int KG = K / 4;
-  constexpr int kDma = (kChunkQuads + KG - 1) / KG;  // matrix loads per wave per chunk
-  __shared__ float4 stg[kStages][kChunkQuads][64];
+  constexpr int kDma = (kRowQuads + KG - 1) / KG;  // matrix loads per wave per stencil row
+  __shared__ float4 stg[kStages][kRowQuads][64];
   static_assert(sizeof(stg[0][0]) == 1024, "one quad of a 64-node slice");
   if (cur->all_done) return;  // whole grid
   const int slice = xcd_block(per);
@@ -504,23 +524,21 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
   const int n = slice * 64 + lane, nxny = g.nx * g.ny;
   const float4 *ablk = A + (size_t)slice * kNodeQuads * 64;
   const float4 *pg = P + (size_t)grp * NP;
-  // Every wave issues kDma loads per chunk so that the number of loads in
-  // flight is known at compile time.  Quads past the chunk (q >= 9) or past
-  // the node's 61 quads repeat a quad the block already loads into the same
-  // LDS slot (the same bytes) or go to the unused tail of the last stage.
   // The slice's matrix as a buffer (raw, 16-byte loads straight to LDS,
   // bounds-checked).  Buffer (MUBUF) rather than global (FLAT) direct loads:
   // the compiler treats an outstanding FLAT load that writes LDS as possibly
   // out of order and waits for every vector load before using any gathered p.
   const __amdgpu_buffer_rsrc_t arsrc =
       __builtin_amdgcn_make_buffer_rsrc((void *)ablk, 0, kNodeQuads * 64 * 16, kBufWord3);
-  auto issue = [&](int ch) {
+  // Every wave issues kDma loads per row so that the number of loads in
+  // flight is known at compile time; quads past the row (q >= 7) repeat the
+  // row's last quad into the same LDS slot (the same bytes).
+  auto issue = [&](int w) {
 #pragma unroll
     for (int u = 0; u < kDma; ++u) {
-      const int q = min(grp + KG * u, kChunkQuads - 1);
-      const int gq = min(ch * kChunkQuads + q, kNodeQuads - 1);
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(arsrc, (lds_void *)&stg[ch % kStages][q][0], 16,
-                                               (gq * 64 + lane) * 16, 0, 0, 0);
+      const int q = min(grp + KG * u, kRowQuads - 1);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(arsrc, (lds_void *)&stg[w % kStages][q][0], 16,
+                                               ((w * kRowQuads + q) * 64 + lane) * 16, 0, 0, 0);
     }
   };
   // A wave whose four fields have stopped still moves its share of the
@@ -529,10 +547,10 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
     issue(0);
     issue(1);
 #pragma unroll
-    for (int ch = 0; ch < kChunks; ++ch) {
+    for (int w = 0; w < kRows; ++w) {
       __builtin_amdgcn_s_waitcnt(vm_wait(0));
       __builtin_amdgcn_s_barrier();
-      if (ch + 2 < kChunks) issue(ch + 2);
+      if (w + 2 < kRows) issue(w + 2);
     }
     return;
   }
@@ -541,58 +559,64 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
   for (int d = 0; d < 3; ++d)
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[d][r] = 0.0f;
+  // lanes 0 and 63 also load their outer x-neighbour; the other lanes load
+  // their own node again (an L1 hit), so the load needs no branch
+  const int dedge = lane == 0 ? -1 : (lane == 63 ? 1 : 0);
   issue(0);
-  asm volatile("" ::: "memory");  // no gather of chunk 0 ahead of chunk 0's matrix loads
+  asm volatile("" ::: "memory");  // no gather of row 0 ahead of row 0's matrix loads
   issue(1);
-  // Chunk ch: gather p of its neighbours (12 loads, the last chunk 9), wait
-  // for the matrix part of chunk ch (the loads of chunk ch + 1 and the
+  // Row w: gather p at the dx = 0 neighbours and the edge values (6 loads),
+  // wait for the matrix part of row w (the loads of row w + 1 and the
   // gathers issued after it may stay in flight), barrier, read the stage,
-  // start chunk ch + 2 into the stage chunk ch - 1 used (every wave has read
-  // it before this barrier), then the FMAs.  The wait counts hold because no
-  // load moves across the asm blocks: whatever is younger than chunk ch's
-  // matrix loads includes chunk ch + 1's loads and chunk ch's gathers.
+  // start row w + 2 into the stage row w - 1 used (every wave has read it
+  // before this barrier), then the FMAs of neighbours dx = -1, 0, 1 in the
+  // reference order.  The wait counts hold because no load moves across the
+  // asm blocks: whatever is younger than row w's matrix loads includes row
+  // w + 1's loads and row w's gathers.
 #pragma unroll
-  for (int ch = 0; ch < kChunks; ++ch) {
-    float4 pv[4][3];
+  for (int w = 0; w < kRows; ++w) {
+    const int off = (w / 3 - 1) * nxny + (w % 3 - 1) * g.nx;
+    const int m = min(max(n + off, 0), NP - 1), me = min(max(n + off + dedge, 0), NP - 1);
+    float4 pm[3], pe[3];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int nb = ch * 4 + t;
-      const int off = (nb / 9 - 1) * nxny + ((nb / 3) % 3 - 1) * g.nx + (nb % 3 - 1);
-      const int m = min(max(n + off, 0), NP - 1);
-      if (nb < 27)
+    for (int fj = 0; fj < 3; ++fj) pm[fj] = pg[(size_t)fj * KG * NP + m];
 #pragma unroll
-        for (int fj = 0; fj < 3; ++fj) pv[t][fj] = pg[(size_t)fj * KG * NP + m];
+    for (int fj = 0; fj < 3; ++fj) pe[fj] = pg[(size_t)fj * KG * NP + me];
+    asm volatile("" ::: "memory");  // every gather of row w is issued before the wait
+    if (w + 1 < kRows)
+      wait_loads<6 + kDma>();
+    else
+      wait_loads<6>();
+    __builtin_amdgcn_s_barrier();  // every wave's share of row w is in LDS
+    float a[28];
+    read_stage(&stg[w % kStages][0][lane], a);
+    if (w + 2 < kRows) issue(w + 2);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float4 v[3];
+#pragma unroll
+      for (int fj = 0; fj < 3; ++fj) {
+        if (j == 0)
+          v[fj] = from_left(pm[fj], pe[fj]);
+        else if (j == 1)
+          v[fj] = pm[fj];
+        else
+          v[fj] = from_right(pm[fj], pe[fj]);
+      }
+#pragma unroll
+      for (int fj = 0; fj < 3; ++fj)
+#pragma unroll
+        for (int fi = 0; fi < 3; ++fi) {
+          const float c = a[j * 9 + fi * 3 + fj];
+          acc[fi][0] += c * v[fj].x;
+          acc[fi][1] += c * v[fj].y;
+          acc[fi][2] += c * v[fj].z;
+          acc[fi][3] += c * v[fj].w;
+        }
     }
-    asm volatile("" ::: "memory");  // every gather of chunk ch is issued before the wait
-    switch (ch) {  // folds: the loop is unrolled
-      case 0: wait_loads<12 + kDma>(); break;
-      case 1: wait_loads<12 + kDma>(); break;
-      case 2: wait_loads<12 + kDma>(); break;
-      case 3: wait_loads<12 + kDma>(); break;
-      case 4: wait_loads<12 + kDma>(); break;
-      case 5: wait_loads<12 + kDma>(); break;
-      default: wait_loads<9>(); break;
-    }
-    __builtin_amdgcn_s_barrier();  // every wave's share of chunk ch is in LDS
-    float a[36];
-    read_stage(&stg[ch % kStages][0][lane], a);
-    if (ch + 2 < kChunks) issue(ch + 2);
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-      if (ch * 4 + t < 27)
-#pragma unroll
-        for (int fj = 0; fj < 3; ++fj)
-#pragma unroll
-          for (int fi = 0; fi < 3; ++fi) {
-            const float c = a[t * 9 + fi * 3 + fj];
-            acc[fi][0] += c * pv[t][fj].x;
-            acc[fi][1] += c * pv[t][fj].y;
-            acc[fi][2] += c * pv[t][fj].z;
-            acc[fi][3] += c * pv[t][fj].w;
-          }
   }
   // p at the node itself is read again (an L2 hit) rather than kept in
-  // registers from chunk 3, which would cost the fourth wave per SIMD
+  // registers from row 4
   double pap[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -1059,7 +1083,7 @@ __device__ __forceinline__ size_t sell_ix(int row, int slot) {
 }
 
 // FP32 node blocks of the interior block of the ELL matrix (ELL slot nb*3+fj
-// of row 3n+fi is node-block slot nb*9+fi*3+fj; columns of Dirichlet DOFs and
+// of row 3n+fi is node-block slot mslot(nb, fi, fj); columns of Dirichlet DOFs and
 // the rows of boundary and padding nodes are zero), and the Jacobi weights
 // 1 / A_nn(d, d) (FP64 and FP32, zero off the interior)
 __global__ void __launch_bounds__(256) k_m_from_ell(Grid g, int NP, const double *vals, float *A, double *kinv64,
@@ -1079,9 +1103,9 @@ __global__ void __launch_bounds__(256) k_m_from_ell(Grid g, int NP, const double
     const bool keep = interior && !node_on_boundary(g, i + dx, j + dy, k + dz);
     for (int fi = 0; fi < 3; ++fi)
       for (int fj = 0; fj < 3; ++fj)
-        put(nb * 9 + fi * 3 + fj, keep ? (float)vals[sell_ix(3 * n + fi, nb * 3 + fj)] : 0.0f);
+        put(mslot(nb, fi, fj), keep ? (float)vals[sell_ix(3 * n + fi, nb * 3 + fj)] : 0.0f);
   }
-  put(kNodeQuads * 4 - 1, 0.0f);
+  for (int r = 0; r < kRows; ++r) put(r * 28 + 27, 0.0f);  // row pads
   for (int d = 0; d < 3; ++d) {
     const double w = interior ? 1.0 / vals[sell_ix(3 * n + d, 13 * 3 + d)] : 0.0;
     kinv64[(size_t)d * NP + n] = w;
