@@ -293,8 +293,8 @@ __global__ void __launch_bounds__(256) k_m_adj_src(Grid g, int NP, int nadj, con
 
 // ---------------------------------------------------------------------------
 // FP64 residual r = F - (K u) on interior DOFs (elastic: element blocks of the
-// phase table; F = 0 unless adjoint sources are given), rounded into the FP32
-// CG start: R = r, X = 0, P = kinv R.
+// phase table, summed per stencil neighbour; F = 0 unless adjoint sources are
+// given), rounded into the FP32 CG start: R = r, X = 0, P = kinv R.
 // Partials: part[q * gridDim.x + block], q < K: ||kinv r||^2 (FP64) of field
 // q; q >= K: (R, P) of field q - K.
 // ---------------------------------------------------------------------------
@@ -315,37 +315,61 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const 
   if (n < NP) {
     const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
     const bool interior = n < g.nn && !node_on_boundary(g, i, j, k);
+    // (K u) at node n by stencil neighbour: the 3x3 block towards neighbour
+    // m = n + (dx, dy, dz) is the sum of the element blocks Ke(n, m) over the
+    // (1, 2, 4 or 8) elements holding both, and is applied to every field
+    // once, so each neighbour's u is loaded once per field.
+    double acc[3][K];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int q = 0; q < K; ++q) acc[d][q] = 0.0;
+    if (interior) {
+      const double *kbe[8];  // element (a, b, c) = (i - 1 + a, j - 1 + b, k - 1 + c): its phase block
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+            kbe[(c * 2 + b) * 2 + a] = ke_s + et[((k - 1 + c) * g.ney + (j - 1 + b)) * g.nex + (i - 1 + a)] * 576;
+#pragma unroll
+      for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+          for (int dx = -1; dx <= 1; ++dx) {
+            double cf[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) cf[t] = 0.0;
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+              for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                  const int mx = 1 - a + dx, my = 1 - b + dy, mz = 1 - c + dz;  // m in element (a, b, c)
+                  if (mx < 0 || mx > 1 || my < 0 || my > 1 || mz < 0 || mz > 1) continue;
+                  const double *kb = kbe[(c * 2 + b) * 2 + a] + local_node(1 - a, 1 - b, 1 - c) * 3 * 24 +
+                                     local_node(mx, my, mz) * 3;
+#pragma unroll
+                  for (int fi = 0; fi < 3; ++fi)
+#pragma unroll
+                    for (int fj = 0; fj < 3; ++fj) cf[fi * 3 + fj] += kb[fi * 24 + fj];
+                }
+            const int m = n + (dz * g.ny + dy) * g.nx + dx;
+#pragma unroll
+            for (int q = 0; q < K; ++q)
+#pragma unroll
+              for (int fj = 0; fj < 3; ++fj) {
+                const double u = U[(size_t)(fj * K + q) * NP + m];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) acc[d][q] += cf[d * 3 + fj] * u;
+              }
+          }
+    }
 #pragma unroll
     for (int kg = 0; kg < K; kg += 4) {
-      double acc[3][4];
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[d][q] = 0.0;
-      if (interior) {
-        for (int a = 0; a < 2; ++a)
-          for (int b = 0; b < 2; ++b)
-            for (int c = 0; c < 2; ++c) {
-              const int ex = i - 1 + a, ey = j - 1 + b, ez = k - 1 + c;
-              const int e = (ez * g.ney + ey) * g.nex + ex;
-              const double *kb = ke_s + et[e] * 576 + local_node(1 - a, 1 - b, 1 - c) * 3 * 24;
-              for (int m = 0; m < 8; ++m) {
-                const int node = ((ez + kHexPos[m][2]) * g.ny + (ey + kHexPos[m][1])) * g.nx + ex + kHexPos[m][0];
-#pragma unroll
-                for (int fj = 0; fj < 3; ++fj) {
-                  double u[4];
-#pragma unroll
-                  for (int q = 0; q < 4; ++q) u[q] = U[(size_t)(fj * K + kg + q) * NP + node];
-#pragma unroll
-                  for (int d = 0; d < 3; ++d) {
-                    const double cf = kb[d * 24 + m * 3 + fj];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) acc[d][q] += cf * u[q];
-                  }
-                }
-              }
-            }
-      }
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const float kv = kinv[(size_t)d * NP + n];
@@ -353,7 +377,7 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const 
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const size_t idx = (size_t)(d * K + kg + q) * NP + n;
-          const double r64 = (F ? F[idx] : 0.0) - acc[d][q];
+          const double r64 = (F ? F[idx] : 0.0) - acc[d][kg + q];
           rr[q] = (float)r64;
           pp[q] = kv * rr[q];
           const double z64 = (double)kv * r64;
