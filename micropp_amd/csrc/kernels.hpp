@@ -419,6 +419,42 @@ __device__ __forceinline__ double row_dot(const int *cols, const double *vals, c
   return acc;
 }
 
+// row_dot on the structured grid: the column of slot nb*3 + fj of row 3n + fi
+// is 3 m + fj for the stencil neighbour m = n + offset(nb) (ell_init,
+// src/ell-common.cpp:102-130), computed instead of loaded, which leaves 8 of
+// the 12 bytes per slot.  m is clamped into the grid: only the rows of
+// boundary nodes (Dirichlet rows, zero off the diagonal) and padding rows
+// (zero) have stencil slots outside the grid, where the reference's column 0
+// + fj and the clamped column both give a zero product for a finite x, and a
+// sum that starts at +0 is unchanged by adding either signed zero.  So the
+// result equals row_dot's bit for bit.
+template <int CH = kChunk>
+__device__ __forceinline__ double row_dot_grid(const Grid &g, const double *vals, const double *x, int row) {
+#pragma clang fp contract(off)
+  static_assert(kSlots % CH == 0 && CH % 3 == 0, "chunks tile the row in whole neighbours");
+  const size_t base = (size_t)(row >> 6) * kSlots * 64 + (row & 63);
+  const double *v = vals + base;
+  const int n = row / 3, nxny = g.nx * g.ny, hi = g.nn - 1;
+  double acc = 0;
+#pragma unroll
+  for (int ch = 0; ch < kSlots / CH; ++ch) {
+    double vi[CH], xv[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) vi[k] = __builtin_nontemporal_load(&v[(ch * CH + k) * 64]);
+#pragma unroll
+    for (int k = 0; k < CH; k += 3) {
+      const int nb = (ch * CH + k) / 3;
+      const int off = (nb / 9 - 1) * nxny + ((nb / 3) % 3 - 1) * g.nx + (nb % 3 - 1);
+      const int m3 = 3 * min(max(n + off, 0), hi);
+#pragma unroll
+      for (int fj = 0; fj < 3; ++fj) xv[k + fj] = x[m3 + fj];
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) acc += vi[k] * xv[k];
+  }
+  return acc;
+}
+
 // ---------------------------------------------------------------------------
 // ell_solve_cgpd (src/ell.cpp:66-122), the reference recurrence in three
 // kernels per iteration:
@@ -484,13 +520,13 @@ __device__ __forceinline__ void cg_init_state_body(const double *part, int npart
 }
 
 template <int CH = kChunk>
-__device__ __forceinline__ void cg_step1_body(int lb, const int *cols, const double *vals, const double *p, double *Ap,
+__device__ __forceinline__ void cg_step1_body(const Grid &g, int lb, const double *vals, const double *p, double *Ap,
                                               double *part1, const CgState *cur) {
 #pragma clang fp contract(off)
   __shared__ double lds[4];
   if (cur->done) return;
   const int row = lb * 256 + threadIdx.x;
-  const double y = row_dot<CH>(cols, vals, p, row);
+  const double y = row_dot_grid<CH>(g, vals, p, row);
   Ap[row] = y;
   double v[1] = {p[row] * y};
   block_sum<1, 256>(v, lds);
