@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(256) kb_cg_step1(const BRve *rv, const int *ma
   if (b >= nb) return;
   const int rid = map[b];
   const BRve &R = rv[rid];
-  cg_step1_body<CH>(R.g, b - blk0[rid], R.vals, R.p, R.Ap, R.part, R.cg + (parity & 1));
+  cg_step1_body<CH, true>(R.g, b - blk0[rid], nullptr, R.vals, R.p, R.Ap, R.part, R.cg + (parity & 1));  // assembled here
 }
 
 __global__ void __launch_bounds__(256) kb_cg_step2(const BRve *rv, const int *map, const int *blk0, int parity) {

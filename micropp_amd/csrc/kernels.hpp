@@ -422,12 +422,13 @@ __device__ __forceinline__ double row_dot(const int *cols, const double *vals, c
 // row_dot on the structured grid: the column of slot nb*3 + fj of row 3n + fi
 // is 3 m + fj for the stencil neighbour m = n + offset(nb) (ell_init,
 // src/ell-common.cpp:102-130), computed instead of loaded, which leaves 8 of
-// the 12 bytes per slot.  m is clamped into the grid: only the rows of
-// boundary nodes (Dirichlet rows, zero off the diagonal) and padding rows
-// (zero) have stencil slots outside the grid, where the reference's column 0
-// + fj and the clamped column both give a zero product for a finite x, and a
-// sum that starts at +0 is unchanged by adding either signed zero.  So the
-// result equals row_dot's bit for bit.
+// the 12 bytes per slot.  m is clamped into the grid.  For a matrix assembled
+// here only the rows of boundary nodes (Dirichlet rows, zero off the
+// diagonal) and padding rows (zero) have stencil slots outside the grid,
+// where the reference's column 0 + fj and the clamped column both give a zero
+// product for a finite x, and a sum that starts at +0 is unchanged by adding
+// either signed zero.  So the result equals row_dot's bit for bit.  (A matrix
+// uploaded from the host may hold values in those slots: use row_dot.)
 template <int CH = kChunk>
 __device__ __forceinline__ double row_dot_grid(const Grid &g, const double *vals, const double *x, int row) {
 #pragma clang fp contract(off)
@@ -519,14 +520,16 @@ __device__ __forceinline__ void cg_init_state_body(const double *part, int npart
   }
 }
 
-template <int CH = kChunk>
-__device__ __forceinline__ void cg_step1_body(const Grid &g, int lb, const double *vals, const double *p, double *Ap,
-                                              double *part1, const CgState *cur) {
+// STENCIL: the matrix has nonzeros only on in-grid stencil slots (assembled
+// here), so the columns are computed (row_dot_grid); otherwise they are read.
+template <int CH = kChunk, bool STENCIL = true>
+__device__ __forceinline__ void cg_step1_body(const Grid &g, int lb, const int *cols, const double *vals,
+                                              const double *p, double *Ap, double *part1, const CgState *cur) {
 #pragma clang fp contract(off)
   __shared__ double lds[4];
   if (cur->done) return;
   const int row = lb * 256 + threadIdx.x;
-  const double y = row_dot_grid<CH>(g, vals, p, row);
+  const double y = STENCIL ? row_dot_grid<CH>(g, vals, p, row) : row_dot<CH>(cols, vals, p, row);
   Ap[row] = y;
   double v[1] = {p[row] * y};
   block_sum<1, 256>(v, lds);

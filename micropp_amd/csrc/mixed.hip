@@ -1156,7 +1156,7 @@ __global__ void __launch_bounds__(256) k_m_resid_ell(Grid g, int NP, const int *
     const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
     float r32 = 0.0f, p32 = 0.0f;
     if (!node_on_boundary(g, i, j, k)) {
-      const double r64 = b[row] - row_dot(cols, vals, du, row);
+      const double r64 = b[row] - row_dot_grid(g, vals, du, row);  // = row_dot(cols, ...) bit for bit
       r32 = (float)r64;
       p32 = kinv[(size_t)d * NP + n] * r32;
       const double z64 = kinv64[(size_t)d * NP + n] * r64;

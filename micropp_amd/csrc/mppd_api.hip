@@ -64,6 +64,9 @@ struct mppd_solver {
   CgState *cg = nullptr, *cg_h = nullptr;
   RedState *red = nullptr, *red_h = nullptr;
   struct MixedLin *ml = nullptr;  // mppd_newton_mixed buffers, made on first use
+  // vals came from this library's assembly (nonzeros only on in-grid stencil
+  // slots), so the CG SpMV computes the columns; false after a host upload
+  bool vals_stencil = true;
 };
 
 // FP32 inner-solve buffers of mppd_newton_mixed: a 4-field system (field 0
@@ -377,6 +380,7 @@ int mppd_ell_upload_vals(mppd_solver *s, const double *vals_h) {
   if (e == hipSuccess) e = launch_rows_to_sell(g, v, s->vals, s->st);
   if (e == hipSuccess) e = hipStreamSynchronize(s->st);
   dfree(v);
+  s->vals_stencil = false;  // any slot may hold a value: the CG reads the columns
   return e == hipSuccess ? MPPD_OK : hip_fail(e, "ell_upload_vals");
 }
 
@@ -418,6 +422,7 @@ static hipError_t mat_async(mppd_solver *s, const double *u_d, const double *var
     e = launch_elem_ke(r->g, r->bmat, r->mats_d, r->et, u_d, vars_d, 0, s->ke, s->st);
   }
   if (e == hipSuccess) e = launch_gather_vals(r->g, r->et, s->ke, s->phase_mode, s->vals, s->st);
+  if (e == hipSuccess) s->vals_stencil = true;  // every slot written, zero off the in-grid stencil
   return e;
 }
 
@@ -451,7 +456,7 @@ static int cg_run(mppd_solver *s, const mppd_solver *A, const double *b_d, doubl
     const int n = std::min(chunk, max_its - queued);
     if (n <= 0) break;
     for (int i = 0; i < n; ++i)
-      MPP_HIP(launch_cg_iter(g, A->cols, A->vals, s->x, s->rr, s->z, s->p, s->Ap, s->kinv, s->partials, s->cg,
+      MPP_HIP(launch_cg_iter(g, A->vals_stencil ? nullptr : A->cols, A->vals, s->x, s->rr, s->z, s->p, s->Ap, s->kinv, s->partials, s->cg,
                              queued + i, s->st));
     queued += n;
     MPP_HIP(hipMemcpyAsync(s->cg_h, s->cg + (queued & 1), sizeof(CgState), hipMemcpyDeviceToHost, s->st));
@@ -569,11 +574,12 @@ int mppd_cg_profile(mppd_solver *s, const double *b_d, int iters, double *spmv_m
   // its partials, so repeating it leaves the CG state unchanged
   hipError_t e = hipEventRecord(ev[0], s->st);
   for (int i = 0; i < iters && e == hipSuccess; ++i)
-    e = launch_cg_step1(g, s->cols, s->vals, s->p, s->Ap, s->partials, s->cg, 0, s->st);
+    e = launch_cg_step1(g, s->vals_stencil ? nullptr : s->cols, s->vals, s->p, s->Ap, s->partials, s->cg, 0, s->st);
   // (2) whole iterations
   if (e == hipSuccess) e = hipEventRecord(ev[1], s->st);
   for (int i = 0; i < iters && e == hipSuccess; ++i)
-    e = launch_cg_iter(g, s->cols, s->vals, s->x, s->rr, s->z, s->p, s->Ap, s->kinv, s->partials, s->cg, i, s->st);
+    e = launch_cg_iter(g, s->vals_stencil ? nullptr : s->cols, s->vals, s->x, s->rr, s->z, s->p, s->Ap, s->kinv,
+                       s->partials, s->cg, i, s->st);
   if (e == hipSuccess) e = hipEventRecord(ev[2], s->st);
   if (e == hipSuccess) e = hipStreamSynchronize(s->st);
   float ms1 = 0.f, ms2 = 0.f;

@@ -46,11 +46,12 @@ __global__ void __launch_bounds__(256) k_cg_init_state(const double *part, int n
 // Blocks are dispatched round-robin over the 8 XCDs: the 8 * per launched
 // blocks give XCD x the row blocks [x * per, (x + 1) * per), so each XCD's
 // L2 holds a contiguous slab of the gathered vector (2-5% faster SpMV).
-__global__ void __launch_bounds__(256) k_cg_step1(Grid g, int nb, int per, const double *vals, const double *p,
-                                                  double *Ap, double *part1, const CgState *cur) {
+template <bool STENCIL>
+__global__ void __launch_bounds__(256) k_cg_step1(Grid g, int nb, int per, const int *cols, const double *vals,
+                                                  const double *p, double *Ap, double *part1, const CgState *cur) {
   const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if (lb >= nb) return;
-  cg_step1_body(g, lb, vals, p, Ap, part1, cur);
+  cg_step1_body<kChunk, STENCIL>(g, lb, cols, vals, p, Ap, part1, cur);
 }
 
 __global__ void __launch_bounds__(256) k_cg_step2(double *x, double *r, double *z, const double *p, const double *Ap,
@@ -95,9 +96,13 @@ hipError_t launch_cg_init(const Grid &g, const double *vals, const double *b, do
 hipError_t launch_cg_step1(const Grid &g, const int *cols, const double *vals, const double *p, double *Ap,
                            double *part, CgState *state, int parity, hipStream_t st) {
   const int nb = (int)nblk(g.nrow_pad, 256), per = (nb + 7) / 8;
-  (void)cols;  // the stencil columns are computed (row_dot_grid)
-  hipLaunchKernelGGL(k_cg_step1, dim3(8 * per), dim3(256), 0, st, g, nb, per, vals, p, Ap, part,
-                     state + (parity & 1));
+  // cols == nullptr: the matrix was assembled here and the stencil columns are computed
+  if (cols)
+    hipLaunchKernelGGL(k_cg_step1<false>, dim3(8 * per), dim3(256), 0, st, g, nb, per, cols, vals, p, Ap, part,
+                       state + (parity & 1));
+  else
+    hipLaunchKernelGGL(k_cg_step1<true>, dim3(8 * per), dim3(256), 0, st, g, nb, per, cols, vals, p, Ap, part,
+                       state + (parity & 1));
   return hipGetLastError();
 }
 

@@ -191,6 +191,53 @@ def test_cg_parity(mp, oracle, tol):
     assert err > 0 and abs(np.log10(err / eo)) < 1
 
 
+def test_cg_uploaded_matrix_reads_columns(mp, oracle):
+    """The CG SpMV computes the stencil columns for a matrix assembled here (its stencil slots
+    outside the grid are zero).  A host-uploaded matrix may hold values there -- the reference
+    layout points those slots at column fj -- so after mppd_ell_upload_vals the CG reads the
+    columns, and after the next assembly it computes them again."""
+    gm, _ = mats_pair(mp, "elastic")
+    size = (7, 7, 7)
+    r = mp.Rve(size, 1, (0.2,) * 4, gm)
+    s = mp.Solver(r)
+    u = mp.DeviceArray(r.nrow)
+    s.set_displ_bc([5e-3, 0, 0, 0, 0, 0], u)
+    s.assembly_mat(u)
+    b = mp.DeviceArray(r.nrow)
+    s.assembly_rhs(u, None, b)
+    # node 0 (a boundary node: identity rows) gets a nonzero right-hand side, so the CG vectors are
+    # nonzero at columns 0..2, where the reference layout points the slots outside the grid
+    bh = b.numpy()
+    bh[:3] = [1.0, -2.0, 3.0]
+    b = mp.DeviceArray.from_numpy(bh)
+    cols, vals = s.ell()
+    nx, ny, nz = size
+    node = np.arange(r.nrow) // 3
+    i, j, k = node % nx, (node // nx) % ny, node // (nx * ny)
+    nb = np.arange(81) // 3
+    dz, dy, dx = nb // 9 - 1, (nb // 3) % 3 - 1, nb % 3 - 1
+    outside = ((i[:, None] + dx < 0) | (i[:, None] + dx >= nx) | (j[:, None] + dy < 0) | (j[:, None] + dy >= ny)
+               | (k[:, None] + dz < 0) | (k[:, None] + dz >= nz))
+    assert outside.any() and np.all(vals[outside] == 0)
+    v2 = vals.copy()
+    v2[outside] = 0.1  # boundary rows: 0.1 x[fj] (node 0's DOFs) joins the identity row
+    kw = dict(min_err=1e-50, rel_err=1e-30, max_its=15)
+    x0 = mp.DeviceArray(r.nrow)
+    s.cg(b, x0, **kw)
+    s.upload_vals(v2)
+    x2 = mp.DeviceArray(r.nrow)
+    s.cg(b, x2, **kw)
+    xo, _, _ = oracle.ell_cg(r.nrow // 3, 3, 81, 13, cols, v2, bh, 1e-50, 1e-30, 15)
+    e_up, e_asm = rel(x2.numpy(), xo), rel(x0.numpy(), xo)
+    print(f"uploaded matrix: gpu vs oracle {e_up:.1e}; assembled-matrix solve vs the same oracle {e_asm:.1e}",
+          flush=True)
+    assert e_up < 1e-10 and e_asm > 1e-6
+    s.assembly_mat(u)  # assembled again: computed columns, the first solve bit for bit
+    x3 = mp.DeviceArray(r.nrow)
+    s.cg(b, x3, **kw)
+    assert np.array_equal(x3.numpy(), x0.numpy())
+
+
 def test_cg_max_its_and_zero_rhs(mp):
     gm, _ = mats_pair(mp, "elastic")
     r = mp.Rve((6, 6, 6), 1, (0.2,) * 4, gm)
