@@ -64,6 +64,27 @@ def test_mixed_matches_fp64_and_oracle(mp, oracle, size):
     assert rel(ms.u(6), u64) < 1e-6
 
 
+@pytest.mark.parametrize("size", [(3, 3, 3), (4, 5, 7), (9, 6, 13), (17, 5, 8)])
+def test_mixed_odd_shapes(mp, size):
+    """Non-cubic and tiny grids (one interior node at 3^3; x-rows shorter than a 64-node slice, so the
+    DPP neighbour lanes and the clamped addresses cross rows and slices): sigma and ctan against the
+    FP64 micropp<3> path."""
+    rng = np.random.default_rng(sum(size))
+    strains = rng.uniform(-5e-3, 5e-3, (6, 6))
+    r = mp.Rve(size, mp.MIC3D_RANDOM_SPHERES, GEO, elastic(mp))
+    sig, ctan, stats, resid, ok = mp.homogenize_mixed(mp.MSolver(r, 12), strains)
+    assert ok and resid.max() < 1e-8
+    kw = dict(lin_stress=0, cg_rel_tol=1e-8, cg_max_its=3000)
+    m = mp.Micropp3(size, mp.MIC3D_RANDOM_SPHERES, GEO, elastic(mp), ngp=6, coupling=[mp.FE_FULL] * 6, **kw)
+    for g in range(6):
+        m.set_strain(g, strains[g])
+    m.homogenize()
+    sig64 = np.array([m.get_stress(g) for g in range(6)])
+    print(f"{size}: outer {stats[0]} inner {stats[1]}; sigma rel {rel(sig, sig64):.1e} ctan rel "
+          f"{rel(ctan, m.ctan_lin()):.1e}", flush=True)
+    assert rel(sig, sig64) < 1e-6 and rel(ctan, m.ctan_lin()) < 1e-6
+
+
 @pytest.mark.parametrize("K", [4, 8, 16])
 def test_mixed_rhs_counts_agree(mp, K):
     r = mp.Rve(11, mp.MIC_SPHERE, (0.2, 0, 0, 0), elastic(mp))
