@@ -378,8 +378,8 @@ def main():
     ap.add_argument("--load-steps", type=int, default=0, help="load steps of a batch workload (0: its default)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); RVEs are independent, so a "
-              "batch would shard across GPUs with no exchange (see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for {args.gpus}); RVEs are independent, so a batch "
+              "would shard across GPUs by estimated cost, with only sigma and ctan gathered back", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
