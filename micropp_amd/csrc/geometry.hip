@@ -45,22 +45,25 @@ __global__ void k_phase_map(Grid g, int type, Geo4 geo, const double *spheres, i
   if (t < 0) atomicOr(bad, 1);
 }
 
-// calc_volume_fractions (src/micropp.cpp:609-627).  The reference adds evol
-// element by element, so one thread walks the elements in order and the sums
-// round exactly as the reference's.
-__global__ void k_volume_fractions(Grid g, const int *et, double *vf) {
-#pragma clang fp contract(off)
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double vm = 0.0, vfb = 0.0;
-  for (int e = 0; e < g.nelem; ++e) {
+// calc_volume_fractions (src/micropp.cpp:609-627) adds the same evol once per
+// element of each phase, so its sums depend only on the two element counts:
+// these are counted here (integer sums, exact in any order) and the host
+// repeats the reference's additions (mppd_rve_create).  counts must be zero
+// on entry.
+__global__ void __launch_bounds__(256) k_phase_counts(Grid g, const int *et, unsigned long long *counts) {
+  __shared__ unsigned c_s[2];
+  if (threadIdx.x < 2) c_s[threadIdx.x] = 0u;
+  __syncthreads();
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < g.nelem) {
     const int t = et[e];
     if (t == 0)
-      vm += g.evol;
+      atomicAdd(&c_s[0], 1u);
     else if (t >= 1)
-      vfb += g.evol;
+      atomicAdd(&c_s[1], 1u);
   }
-  vf[0] = vm / g.vol_tot;
-  vf[1] = vfb / g.vol_tot;
+  __syncthreads();
+  if (threadIdx.x < 2 && c_s[threadIdx.x]) atomicAdd(&counts[threadIdx.x], (unsigned long long)c_s[threadIdx.x]);
 }
 
 // ell_init, dim 3, nfield 3 (src/ell-common.cpp:98-131) written straight into
@@ -127,8 +130,8 @@ hipError_t launch_phase_map(const Grid &g, int type, const double *geo, const do
   return hipGetLastError();
 }
 
-hipError_t launch_volume_fractions(const Grid &g, const int *et, double *vf, hipStream_t st) {
-  hipLaunchKernelGGL(k_volume_fractions, dim3(1), dim3(64), 0, st, g, et, vf);
+hipError_t launch_phase_counts(const Grid &g, const int *et, unsigned long long *counts, hipStream_t st) {
+  hipLaunchKernelGGL(k_phase_counts, dim3(nblk(g.nelem, 256)), dim3(256), 0, st, g, et, counts);
   return hipGetLastError();
 }
 

@@ -93,7 +93,8 @@ hipError_t launch_bmat(const Grid &g, double *bmat, hipStream_t st);
 hipError_t launch_random_spheres(const double *geo, double *spheres, hipStream_t st);
 hipError_t launch_phase_map(const Grid &g, int type, const double *geo, const double *spheres, int nsph, int *et,
                             int *bad, hipStream_t st);
-hipError_t launch_volume_fractions(const Grid &g, const int *et, double *vf, hipStream_t st);
+// elements of phase 0 and of phases >= 1 (counts[2], zero on entry)
+hipError_t launch_phase_counts(const Grid &g, const int *et, unsigned long long *counts, hipStream_t st);
 hipError_t launch_ell_cols(const Grid &g, int *cols, hipStream_t st);
 hipError_t launch_sell_to_rows(const Grid &g, const int *cols, const double *vals, int *cols_rm, double *vals_rm,
                                hipStream_t st);

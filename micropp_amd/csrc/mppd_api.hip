@@ -187,7 +187,7 @@ int mppd_rve_create(mppd_rve **out, const int size[3], int micro_type, const dou
   for (int i = 0; i < 4; ++i) r->geo[i] = geo[i];
   int rc = MPPD_OK;
   int *bad = nullptr;
-  double *vf_d = nullptr;
+  unsigned long long *cnt_d = nullptr;
   do {
     if ((rc = dalloc(&r->bmat, 8 * 144))) break;
     if ((rc = dalloc(&r->mats_d, 3))) break;
@@ -198,7 +198,7 @@ int mppd_rve_create(mppd_rve **out, const int size[3], int micro_type, const dou
     }
     if ((rc = dalloc(&r->spheres, (size_t)kMaxSpheres * 4))) break;
     if ((rc = dalloc(&bad, 1))) break;
-    if ((rc = dalloc(&vf_d, 2))) break;
+    if ((rc = dalloc(&cnt_d, 2))) break;
     hipError_t e = launch_bmat(g, r->bmat, nullptr);
     if (e == hipSuccess && micro_type == kMicRandomSpheres) {
       int n = (int)geo[1];
@@ -206,22 +206,30 @@ int mppd_rve_create(mppd_rve **out, const int size[3], int micro_type, const dou
       e = launch_random_spheres(geo, r->spheres, nullptr);
     }
     if (e == hipSuccess) e = launch_phase_map(g, micro_type, geo, r->spheres, r->nsph, r->et, bad, nullptr);
-    if (e == hipSuccess) e = launch_volume_fractions(g, r->et, vf_d, nullptr);
+    if (e == hipSuccess) e = launch_phase_counts(g, r->et, cnt_d, nullptr);
     if (e != hipSuccess) {
       rc = hip_fail(e, "rve set-up kernels");
       break;
     }
     int bad_h = 0;
+    unsigned long long cnt_h[2] = {0, 0};
     e = hipMemcpy(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(r->vf, vf_d, 2 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(cnt_h, cnt_d, sizeof(cnt_h), hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
       rc = hip_fail(e, "rve set-up readback");
       break;
     }
+    // calc_volume_fractions (src/micropp.cpp:609-627): evol added once per
+    // element of each phase, in the reference's sequence of additions
+    double vm = 0.0, vfb = 0.0;
+    for (unsigned long long q = 0; q < cnt_h[0]; ++q) vm += g.evol;
+    for (unsigned long long q = 0; q < cnt_h[1]; ++q) vfb += g.evol;
+    r->vf[0] = vm / g.vol_tot;
+    r->vf[1] = vfb / g.vol_tot;
     if (bad_h) rc = fail(MPPD_ERR_ARG, "invalid micro_type");
   } while (0);
   dfree(bad);
-  dfree(vf_d);
+  dfree(cnt_d);
   if (rc) {
     mppd_rve_destroy(r);
     return rc;
