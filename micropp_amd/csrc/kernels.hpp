@@ -121,6 +121,8 @@ __device__ __forceinline__ void elem_ke_body(const Grid &g, int lb, const double
   __shared__ double ue_s[kKeWaves][24];
   __shared__ double C_s[kKeWaves][8 * 36];
   __shared__ double cb_s[kKeWaves][8 * 144];
+  __shared__ double eps_s[kKeWaves][8 * 6];
+  __shared__ double sig_s[kKeWaves][8 * 7 * 6];  // FD tangent: base and 6 perturbed stresses per GP
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   stage_bmat(bmat_g, bm);
   const int item = lb * kKeWaves + w;
@@ -141,11 +143,36 @@ __device__ __forceinline__ void elem_ke_body(const Grid &g, int lb, const double
     ue_s[w][lane] = v;
   }
   __syncthreads();
+  // mat_ctan per GP.  Elastic: the closed form, one lane per GP.  Otherwise
+  // the forward-difference tangent's 7 stress evaluations per GP (base and
+  // 6 perturbed strains) run on 56 lanes at once, then each tangent entry is
+  // (s1 - s0) / D_EPS_CTAN as in mat_ctan: the same values bit for bit.
+  const bool fd = active && mats[ph].type != MATERIAL_ELASTIC;
   if (active && lane < 8) {
     double eps[6];
     strain_at(&bm[lane * 144], ue_s[w], eps);
-    const double *h = (vars && !phase_mode) ? vars + ((size_t)item * 8 + lane) * NVAR : nullptr;
-    mat_ctan(mats[ph], eps, h, &C_s[w][lane * 36]);
+    if (fd) {
+      for (int i = 0; i < 6; ++i) eps_s[w][lane * 6 + i] = eps[i];
+    } else {
+      mat_ctan(mats[ph], eps, nullptr, &C_s[w][lane * 36]);
+    }
+  }
+  __syncthreads();
+  if (fd && lane < 56) {
+    const int gp = lane / 7, c = lane % 7;  // c = 0: base strain; c > 0: component c - 1 perturbed
+    double e1[6];
+    for (int i = 0; i < 6; ++i) e1[i] = eps_s[w][gp * 6 + i];
+    if (c > 0) e1[c - 1] += D_EPS_CTAN;
+    const double *h = (vars && !phase_mode) ? vars + ((size_t)item * 8 + gp) * NVAR : nullptr;
+    mat_stress(mats[ph], e1, h, &sig_s[w][(gp * 7 + c) * 6]);
+  }
+  __syncthreads();
+  if (fd) {
+    for (int q = lane; q < 8 * 36; q += 64) {
+      const int gp = q / 36, r = (q / 6) % 6, c = q % 6;
+      const double *sg = &sig_s[w][gp * 7 * 6];
+      C_s[w][gp * 36 + r * 6 + c] = (sg[(c + 1) * 6 + r] - sg[r]) / D_EPS_CTAN;
+    }
   }
   __syncthreads();
   if (active) {
