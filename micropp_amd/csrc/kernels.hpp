@@ -186,15 +186,28 @@ __device__ __forceinline__ void elem_ke_body(const Grid &g, int lb, const double
     }
   }
   __syncthreads();
+  // Ke = sum_gp B^T (C B wg): lane (bi, bj) owns the 3x3 tile of rows 3 bi ..
+  // 3 bi + 2 and columns 3 bj .. 3 bj + 2 (6 LDS reads per 9 products); each
+  // entry still sums gp, then m, in order
   if (active) {
     double *out = ke + (size_t)item * 576;
-    for (int q = lane; q < 576; q += 64) {
-      const int ii = q / 24, jj = q % 24;
-      double acc = 0.0;
-      for (int gp = 0; gp < 8; ++gp)
-        for (int m = 0; m < 6; ++m) acc += bm[gp * 144 + m * 24 + ii] * cb_s[w][gp * 144 + m * 24 + jj];
-      out[q] = acc;
-    }
+    const int bi = lane >> 3, bj = lane & 7;
+    double acc[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    for (int gp = 0; gp < 8; ++gp)
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        const double *bq = &bm[gp * 144 + m * 24 + 3 * bi];
+        const double *cq = &cb_s[w][gp * 144 + m * 24 + 3 * bj];
+        const double b3[3] = {bq[0], bq[1], bq[2]}, c3[3] = {cq[0], cq[1], cq[2]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[a][c] += b3[a] * c3[c];
+      }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) out[(3 * bi + a) * 24 + 3 * bj + c] = acc[a][c];
   }
 }
 
