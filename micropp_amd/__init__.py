@@ -146,7 +146,7 @@ SIGNATURES = {
     "mppd_msolver_get_u": (_I, [_VP, _I, _PD]),
     "mppd_msolver_adjoint": (_I, [_VP, _I, _PD, C.c_void_p, _PD, _PD, _PD, _PI, _PD]),
     "mppd_msolver_profile": (_I, [_VP, _I, _PD, _PD]),
-    "mppd_newton_mixed": (_I, [_VP, _PD, _VP, _VP, C.POINTER(NewtonParams), _VP, _PI, _PI]),
+    "mppd_newton_mixed": (_I, [_VP, _VP, _PD, _VP, _VP, C.POINTER(NewtonParams), _VP, _PI, _PI]),
     "mppd_batch_params_default": (None, [_VP]),
     "mppd_batch_create": (_I, [C.POINTER(C.c_void_p), _I, _VP, _VP]),
     "mppd_batch_destroy": (_I, [_VP]),
@@ -524,15 +524,15 @@ class Solver:
 
     def newton_mixed(self, eps, u: DeviceArray, vars_old: DeviceArray | None = None, nr_max_its=4, nr_max_tol=1e-10,
                      nr_rel_tol=1e-3, cg_max_its=1000, cg_abs_tol=1e-50, cg_rel_tol=1e-5, inner_tol=1e-5,
-                     max_inner_its=None, max_outer=10):
+                     max_inner_its=None, max_outer=10, A0=None, its_with_A0=1):
         """newton_raphson with FP32-inner / FP64-refined linear solves: ((its, solver_its, converged),
-        (outer steps, iterations, FP64 fallbacks, solves stopped short by max_outer))."""
-        p = NewtonParams(nr_max_its, nr_max_tol, nr_rel_tol, cg_max_its, cg_abs_tol, cg_rel_tol, 1)
+        (outer steps, iterations, FP64 fallbacks, solves stopped short by max_outer)).  A0 as in newton."""
+        p = NewtonParams(nr_max_its, nr_max_tol, nr_rel_tol, cg_max_its, cg_abs_tol, cg_rel_tol, its_with_A0)
         mpar = MixedParams(cg_rel_tol, cg_abs_tol, inner_tol, max_inner_its or cg_max_its, max_outer)
         e = np.ascontiguousarray(eps, dtype=np.float64)
         out = np.zeros(3, dtype=np.int32)
         st = np.zeros(4, dtype=np.int32)
-        _chk(load().mppd_newton_mixed(self.h, _dp(e), _ptr(vars_old), u.ptr, C.byref(p),
+        _chk(load().mppd_newton_mixed(self.h, None if A0 is None else A0.h, _dp(e), _ptr(vars_old), u.ptr, C.byref(p),
                                       C.cast(C.byref(mpar), C.c_void_p), _ip(out), _ip(st)), "mppd_newton_mixed")
         return tuple(int(v) for v in out), tuple(int(v) for v in st)
 

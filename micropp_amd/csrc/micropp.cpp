@@ -113,7 +113,8 @@ struct micropp<tdim>::impl {
   newton_t newton(worker &w, const double eps[6], const double *vars_old) {
     int out[3] = {0, 0, 0};
     if (mixed)
-      check(mppd_newton_mixed(w.sol, eps, vars_old, w.u, &np, nullptr, out, nullptr), "newton_raphson (mixed)");
+      check(mppd_newton_mixed(w.sol, use_A0 ? A0 : nullptr, eps, vars_old, w.u, &np, nullptr, out, nullptr),
+            "newton_raphson (mixed)");
     else
       check(mppd_newton(w.sol, use_A0 ? A0 : nullptr, eps, vars_old, w.u, &np, out), "newton_raphson");
     newton_t n;

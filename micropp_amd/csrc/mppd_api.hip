@@ -489,9 +489,19 @@ static int cg_run(mppd_solver *s, const mppd_solver *A, const double *b_d, doubl
   return MPPD_OK;
 }
 
+// A0's matrix is read with s's row layout: it must be a solver of an RVE grid
+// of the same size (in micropp<3>, the instance's own RVE).
+static int check_A0(const mppd_solver *s, const mppd_solver *A0) {
+  if (!A0 || A0 == s) return MPPD_OK;
+  const Grid &a = s->r->g, &b = A0->r->g;
+  if (a.nx != b.nx || a.ny != b.ny || a.nz != b.nz) return fail(MPPD_ERR_ARG, "A0 belongs to an RVE of another size");
+  return MPPD_OK;
+}
+
 int mppd_cg(mppd_solver *s, const mppd_solver *A0, const double *b_d, double *x_d, double min_err, double rel_err,
             int max_its, int *its, double *err) {
   if (!s || !b_d || !x_d) return fail(MPPD_ERR_ARG, "null argument");
+  if (int rc = check_A0(s, A0)) return rc;
   const mppd_solver *A = A0 ? A0 : s;
   int rc = cg_run(s, A, b_d, min_err, rel_err, max_its, its, err);
   if (rc) return rc;
@@ -535,6 +545,7 @@ int mppd_vars_new(mppd_solver *s, const double *u_d, const double *vars_old_d, d
 int mppd_newton(mppd_solver *s, const mppd_solver *A0, const double eps[6], const double *vars_old_d, double *u_d,
                 const mppd_newton_params *p, int out[3]) {
   if (!s || !eps || !u_d || !p || !out) return fail(MPPD_ERR_ARG, "null argument");
+  if (int rc = check_A0(s, A0)) return rc;
   const Grid &g = s->r->g;
   int its = 0, solver_its = 0, conv = 0;
   MPP_HIP(launch_set_displ_bc(g, eps, u_d, s->st));
@@ -868,9 +879,15 @@ int mppd_msolver_get_u(mppd_msolver *m, int k, double *u_h) {
 // Should the FP32 inner CG break down (a non-finite ||z||: the tangent of a
 // softening law can be indefinite), that linear solve is redone by the FP64
 // Jacobi-PCG of mppd_newton, so the result is never worse than the FP64 path.
-int mppd_newton_mixed(mppd_solver *s, const double eps[6], const double *vars_old_d, double *u_d,
-                      const mppd_newton_params *p, const mppd_mixed_params *mp, int out[3], int stats[4]) {
+// With A0, the first its_with_A0 Newton steps solve with A0's matrix instead
+// of assembling (mppd_newton's rule); its FP32 copy is made once per call.
+int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6], const double *vars_old_d,
+                      double *u_d, const mppd_newton_params *p, const mppd_mixed_params *mp, int out[3],
+                      int stats[4]) {
   if (!s || !eps || !u_d || !p || !out) return fail(MPPD_ERR_ARG, "null argument");
+  if (int rc = check_A0(s, A0)) return rc;
+  // the FP32 copy and the FP64 defect residual take the columns from the grid
+  if (A0 && !A0->vals_stencil) return fail(MPPD_ERR_ARG, "A0 holds uploaded values: the mixed solve needs an assembled A0");
   const Grid &g = s->r->g;
   constexpr int K = MixedLin::K;
   const double inner_tol = mp ? mp->inner_tol : 1e-5;
@@ -896,6 +913,7 @@ int mppd_newton_mixed(mppd_solver *s, const double eps[6], const double *vars_ol
   MixedLin *ml = s->ml;
   int its = 0, solver_its = 0, conv = 0, outers = 0, fallbacks = 0, short_solves = 0;
   double pn[K], pn0[K];
+  const mppd_solver *copied = nullptr;  // whose matrix ml->A holds
   MPP_HIP(launch_set_displ_bc(g, eps, u_d, s->st));
   MPP_HIP(rhs_async(s, u_d, vars_old_d, s->b));
   MPP_HIP(read_red(s));
@@ -906,12 +924,20 @@ int mppd_newton_mixed(mppd_solver *s, const double eps[6], const double *vars_ol
       conv = 1;
       break;
     }
-    MPP_HIP(mat_async(s, u_d, vars_old_d));
-    MPP_HIP(launch_m_from_ell(g, s->vals, ml->A, ml->kinv64, ml->kinv, s->st));
+    const mppd_solver *A = s;
+    if (!A0 || its > p->its_with_A0 - 1) {
+      MPP_HIP(mat_async(s, u_d, vars_old_d));
+      MPP_HIP(launch_m_from_ell(g, s->vals, ml->A, ml->kinv64, ml->kinv, s->st));
+      copied = nullptr;  // s's matrix changes with every assembly
+    } else {
+      A = A0;
+      if (copied != A0) MPP_HIP(launch_m_from_ell(g, A0->vals, ml->A, ml->kinv64, ml->kinv, s->st));
+      copied = A0;
+    }
     MPP_HIP(hipMemsetAsync(s->du, 0, sizeof(double) * g.nrow_pad, s->st));
     bool broke = false;
     for (int outer = 0;; ++outer) {
-      MPP_HIP(launch_m_resid_ell(g, K, s->cols, s->vals, s->b, s->du, ml->kinv64, ml->kinv, ml->R, ml->X, ml->P,
+      MPP_HIP(launch_m_resid_ell(g, K, A->cols, A->vals, s->b, s->du, ml->kinv64, ml->kinv, ml->R, ml->X, ml->P,
                                  ml->part, outer == 0, p->cg_rel_tol, p->cg_abs_tol, inner_tol, max_inner,
                                  ml->state, s->st));
       MPP_HIP(hipMemcpyAsync(ml->state_h, ml->state, mixed_state_bytes(), hipMemcpyDeviceToHost, s->st));
@@ -943,7 +969,7 @@ int mppd_newton_mixed(mppd_solver *s, const double eps[6], const double *vars_ol
     if (broke) {  // the FP64 Jacobi-PCG of mppd_newton on the same system, from 0
       int cg_its = 0;
       double err = 0;
-      if ((rc = cg_run(s, s, s->b, p->cg_abs_tol, p->cg_rel_tol, p->cg_max_its, &cg_its, &err))) return rc;
+      if ((rc = cg_run(s, A, s->b, p->cg_abs_tol, p->cg_rel_tol, p->cg_max_its, &cg_its, &err))) return rc;
       solver_its += cg_its;
       ++fallbacks;
       MPP_HIP(launch_axpy(g.nrow, 1.0, s->x, u_d, s->st));

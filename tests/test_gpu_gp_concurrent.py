@@ -51,6 +51,7 @@ def step(m, rp, t, monkeypatch, workers, batch=None):
 CASES = {
     "mixed": dict(mixed_precision=1),
     "use_A0": dict(use_A0=1, its_with_A0=1),
+    "mixed_A0": dict(mixed_precision=1, use_A0=1, its_with_A0=1),
     "fp64_no_batch": dict(batch=0),
 }
 
@@ -71,5 +72,5 @@ def test_concurrent_gps_equal_one_solver(mp, monkeypatch, case):
         nl += sum(r[2][0] for r in a)
         sub += sum(r[2][3] for r in a)
     assert nl > 0, "the case must allocate history"
-    if case != "mixed":
+    if not case.startswith("mixed"):
         assert sub > 0, "the case must subiterate"

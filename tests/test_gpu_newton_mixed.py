@@ -50,15 +50,18 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("kind", list(LAWS))
-def test_micropp_mixed_newton(mp, oracle, kind):
+@pytest.mark.parametrize("case", list(LAWS) + ["plastic+A0"])
+def test_micropp_mixed_newton(mp, oracle, case):
+    """case "plastic+A0": Newton on the initial matrix (use_A0, its_with_A0 = 1) on all three."""
+    kind = case.split("+")[0]
+    kw = dict(KW, use_A0=1, its_with_A0=1) if case.endswith("A0") else KW
     cp = [mp.FE_FULL, mp.FE_ONE_WAY]
     mats = [mp.material(*s) for s in LAWS[kind]]
     size = (9, 10, 11)
-    mx = mp.Micropp3(size, mp.MIC_SPHERE, (0.3, 0, 0, 0), mats, ngp=2, coupling=cp, mixed_precision=1, **KW)
-    fp = mp.Micropp3(size, mp.MIC_SPHERE, (0.3, 0, 0, 0), mats, ngp=2, coupling=cp, **KW)
+    mx = mp.Micropp3(size, mp.MIC_SPHERE, (0.3, 0, 0, 0), mats, ngp=2, coupling=cp, mixed_precision=1, **kw)
+    fp = mp.Micropp3(size, mp.MIC_SPHERE, (0.3, 0, 0, 0), mats, ngp=2, coupling=cp, **kw)
     p, k = ob.make_params(size, mp.MIC_SPHERE, (0.3, 0, 0, 0), [ob.material(*s) for s in LAWS[kind]], ngp=2,
-                          coupling=cp, **KW)
+                          coupling=cp, **kw)
     orc = oracle.micro(p, k)
     rng = np.random.default_rng(3)
     d = [rng.uniform(-1, 1, 6) * AMP[kind], rng.uniform(-1, 1, 6) * AMP[kind] * 0.7]
@@ -72,7 +75,7 @@ def test_micropp_mixed_newton(mp, oracle, kind):
         for g in range(2):
             s_mx, s_fp, s_or = mx.get_stress(g), fp.get_stress(g), orc.get_stress(g)
             c_mx, c_fp = mx.get_ctan(g), fp.get_ctan(g)
-            print(f"{kind} step {t} gp {g}: nl {mx.is_non_linear(g)} cost mixed {mx.get_cost(g)} fp64 "
+            print(f"{case} step {t} gp {g}: nl {mx.is_non_linear(g)} cost mixed {mx.get_cost(g)} fp64 "
                   f"{fp.get_cost(g)}; sigma mixed-fp64 {rel(s_mx, s_fp):.2e} mixed-oracle {rel(s_mx, s_or):.2e}; "
                   f"ctan mixed-fp64 {rel(c_mx, c_fp):.2e}", flush=True)
             assert rel(s_mx, s_fp) < tol_s and rel(s_mx, s_or) < tol_s
@@ -106,9 +109,70 @@ def test_solver_newton_mixed_matches_fp64(mp):
     assert rel(sig32, sig64) < 1e-6
 
 
+@pytest.mark.parametrize("its_with_A0", [1, 2])
+def test_solver_newton_mixed_with_A0(mp, its_with_A0):
+    """Newton on the initial matrix (use_A0): A0 is the tangent assembled at u = 0, and the first its_with_A0
+    Newton steps of both paths solve with it.  The mixed path takes the FP64 path's Newton iteration count and
+    flag, with sigma within 1e-6 and no FP64 fallback or short solve.  (On this load the elastic first step
+    overshoots far into the plastic range and neither path meets nr_rel_tol = 1e-8 within 12 steps; the bar is
+    that they take the same steps.)"""
+    mats = [mp.material(*s) for s in LAWS["plastic"]]
+    r = mp.Rve(21, mp.MIC3D_RANDOM_SPHERES, (11.0, 6.0, 0.05, 0.2), mats)
+    s, a0 = mp.Solver(r), mp.Solver(r)
+    a0.assembly_mat(None)
+    eps = np.array([4e-3, -1e-3, 5e-4, 2e-3, 0.0, -1e-3])
+    u64, u32 = mp.DeviceArray(r.nrow), mp.DeviceArray(r.nrow)
+    kw = dict(nr_max_its=12, nr_rel_tol=1e-8, cg_rel_tol=1e-12, cg_max_its=4000, A0=a0, its_with_A0=its_with_A0)
+    out64 = s.newton(eps, u64, **kw)
+    out32, stats = s.newton_mixed(eps, u32, **kw)
+    sig64, sig32 = s.ave_stress(u64), s.ave_stress(u32)
+    print(f"its_with_A0 {its_with_A0}: fp64 {out64}, mixed {out32}, stats {stats}; "
+          f"sigma rel {rel(sig32, sig64):.2e}", flush=True)
+    assert out32[0] == out64[0] and out32[2] == out64[2]
+    assert stats[2] == 0 and stats[3] == 0
+    assert rel(sig32, sig64) < 1e-6
+
+
+def test_solver_newton_mixed_solves_with_A0(mp):
+    """One Newton step with its_with_A0 = 1 solves A0 du = b and nothing else: with A0 the tangent of a 3x
+    stiffer RVE of the same grid, the mixed and FP64 steps agree to the solve tolerance and differ from the step
+    on the RVE's own tangent by the stiffness ratio (u = BC + du, du = 1/3 of the own-tangent du; 1e-6 covers the
+    CG tolerance times the condition number and the forward-difference tangent of the J2 phase).  Without A0 in
+    the step, the two would differ by a factor of 3."""
+    mats = [mp.material(*s) for s in LAWS["plastic"]]
+    stiff = [mp.material(0, 3e6, 0.3), mp.material(0, 3e7, 0.25), mp.material(0, 3e7, 0.25)]
+    geo = (mp.MIC3D_RANDOM_SPHERES, (11.0, 6.0, 0.05, 0.2))
+    r, r3 = mp.Rve(13, *geo, mats), mp.Rve(13, *geo, stiff)
+    s, a0 = mp.Solver(r), mp.Solver(r3)
+    a0.assembly_mat(None)
+    eps = np.array([1e-4, -2e-5, 3e-5, 5e-5, 0.0, -2e-5])  # elastic: one step of either tangent is exact for it
+    bc = mp.DeviceArray(r.nrow)
+    s.set_displ_bc(eps, bc)
+    kw = dict(nr_max_its=1, cg_rel_tol=1e-12, cg_max_its=4000)
+    us = [mp.DeviceArray(r.nrow) for _ in range(3)]
+    s.newton(eps, us[0], A0=a0, its_with_A0=1, **kw)
+    s.newton_mixed(eps, us[1], A0=a0, its_with_A0=1, **kw)
+    s.newton_mixed(eps, us[2], **kw)
+    d64, d32, down = (u.numpy() - bc.numpy() for u in us)
+    print(f"|du| own {np.abs(down).max():.3e}, A0 {np.abs(d64).max():.3e}; mixed-fp64 {rel(d32, d64):.2e}",
+          flush=True)
+    assert rel(d32, d64) < 1e-6
+    assert rel(d64, down / 3) < 1e-6  # the stiffer RVE has the same Poisson ratios: its tangent is 3x
+
+
 def test_newton_mixed_rejects_bad_arguments(mp):
     r = mp.Rve(5, mp.MIC_SPHERE, (0.2, 0, 0, 0), [mp.material(0, 1e6, 0.3)] * 3)
     s = mp.Solver(r)
     u = mp.DeviceArray(r.nrow)
     with pytest.raises(mp.MppdError):
         s.newton_mixed(np.zeros(6), u, max_outer=0)
+    other = mp.Solver(mp.Rve(6, mp.MIC_SPHERE, (0.2, 0, 0, 0), [mp.material(0, 1e6, 0.3)] * 3))
+    other.assembly_mat(None)
+    with pytest.raises(mp.MppdError):  # A0 of another grid size
+        s.newton_mixed(np.zeros(6), u, A0=other)
+    with pytest.raises(mp.MppdError):
+        s.newton(np.zeros(6), u, A0=other)
+    up = mp.Solver(r)
+    up.upload_vals(s.ell()[1])
+    with pytest.raises(mp.MppdError):  # uploaded values: the mixed solve needs an assembled A0
+        s.newton_mixed(np.zeros(6), u, A0=up)
