@@ -228,7 +228,7 @@ typedef struct {
   int coupling;        /* MPPD_FE_LINEAR, MPPD_FE_ONE_WAY or MPPD_FE_FULL */
 } mppd_batch_rve;
 typedef struct {
-  mppd_newton_params newton; /* its_with_A0 unused */
+  mppd_newton_params newton; /* its_with_A0: used with use_A0 */
   int lin_stress;            /* sigma = ctan . eps (micropp_params_t::lin_stress) */
   int calc_ctan_lin;         /* initial ctan from the six linear solves */
   int subiterations;         /* subiterate a non-converged Newton */
@@ -236,6 +236,9 @@ typedef struct {
   int graph_chunk;           /* CG iterations per captured graph; 0 = choose by timing */
   int spmv_chunk;            /* ELL slots per load round of the SpMV: 9, 27 or 81; 0 = choose by timing */
   int block_order;           /* SpMV blocks: 0 in order, 1 grouped per XCD; -1 = choose by timing */
+  int use_A0;                /* Newton on the initial matrix (micropp_params_t::use_A0): each distinct
+                                RVE's assembly_mat at u = 0 is made at mppd_batch_create, and the first
+                                newton.its_with_A0 Newton steps of every RVE solve with it */
 } mppd_batch_params;
 /* the micropp_params_t defaults (include/types.hpp:43-83); graph_chunk,
  * spmv_chunk and block_order chosen by timing at mppd_batch_create */

@@ -176,7 +176,7 @@ class BatchParams(C.Structure):
 
     _fields_ = [("newton", NewtonParams), ("lin_stress", C.c_int), ("calc_ctan_lin", C.c_int),
                 ("subiterations", C.c_int), ("nsubiterations", C.c_int), ("graph_chunk", C.c_int),
-                ("spmv_chunk", C.c_int), ("block_order", C.c_int)]
+                ("spmv_chunk", C.c_int), ("block_order", C.c_int), ("use_A0", C.c_int)]
 
 
 class MixedParams(C.Structure):
@@ -621,12 +621,13 @@ class Batch:
 
     def __init__(self, rves, **params):
         """rves: list of dicts with size, micro_type, geo, materials (3), coupling (FE_* id).
-        params: newton fields (nr_max_its, nr_max_tol, nr_rel_tol, cg_max_its, cg_abs_tol, cg_rel_tol),
-        lin_stress, calc_ctan_lin, subiterations, nsubiterations, graph_chunk, spmv_chunk, block_order."""
+        params: newton fields (nr_max_its, nr_max_tol, nr_rel_tol, cg_max_its, cg_abs_tol, cg_rel_tol,
+        its_with_A0), lin_stress, calc_ctan_lin, subiterations, nsubiterations, graph_chunk, spmv_chunk,
+        block_order, use_A0."""
         p = BatchParams()
         load().mppd_batch_params_default(C.byref(p))
         for k, v in params.items():
-            if hasattr(p.newton, k) and k != "its_with_A0":
+            if hasattr(p.newton, k):
                 setattr(p.newton, k, v)
             elif k in dict(BatchParams._fields_):
                 setattr(p, k, int(v))

@@ -133,9 +133,14 @@ __global__ void kb_zero_ints(int n, int *p) {
 }
 
 // ---------------- CG ----------------
-__global__ void __launch_bounds__(256) kb_cg_init(const BRve *rv, const int *map, const int *act, const int *blk0) {
+__device__ __forceinline__ const double *cg_matrix(const BRve &R, const int *a0on, int rid) {
+  return a0on[rid] && R.a0 ? R.a0 : R.vals;
+}
+
+__global__ void __launch_bounds__(256) kb_cg_init(const BRve *rv, const int *map, const int *act, const int *blk0,
+                                                  const int *a0on) {
   MPP_BATCH_PROLOGUE(map)
-  cg_init_body(R.g, blockIdx.x - blk0[rid], R.vals, R.b, R.x, R.r, R.z, R.p, R.kinv, R.part);
+  cg_init_body(R.g, blockIdx.x - blk0[rid], cg_matrix(R, a0on, rid), R.b, R.x, R.r, R.z, R.p, R.kinv, R.part);
 }
 
 // one block per RVE; an inactive RVE's CG starts (and stays) done
@@ -160,13 +165,15 @@ __global__ void __launch_bounds__(256) kb_cg_init_state(const BRve *rv, const in
 // (blocks are dispatched round-robin over the 8 XCDs).  The arithmetic is
 // the same in every shape.
 template <int CH, bool XCD>
-__global__ void __launch_bounds__(256) kb_cg_step1(const BRve *rv, const int *map, const int *blk0, int nb, int per,
-                                                   int parity) {
+__global__ void __launch_bounds__(256) kb_cg_step1(const BRve *rv, const int *map, const int *blk0, const int *a0on,
+                                                   int nb, int per, int parity) {
   const int b = XCD ? (int)((blockIdx.x & 7) * per + (blockIdx.x >> 3)) : (int)blockIdx.x;
   if (b >= nb) return;
   const int rid = map[b];
   const BRve &R = rv[rid];
-  cg_step1_body<CH, true>(R.g, b - blk0[rid], nullptr, R.vals, R.p, R.Ap, R.part, R.cg + (parity & 1));  // assembled here
+  // both matrices are assembled here: the stencil columns follow from the grid
+  cg_step1_body<CH, true>(R.g, b - blk0[rid], nullptr, cg_matrix(R, a0on, rid), R.p, R.Ap, R.part,
+                          R.cg + (parity & 1));
 }
 
 __global__ void __launch_bounds__(256) kb_cg_step2(const BRve *rv, const int *map, const int *blk0, int parity) {
@@ -228,7 +235,7 @@ hipError_t launch_b_vars_new(const BLaunch &L, int *flags, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_b_cg_init(const BLaunch &L, double min_err, double rel_err, int max_its, hipStream_t st) {
-  hipLaunchKernelGGL(kb_cg_init, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0);
+  hipLaunchKernelGGL(kb_cg_init, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0, L.a0on);
   hipLaunchKernelGGL(kb_cg_init_state, dim3(L.nrve), dim3(256), 0, st, L.rv, L.act, min_err, rel_err, max_its);
   return hipGetLastError();
 }
@@ -236,12 +243,12 @@ hipError_t launch_b_spmv(const BLaunch &L, int shape, int parity, hipStream_t st
   const int nb = L.row.nb, per = (nb + 7) / 8;
   const dim3 plain(nb), xcd(8 * per);
   switch (shape) {
-    case 0: hipLaunchKernelGGL((kb_cg_step1<9, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
-    case 1: hipLaunchKernelGGL((kb_cg_step1<27, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
-    case 2: hipLaunchKernelGGL((kb_cg_step1<81, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
-    case 3: hipLaunchKernelGGL((kb_cg_step1<9, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
-    case 4: hipLaunchKernelGGL((kb_cg_step1<27, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
-    case 5: hipLaunchKernelGGL((kb_cg_step1<81, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 0: hipLaunchKernelGGL((kb_cg_step1<9, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, L.a0on, nb, per, parity); break;
+    case 1: hipLaunchKernelGGL((kb_cg_step1<27, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, L.a0on, nb, per, parity); break;
+    case 2: hipLaunchKernelGGL((kb_cg_step1<81, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, L.a0on, nb, per, parity); break;
+    case 3: hipLaunchKernelGGL((kb_cg_step1<9, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, L.a0on, nb, per, parity); break;
+    case 4: hipLaunchKernelGGL((kb_cg_step1<27, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, L.a0on, nb, per, parity); break;
+    case 5: hipLaunchKernelGGL((kb_cg_step1<81, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, L.a0on, nb, per, parity); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

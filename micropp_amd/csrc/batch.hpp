@@ -18,6 +18,7 @@ struct BRve {
   const int *et;               // nelem phase ids
   const int *cols;             // SELL-64 columns (shared by RVEs of one size)
   double *vals;                // SELL-64 values
+  const double *a0;            // the initial matrix (use_A0; shared by equal RVEs), else nullptr
   double *ke;                  // 576 per element, or per phase in phase mode
   double *be;                  // 24 per element
   double *uw, *u_n, *u_k;      // working / converged / current displacements
@@ -42,6 +43,7 @@ struct BLaunch {
   int nrve;
   const BRve *rv;
   const int *act;  // per-RVE activity mask of this launch
+  const int *a0on;  // per RVE: the CG solves with a0 instead of vals
   BShape row, elem, ke;
   int spmv_shape = kSpmvDefault;  // see launch_b_spmv
 };

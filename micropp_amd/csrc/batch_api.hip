@@ -33,7 +33,7 @@ struct BState {
   int vals_ready = 0;     // phase mode: vals assembled once
 };
 
-enum { M_MAIN = 0, M_KE = 1, M_GATHER = 2, M_NUM = 3 };
+enum { M_MAIN = 0, M_KE = 1, M_GATHER = 2, M_A0 = 3, M_NUM = 4 };
 
 }  // namespace
 
@@ -50,11 +50,12 @@ struct mppd_batch {
   int nb[3] = {0, 0, 0};
   // arenas
   double *vec = nullptr, *vals = nullptr, *ke = nullptr, *be = nullptr, *vars = nullptr, *part = nullptr;
+  double *a0 = nullptr;  // initial matrices (use_A0), one per distinct RVE
   CgState *cg = nullptr;
   std::map<std::tuple<int, int, int>, int *> cols;
   // per-call device arrays and pinned staging
-  int *mask_d[M_NUM] = {nullptr, nullptr, nullptr};
-  int *mask_h[M_NUM] = {nullptr, nullptr, nullptr};
+  int *mask_d[M_NUM] = {};
+  int *mask_h[M_NUM] = {};
   int *flags_d = nullptr, *flags_h = nullptr;
   double *eps_d = nullptr, *eps_h = nullptr, *norms_d = nullptr, *norms_h = nullptr, *sig_d = nullptr,
          *sig_h = nullptr;
@@ -71,6 +72,7 @@ struct mppd_batch {
     L.nrve = n;
     L.rv = rv;
     L.act = mask_d[mask];
+    L.a0on = mask_d[M_A0];
     L.row = {nb[0], map_d[0], blk0_d[0]};
     L.elem = {nb[1], map_d[1], blk0_d[1]};
     L.ke = {nb[2], map_d[2], blk0_d[2]};
@@ -205,11 +207,17 @@ int cg_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &its) {
   return MPPD_OK;
 }
 
-// element stiffness + vals of the RVEs in `on` (phase-mode RVEs: once)
-int mat_batch(mppd_batch *B, const std::vector<int> &on) {
-  std::vector<int> ke(B->n, 0), ga(B->n, 0);
+// The tangent of Newton iteration its[r] of the RVEs in `on`: element
+// stiffness + vals (phase-mode RVEs: once), or, with use_A0, the initial
+// matrix for the first its_with_A0 iterations (src/solve.cpp:50-66).
+int mat_batch(mppd_batch *B, const std::vector<int> &on, const std::vector<int> &its) {
+  std::vector<int> ke(B->n, 0), ga(B->n, 0), a0(B->n, 0);
   for (int r = 0; r < B->n; ++r) {
     if (!on[r]) continue;
+    if (B->a0 && its[r] < B->prm.newton.its_with_A0) {
+      a0[r] = 1;
+      continue;
+    }
     const bool phase = B->h[r].phase_mode != 0;
     ke[r] = !phase;
     ga[r] = !phase || !B->s[r].vals_ready;
@@ -217,6 +225,7 @@ int mat_batch(mppd_batch *B, const std::vector<int> &on) {
   }
   int rc = set_mask(B, M_KE, ke);
   if (!rc) rc = set_mask(B, M_GATHER, ga);
+  if (!rc && B->a0) rc = set_mask(B, M_A0, a0);
   if (rc) return rc;
   MPP_HIP(launch_b_mat(B->launch(M_MAIN), B->mask_d[M_KE], B->mask_d[M_GATHER], B->st));
   return MPPD_OK;
@@ -255,7 +264,7 @@ int newton_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &it
       }
     }
     if (!any) break;
-    if ((rc = mat_batch(B, run))) return rc;
+    if ((rc = mat_batch(B, run, its))) return rc;
     if ((rc = cg_batch(B, run, cg_its))) return rc;  // leaves the main mask = run
     for (int r = 0; r < n; ++r) sits[r] += cg_its[r];
     MPP_HIP(launch_b_axpy(L, B->st));
@@ -409,8 +418,8 @@ void destroy(mppd_batch *B) {
     dfree(B->map_d[k]);
     dfree(B->blk0_d[k]);
   }
-  for (void *p : {(void *)B->rv, (void *)B->vec, (void *)B->vals, (void *)B->ke, (void *)B->be, (void *)B->vars,
-                  (void *)B->part, (void *)B->cg, (void *)B->flags_d, (void *)B->eps_d, (void *)B->norms_d,
+  for (void *p : {(void *)B->rv, (void *)B->vec, (void *)B->vals, (void *)B->a0, (void *)B->ke, (void *)B->be,
+                  (void *)B->vars, (void *)B->part, (void *)B->cg, (void *)B->flags_d, (void *)B->eps_d, (void *)B->norms_d,
                   (void *)B->sig_d})
     dfree(p);
   for (auto &kv : B->cols) dfree(kv.second);
@@ -430,6 +439,60 @@ int pinned(T **p, size_t n) {
   MPP_HIP(hipHostMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocDefault));
   std::memset(*p, 0, sizeof(T) * std::max<size_t>(n, 1));
   return MPPD_OK;
+}
+
+// RVE descriptors that make the same RVE (the coupling aside)
+bool same_rve(const mppd_batch_rve &a, const mppd_batch_rve &b) {
+  for (int i = 0; i < 3; ++i)
+    if (a.size[i] != b.size[i]) return false;
+  if (a.micro_type != b.micro_type) return false;
+  for (int i = 0; i < 4; ++i)
+    if (a.geo_params[i] != b.geo_params[i]) return false;
+  for (int i = 0; i < 3; ++i) {
+    const material_base &x = a.materials[i], &y = b.materials[i];
+    if (x.E != y.E || x.nu != y.nu || x.Ka != y.Ka || x.Sy != y.Sy || x.k != y.k || x.mu != y.mu ||
+        x.lambda != y.lambda || x.Xt != y.Xt || x.type != y.type)
+      return false;
+  }
+  return true;
+}
+
+// use_A0: the initial matrix of each distinct RVE (assembly_mat at u = 0
+// without history, src/micropp.cpp:128-143), assembled in the slot of the
+// first RVE that has it and kept; equal RVEs share it.
+int setup_a0(mppd_batch *B, const mppd_batch_rve *in) {
+  const int n = B->n;
+  std::vector<int> owner(n), first;
+  for (int r = 0; r < n; ++r) {
+    owner[r] = r;
+    for (int q : first)
+      if (same_rve(in[q], in[r])) {
+        owner[r] = q;
+        break;
+      }
+    if (owner[r] == r) first.push_back(r);
+  }
+  std::vector<size_t> off(n, 0);
+  size_t tot = 0;
+  for (int q : first) {
+    off[q] = tot;
+    tot += (size_t)B->h[q].g.nrow_pad * kSlots;
+  }
+  int rc = dalloc(&B->a0, tot, false);
+  if (rc) return rc;
+  std::vector<int> ke(n, 0), ga(n, 0);
+  for (int q : first) {
+    ke[q] = !B->h[q].phase_mode;  // phase-mode RVEs: per-phase stiffness already made
+    ga[q] = 1;
+  }
+  if ((rc = set_mask(B, M_KE, ke))) return rc;
+  if ((rc = set_mask(B, M_GATHER, ga))) return rc;
+  MPP_HIP(launch_b_mat(B->launch(M_MAIN), B->mask_d[M_KE], B->mask_d[M_GATHER], B->st));
+  for (int q : first)
+    MPP_HIP(hipMemcpyAsync(B->a0 + off[q], B->h[q].vals, sizeof(double) * B->h[q].g.nrow_pad * kSlots,
+                           hipMemcpyDeviceToDevice, B->st));
+  for (int r = 0; r < n; ++r) B->h[r].a0 = B->a0 + off[owner[r]];
+  return upload_table(B);
 }
 
 int setup(mppd_batch *B, const mppd_batch_rve *in) {
@@ -545,6 +608,7 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
   if ((rc = set_mask(B, M_KE, ph))) return rc;
   if ((rc = set_mask(B, M_GATHER, std::vector<int>(n, 0)))) return rc;
   MPP_HIP(launch_b_mat(B->launch(M_MAIN), B->mask_d[M_KE], B->mask_d[M_GATHER], B->st));
+  if (B->prm.use_A0 && (rc = setup_a0(B, in))) return rc;
   // the SpMV shape, then the CG graph (its kernels read the table and masks
   // through fixed pointers)
   if ((rc = tune_spmv(B))) return rc;
@@ -585,6 +649,7 @@ void mppd_batch_params_default(mppd_batch_params *p) {
   p->graph_chunk = 0;
   p->spmv_chunk = 0;
   p->block_order = -1;
+  p->use_A0 = 0;
 }
 
 int mppd_batch_create(mppd_batch **out, int nrve, const mppd_batch_rve *rves, const mppd_batch_params *p) {

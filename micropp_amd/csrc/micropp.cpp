@@ -211,14 +211,14 @@ struct micropp<tdim>::impl {
   // kernel bodies as the one-GP path, so the results are bit-identical.
   // The batch is made at the first homogenize, sized to half the free device
   // memory; MPPD_GP_BATCH=0 in the environment keeps the one-GP-at-a-time
-  // path, MPPD_GP_BATCH=n caps the slots at n.  With A0 (Newton on the
-  // initial matrix), with mixed precision (the batch solves in FP64) or with
-  // fewer than two FE GPs the one-GP path runs.
+  // path, MPPD_GP_BATCH=n caps the slots at n.  With use_A0 the batch keeps
+  // its own copy of the initial matrix.  With mixed precision (the batch
+  // solves in FP64) or with fewer than two FE GPs the one-GP path runs.
   bool batch_ready() {
     if (batch) return true;
     if (batch_tried) return false;
     batch_tried = true;
-    if (use_A0 || mixed || fe_ids.size() < 2) return false;
+    if (mixed || fe_ids.size() < 2) return false;
     const char *env = getenv("MPPD_GP_BATCH");
     const long cap = env ? strtol(env, nullptr, 10) : -1;
     if (cap == 0) return false;
@@ -243,6 +243,7 @@ struct micropp<tdim>::impl {
     p.newton = np;
     p.lin_stress = lin_stress;
     p.calc_ctan_lin = 0;  // the instance's ctan_lin_fe is loaded into every slot
+    p.use_A0 = use_A0;
     p.subiterations = subiterations;
     p.nsubiterations = nsubiterations;
     if (mppd_batch_create(&batch, (int)n, ds.data(), &p) != MPPD_OK) {
@@ -305,7 +306,7 @@ struct micropp<tdim>::impl {
     }
   }
 
-  // FE GPs that do not run as a batch (use_A0, mixed precision,
+  // FE GPs that do not run as a batch (mixed precision,
   // MPPD_GP_BATCH=0, or no batch could be made) run concurrently, in place of
   // the reference's OpenMP loop over GPs (src/homogenize.cpp:72): up to
   // MPPD_GP_WORKERS (default 4) solvers, each with its own stream, take GPs

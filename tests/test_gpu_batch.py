@@ -95,15 +95,22 @@ def run_single(lib_micro, descs, steps):
     return out
 
 
-def test_batch_equals_single_instances(mp):
+@pytest.mark.parametrize("its_with_A0", [0, 2])
+def test_batch_equals_single_instances(mp, its_with_A0):
+    """its_with_A0 > 0: Newton on the initial matrix (use_A0) in the batch and in every instance; a seventh
+    RVE repeats the first one's descriptor (another coupling), so the two share one initial matrix."""
     descs = mixed_batch(mp)
+    kw = dict(KW)
+    if its_with_A0:
+        descs.append(dict(descs[0], coupling=mp.FE_ONE_WAY))
+        kw.update(use_A0=1, its_with_A0=its_with_A0)
     steps = 3
-    rec, stats = run_batch(mp, descs, steps, **KW)
+    rec, stats = run_batch(mp, descs, steps, **kw)
     print("batch stats (CG its queued, graph chunk, graph launches, ms):", stats, flush=True)
 
     def gpu_micro(d):
         return mp.Micropp3(d["size"], d["micro_type"], d["geo"], [mp.material(*s) for s in SPECS[d["kind"]]],
-                           ngp=1, coupling=[d["coupling"]], **KW)
+                           ngp=1, coupling=[d["coupling"]], **kw)
 
     single = run_single(gpu_micro, descs, steps)
     for t in range(steps):

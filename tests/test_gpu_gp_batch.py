@@ -87,10 +87,12 @@ def same(a, b, where):
 KW = dict(subiterations=1, nsubiterations=3, nr_max_its=3, nr_rel_tol=1e-6, cg_rel_tol=1e-10, cg_max_its=3000)
 
 
-@pytest.mark.parametrize("lin_stress", [0, 1])
-def test_gp_batch_equals_one_gp_path(mp, monkeypatch, lin_stress):
+@pytest.mark.parametrize("lin_stress,its_with_A0", [(0, 0), (1, 0), (0, 1), (0, 2)])
+def test_gp_batch_equals_one_gp_path(mp, monkeypatch, lin_stress, its_with_A0):
+    """its_with_A0 > 0: Newton on the initial matrix (use_A0) in both paths."""
     steps = 3
-    seq, grp, one = (make(mp, lin_stress=lin_stress, **KW) for _ in range(3))
+    a0 = dict(use_A0=1, its_with_A0=its_with_A0) if its_with_A0 else {}
+    seq, grp, one = (make(mp, lin_stress=lin_stress, **KW, **a0) for _ in range(3))
     rp = ramps(seq.ngp, steps)
     nl = sub = 0
     for t in range(steps):

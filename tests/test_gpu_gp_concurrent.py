@@ -1,7 +1,7 @@
 """GPU: micropp<3>::homogenize running FE GPs concurrently on several solvers.
 
-When the FE GPs of an instance cannot run as one batch (Newton on the initial
-matrix, use_A0; mixed-precision solves; MPPD_GP_BATCH=0), the reference's
+When the FE GPs of an instance do not run as one batch (mixed-precision
+solves; MPPD_GP_BATCH=0), the reference's
 OpenMP loop over GPs (src/homogenize.cpp:72) becomes up to MPPD_GP_WORKERS
 solvers, each with its own HIP stream, taking GPs in turn from host threads.
 A GP's kernels and their order do not depend on which solver runs it, so the
@@ -50,7 +50,7 @@ def step(m, rp, t, monkeypatch, workers, batch=None):
 
 CASES = {
     "mixed": dict(mixed_precision=1),
-    "use_A0": dict(use_A0=1, its_with_A0=1),
+    "use_A0": dict(use_A0=1, its_with_A0=1, batch=0),  # an instance with use_A0 otherwise runs as a batch
     "mixed_A0": dict(mixed_precision=1, use_A0=1, its_with_A0=1),
     "fp64_no_batch": dict(batch=0),
 }
