@@ -122,24 +122,42 @@ def rel(a, b):
 # ---------------------------------------------------------------------------
 # secondary workloads (not the headline): batched nonlinear RVEs and adjoint derivatives
 # ---------------------------------------------------------------------------
-def _batch_run(mp, descs, ramps, params, label):
-    """Batched homogenize over the load steps, then the same RVEs one micropp<3> instance after another (a
-    subset, for the per-RVE rate); returns the measurement dict."""
+def _batch_timed(mp, descs, ramps, params, label):
+    """Batched homogenize + update_vars over the load steps: (seconds, info, stats, tuning, sigma, mixed stats)."""
     b = mp.Batch(descs, **params)
-    nrve, steps = len(descs), len(ramps[0])
     mp.load().mppd_synchronize()
     t0 = time.perf_counter()
-    for t in range(steps):
+    for t in range(len(ramps[0])):
         b.set_strains(np.array([r[t] for r in ramps]))
         b.homogenize()
         b.update_vars()
         print(f"{label} batched step {t} done", file=sys.stderr, flush=True)
-    dt_b = time.perf_counter() - t0
-    info = b.info()
-    stats = b.stats()
-    _, spmv_slots, spmv_xcd = b.tuning()
-    sig_b = b.stress()
+    dt = time.perf_counter() - t0
+    out = (dt, b.info(), b.stats(), b.tuning(), b.stress(), b.mixed_stats() if params.get("mixed_precision") else None)
     del b
+    return out
+
+
+def _batch_run(mp, descs, ramps, params, label):
+    """Batched homogenize over the load steps, then the same RVEs one micropp<3> instance after another (a
+    subset, for the per-RVE rate); returns the measurement dict.  With mixed_precision, the FP64 batch of the
+    same RVEs is timed instead of the one-by-one runs, and sigma is compared with it."""
+    nrve, steps = len(descs), len(ramps[0])
+    dt_b, info, stats, (_, spmv_slots, spmv_xcd), sig_b, mstats = _batch_timed(mp, descs, ramps, params, label)
+    common = {"rves": nrve, "load_steps": steps, "batched_s": dt_b,
+              "nonlinear_rves": int(info[:, 0].sum()), "converged_rves": int(info[:, 2].sum()),
+              "graph_chunk": int(stats[1]), "spmv_slots_per_round": spmv_slots, "spmv_blocks_per_xcd": spmv_xcd,
+              "cg_iterations_queued": int(stats[0])}
+    if mstats is not None:
+        dt64, info64, stats64, _, sig64, _ = _batch_timed(mp, descs, ramps, dict(params, mixed_precision=0),
+                                                           label + " fp64")
+        err = np.abs(sig_b - sig64).max(axis=1) / np.abs(sig64).max(axis=1)
+        return dict(common, value=nrve * steps / dt_b, unit="RVE solves/s (sigma+ctan, FE_FULL)",
+                    precision="FP32 inner CG, FP64 defect correction", fp64_batched_rve_solves_per_s=nrve * steps / dt64,
+                    speedup_vs_fp64_batch=dt64 / dt_b, sigma_max_rel_err_vs_fp64=float(err.max()),
+                    same_flags_as_fp64=bool(np.array_equal(info[:, [0, 2]], info64[:, [0, 2]])),
+                    fp64_cg_iterations_queued=int(stats64[0]),
+                    mixed_outer_steps=mstats[0], mixed_fp64_fallbacks=mstats[1], mixed_short_solves=mstats[2])
     nseq = min(nrve, 4)
     kw = {k: v for k, v in params.items() if k != "graph_chunk"}
     t0 = time.perf_counter()
@@ -155,16 +173,12 @@ def _batch_run(mp, descs, ramps, params, label):
         m.close()
         print(f"{label} single RVE {i} done", file=sys.stderr, flush=True)
     dt_s = (time.perf_counter() - t0) / nseq
-    return {"value": nrve * steps / dt_b, "unit": "RVE solves/s (sigma+ctan, FE_FULL)", "rves": nrve,
-            "load_steps": steps, "batched_s": dt_b, "one_by_one_rve_solves_per_s": steps / dt_s,
-            "speedup_vs_one_by_one": (nrve * steps / dt_b) / (steps / dt_s),
-            "batched_equals_one_by_one": bool(np.array_equal(np.array(sig_s), sig_b[:nseq])),
-            "nonlinear_rves": int(info[:, 0].sum()), "converged_rves": int(info[:, 2].sum()),
-            "graph_chunk": int(stats[1]), "spmv_slots_per_round": spmv_slots, "spmv_blocks_per_xcd": spmv_xcd,
-            "cg_iterations_queued": int(stats[0])}
+    return dict(common, value=nrve * steps / dt_b, unit="RVE solves/s (sigma+ctan, FE_FULL)",
+                one_by_one_rve_solves_per_s=steps / dt_s, speedup_vs_one_by_one=(nrve * steps / dt_b) / (steps / dt_s),
+                batched_equals_one_by_one=bool(np.array_equal(np.array(sig_s), sig_b[:nseq])))
 
 
-def batch_j2(nrve, steps):
+def batch_j2(nrve, steps, mixed=False):
     """RVEs of 30^3 hexes, a z-fibre of radius U[0.1, 0.3], J2 matrix (E 1e6, nu 0.3, Sy 1e4, Ka 1e5), elastic
     fibre; cyclic uniaxial + shear strain of amplitude 2e-3."""
     import micropp_amd as mp
@@ -180,14 +194,15 @@ def batch_j2(nrve, steps):
         d[0], d[3] = 2e-3, rng.uniform(-2e-3, 2e-3)
         ramps.append([d * np.sin(2 * np.pi * (t + 1) / 20) / np.sin(2 * np.pi / 20) for t in range(steps)])
     # the workload names no Newton tolerance: the reference defaults (4 iterations, 1e-3)
-    params = dict(lin_stress=0, nr_max_its=4, nr_rel_tol=1e-3, cg_rel_tol=1e-8, cg_max_its=2000)
+    params = dict(lin_stress=0, nr_max_its=4, nr_rel_tol=1e-3, cg_rel_tol=1e-8, cg_max_its=2000,
+                  mixed_precision=int(mixed))
     out = {"metric": "batched J2 RVEs (cylindrical fibre), RVE solves/s", "workload":
            f"{nrve} RVEs of 30^3 hexes, z-fibre radius U[0.1,0.3], J2 matrix, {steps} cyclic load steps"}
     out.update(_batch_run(mp, descs, ramps, params, "batch_j2"))
     return out
 
 
-def batch_damage(nrve, steps):
+def batch_damage(nrve, steps, mixed=False):
     """RVEs of n in {20, 30, 40, 50} hexes (seeded mix), a y-layer of width U[0.1, 0.3], damage matrix (E 1e6,
     nu 0.3, Xt = 1e3 sqrt(E)), elastic fibre; a seeded strain ramp to |eps| = 1e-2 over 10 steps."""
     import micropp_amd as mp
@@ -203,7 +218,8 @@ def batch_damage(nrve, steps):
         d = rng.normal(size=6)
         d *= 1e-2 / np.linalg.norm(d)
         ramps.append([d * (t + 1) / 10 for t in range(steps)])
-    params = dict(lin_stress=0, nr_max_its=10, nr_rel_tol=1e-6, cg_rel_tol=1e-8, cg_max_its=2000)
+    params = dict(lin_stress=0, nr_max_its=10, nr_rel_tol=1e-6, cg_rel_tol=1e-8, cg_max_its=2000,
+                  mixed_precision=int(mixed))
     out = {"metric": "batched damage RVEs (fibre layer, uneven sizes), RVE solves/s", "workload":
            f"{nrve} RVEs of 20..50^3 hexes, y-layer width U[0.1,0.3], damage matrix, {steps} ramp steps"}
     out.update(_batch_run(mp, descs, ramps, params, "batch_damage"))
@@ -376,6 +392,8 @@ def main():
                     help="mixed50 (default, the headline); the others measure the batched and adjoint paths")
     ap.add_argument("--rves", type=int, default=0, help="RVEs of a batch / adjoint workload (0: its default)")
     ap.add_argument("--load-steps", type=int, default=0, help="load steps of a batch workload (0: its default)")
+    ap.add_argument("--mixed", action="store_true",
+                    help="batch_j2 / batch_damage: mixed-precision linear solves (compared with the FP64 batch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for {args.gpus}); RVEs are independent, so a batch "
@@ -387,7 +405,8 @@ def main():
     if args.workload != "mixed50":
         fn = {"batch_j2": batch_j2, "batch_damage": batch_damage, "adjoint": adjoint_workload,
               "newton_mixed": newton_mixed_workload, "gp_concurrent": gp_concurrent_workload}[args.workload]
-        print(json.dumps(fn(args.rves, args.load_steps)), flush=True)
+        extra = {"mixed": True} if args.mixed and args.workload in ("batch_j2", "batch_damage") else {}
+        print(json.dumps(fn(args.rves, args.load_steps, **extra)), flush=True)
         return
 
     import micropp_amd as mp

@@ -239,6 +239,9 @@ typedef struct {
   int use_A0;                /* Newton on the initial matrix (micropp_params_t::use_A0): each distinct
                                 RVE's assembly_mat at u = 0 is made at mppd_batch_create, and the first
                                 newton.its_with_A0 Newton steps of every RVE solve with it */
+  int mixed_precision;       /* every linear solve as FP64 defect correction around an FP32 Jacobi-PCG
+                                on an FP32 copy of the matrix, to the reference CG's FP64 metric (the
+                                rules of mppd_newton_mixed with mp = NULL); cost = inner iterations */
 } mppd_batch_params;
 /* the micropp_params_t defaults (include/types.hpp:43-83); graph_chunk,
  * spmv_chunk and block_order chosen by timing at mppd_batch_create */
@@ -256,6 +259,10 @@ int mppd_batch_get_info(const mppd_batch *b, int *info);
 int mppd_batch_update_vars(mppd_batch *b);
 /* stats: {CG iterations queued, graph chunk length, graph launches, device ms of the last homogenize} */
 int mppd_batch_stats(const mppd_batch *b, double stats[4]);
+/* mixed_precision totals since mppd_batch_create: {outer steps, linear solves redone by the FP64 CG
+ * after an FP32 breakdown, linear solves stopped by the outer-step cap short of the tolerance}
+ * (no reference counterpart) */
+int mppd_batch_mixed_stats(const mppd_batch *b, long out[3]);
 /* the tuned shape: {graph chunk length, SpMV slots per load round, SpMV block order (1 = per XCD)} */
 int mppd_batch_tuning(const mppd_batch *b, int out[3]);
 

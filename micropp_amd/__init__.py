@@ -157,6 +157,7 @@ SIGNATURES = {
     "mppd_batch_get_info": (_I, [_VP, _PI]),
     "mppd_batch_update_vars": (_I, [_VP]),
     "mppd_batch_stats": (_I, [_VP, _PD]),
+    "mppd_batch_mixed_stats": (_I, [_VP, C.POINTER(C.c_long)]),
     "mppd_batch_slot_bytes": (_I, [_VP, C.POINTER(C.c_size_t)]),
     "mppd_batch_tuning": (_I, [_VP, _PI]),
     "mppd_batch_load": (_I, [_VP, _I, _VP, _VP, _VP]),
@@ -176,7 +177,8 @@ class BatchParams(C.Structure):
 
     _fields_ = [("newton", NewtonParams), ("lin_stress", C.c_int), ("calc_ctan_lin", C.c_int),
                 ("subiterations", C.c_int), ("nsubiterations", C.c_int), ("graph_chunk", C.c_int),
-                ("spmv_chunk", C.c_int), ("block_order", C.c_int), ("use_A0", C.c_int)]
+                ("spmv_chunk", C.c_int), ("block_order", C.c_int), ("use_A0", C.c_int),
+                ("mixed_precision", C.c_int)]
 
 
 class MixedParams(C.Structure):
@@ -623,7 +625,7 @@ class Batch:
         """rves: list of dicts with size, micro_type, geo, materials (3), coupling (FE_* id).
         params: newton fields (nr_max_its, nr_max_tol, nr_rel_tol, cg_max_its, cg_abs_tol, cg_rel_tol,
         its_with_A0), lin_stress, calc_ctan_lin, subiterations, nsubiterations, graph_chunk, spmv_chunk,
-        block_order, use_A0."""
+        block_order, use_A0, mixed_precision."""
         p = BatchParams()
         load().mppd_batch_params_default(C.byref(p))
         for k, v in params.items():
@@ -689,6 +691,12 @@ class Batch:
         out = np.zeros(4)
         _chk(load().mppd_batch_stats(self.h, _dp(out)), "mppd_batch_stats")
         return out
+
+    def mixed_stats(self):
+        """mixed_precision totals: (outer steps, FP64 fallbacks, solves stopped short by the outer cap)."""
+        out = np.zeros(3, dtype=np.int64)
+        _chk(load().mppd_batch_mixed_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_long))), "mppd_batch_mixed_stats")
+        return tuple(int(v) for v in out)
 
     def tuning(self):
         """(graph chunk length, SpMV slots per load round, SpMV blocks grouped per XCD)."""

@@ -198,6 +198,112 @@ __global__ void __launch_bounds__(256) kb_axpy(const BRve *rv, const int *map, c
   if (row < R.g.nrow) R.uw[row] += 1.0 * R.x[row];
 }
 
+// ---------------- mixed precision ----------------
+__global__ void __launch_bounds__(256) kb_m_prep(const BRve *rv, const int *map, const int *act, const int *blk0,
+                                                 const int *a0on) {
+  MPP_BATCH_PROLOGUE(map)
+  const int row = (blockIdx.x - blk0[rid]) * 256 + threadIdx.x;  // < nrow_pad
+  const double *A = cg_matrix(R, a0on, rid);
+  const size_t base = (size_t)(row >> 6) * kSlots * 64 + (row & 63);
+  for (int s = 0; s < kSlots; ++s) R.vals32[base + (size_t)s * 64] = (float)A[base + (size_t)s * 64];
+  double k = 0.0;
+  if (row < R.g.nrow) k = 1 / A[base + (size_t)(13 * 3 + row % 3) * 64];
+  R.kinv[row] = k;
+  R.kinv32[row] = (float)k;
+  R.x[row] = 0.0;
+}
+
+// partials per row block: (D^-1 r)^2, r32.z32, z32.z32
+__global__ void __launch_bounds__(256) kb_m_resid(const BRve *rv, const int *map, const int *act, const int *blk0,
+                                                  const int *a0on) {
+#pragma clang fp contract(off)
+  __shared__ double lds[4 * 3];
+  MPP_BATCH_PROLOGUE(map)
+  const int lb = blockIdx.x - blk0[rid];
+  const int row = lb * 256 + threadIdx.x;
+  const double rr = R.b[row] - row_dot_grid(R.g, cg_matrix(R, a0on, rid), (const double *)R.x, row);
+  const double zr = R.kinv[row] * rr;
+  const float r32 = (float)rr;
+  const float z32 = R.kinv32[row] * r32;
+  R.r[row] = rr;
+  R.r32[row] = r32;
+  R.z32[row] = z32;
+  R.p32[row] = z32;
+  R.x32[row] = 0.0f;
+  double v[3] = {zr * zr, (double)r32 * z32, (double)z32 * z32};
+  block_sum<3, 256>(v, lds);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; ++q) R.part[(size_t)lb * 3 + q] = v[q];
+}
+
+// one block per RVE: the outer test and the inner CG's start state (an
+// inactive or converged RVE's inner CG starts done)
+__global__ void __launch_bounds__(256) kb_m_outer(const BRve *rv, const int *act, int first, double rel,
+                                                  double abs_tol, double tau_min, int max_inner, BOuter *out) {
+  const int rid = blockIdx.x;
+  const BRve &R = rv[rid];
+  if (!act[rid]) {
+    if (threadIdx.x == 0) {
+      CgState c{};
+      c.done = 1;
+      R.cg[0] = c;
+    }
+    return;
+  }
+  double t[3];
+  seg_sum<3>(R.part, R.nb_row, t);
+  if (threadIdx.x == 0) {
+    BOuter &o = out[rid];
+    const double pn = sqrt(t[0]);
+    const double target = first ? fmax(abs_tol, rel * pn) : o.target;
+    const int conv = pn < target || pn == 0.0;
+    if (first) {
+      o.pn0 = pn;
+      o.target = target;
+    }
+    o.pn = pn;
+    o.conv = conv;
+    CgState c{};
+    c.rz = t[1];
+    c.pn0 = sqrt(t[2]);
+    c.pn = c.pn0;
+    c.min_err = 0.0;
+    // aim a little below what the outer test needs, but not below what FP32 resolves
+    c.rel_err = pn > 0.0 ? fmax(tau_min, 0.3 * target / pn) : 1.0;
+    c.max_its = max_inner;
+    c.its = 0;
+    c.done = conv || cg_done(c);
+    R.cg[0] = c;
+  }
+}
+
+__global__ void __launch_bounds__(256) kb_m_step1(const BRve *rv, const int *map, const int *blk0, int parity) {
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  cg_step1_body<kChunk, true>(R.g, blockIdx.x - blk0[rid], nullptr, (const float *)R.vals32,
+                              (const float *)R.p32, R.Ap32, R.part, R.cg + (parity & 1));
+}
+
+__global__ void __launch_bounds__(256) kb_m_step2(const BRve *rv, const int *map, const int *blk0, int parity) {
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  cg_step2_body(blockIdx.x - blk0[rid], R.nb_row, R.x32, R.r32, R.z32, (const float *)R.p32, (const float *)R.Ap32,
+                (const float *)R.kinv32, R.part, R.part + R.nb_row, R.cg + (parity & 1));
+}
+
+__global__ void __launch_bounds__(256) kb_m_step3(const BRve *rv, const int *map, const int *blk0, int parity) {
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  cg_step3_body(blockIdx.x - blk0[rid], R.nb_row, (const float *)R.z32, R.p32, R.part, R.part + R.nb_row,
+                R.cg + (parity & 1), R.cg + ((parity + 1) & 1));
+}
+
+__global__ void __launch_bounds__(256) kb_m_accum(const BRve *rv, const int *map, const int *act, const int *blk0) {
+  MPP_BATCH_PROLOGUE(map)
+  const int row = (blockIdx.x - blk0[rid]) * 256 + threadIdx.x;
+  R.x[row] += (double)R.x32[row];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -263,6 +369,28 @@ hipError_t launch_b_cg_iter(const BLaunch &L, int parity, hipStream_t st) {
 }
 hipError_t launch_b_axpy(const BLaunch &L, hipStream_t st) {
   hipLaunchKernelGGL(kb_axpy, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0);
+  return hipGetLastError();
+}
+
+hipError_t launch_bm_prep(const BLaunch &L, hipStream_t st) {
+  hipLaunchKernelGGL(kb_m_prep, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0, L.a0on);
+  return hipGetLastError();
+}
+hipError_t launch_bm_resid(const BLaunch &L, int first, double rel, double abs_tol, double tau_min, int max_inner,
+                           BOuter *out, hipStream_t st) {
+  hipLaunchKernelGGL(kb_m_resid, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0, L.a0on);
+  hipLaunchKernelGGL(kb_m_outer, dim3(L.nrve), dim3(256), 0, st, L.rv, L.act, first, rel, abs_tol, tau_min,
+                     max_inner, out);
+  return hipGetLastError();
+}
+hipError_t launch_bm_iter(const BLaunch &L, int parity, hipStream_t st) {
+  hipLaunchKernelGGL(kb_m_step1, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
+  hipLaunchKernelGGL(kb_m_step2, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
+  hipLaunchKernelGGL(kb_m_step3, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
+  return hipGetLastError();
+}
+hipError_t launch_bm_accum(const BLaunch &L, hipStream_t st) {
+  hipLaunchKernelGGL(kb_m_accum, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0);
   return hipGetLastError();
 }
 

@@ -26,6 +26,10 @@ struct BRve {
   double *vars_n, *vars_k, *vars_aux;
   double *part;                // partials: max(3 nb_row, 6 nb_elem)
   CgState *cg;                 // ping-pong pair
+  // mixed precision (nullptr without): the FP32 copy of the matrix being
+  // solved (vals' layout), the inner CG's FP32 vectors and Jacobi weights;
+  // x holds the FP64 correction du, r the FP64 residual, kinv the FP64 D^-1
+  float *vals32, *x32, *r32, *z32, *p32, *Ap32, *kinv32;
   int phase_mode;              // all phases elastic: one Ke per phase
   int use_vars;                // the RVE has history (vars_n / vars_k in use)
   int nb_row, nb_elem;
@@ -37,6 +41,13 @@ struct BShape {
   int nb;
   const int *map;
   const int *blk0;
+};
+
+// Outer (defect-correction) test of one RVE: pn = ||D^-1 (b - A du)||, pn0
+// its value at du = 0, target = max(abs, rel pn0), conv = pn < target.
+struct BOuter {
+  double pn, pn0, target;
+  int conv, pad;
 };
 
 struct BLaunch {
@@ -70,5 +81,19 @@ hipError_t launch_b_cg_iter(const BLaunch &L, int parity, hipStream_t st);
 hipError_t launch_b_spmv(const BLaunch &L, int shape, int parity, hipStream_t st);
 // uw += x
 hipError_t launch_b_axpy(const BLaunch &L, hipStream_t st);
+
+// Mixed precision: FP32 inner Jacobi-PCG with FP64 defect correction.
+// prep: vals32 = the matrix being solved (vals, or a0 where a0on), kinv /
+// kinv32 = 1 / diag, x (du) = 0
+hipError_t launch_bm_prep(const BLaunch &L, hipStream_t st);
+// r = b - A du (FP64) and the outer test into out[rve]; the inner CG starts
+// from r32 = r (x32 = 0) with rel_err = max(tau_min, 0.3 target / pn) and
+// max_inner iterations; first: du = 0, so pn0 and target are set
+hipError_t launch_bm_resid(const BLaunch &L, int first, double rel, double abs_tol, double tau_min, int max_inner,
+                           BOuter *out, hipStream_t st);
+// one inner CG iteration of every RVE (done RVEs return at once)
+hipError_t launch_bm_iter(const BLaunch &L, int parity, hipStream_t st);
+// du += x32
+hipError_t launch_bm_accum(const BLaunch &L, hipStream_t st);
 
 }  // namespace mpp

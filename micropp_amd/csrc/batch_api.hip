@@ -51,6 +51,8 @@ struct mppd_batch {
   // arenas
   double *vec = nullptr, *vals = nullptr, *ke = nullptr, *be = nullptr, *vars = nullptr, *part = nullptr;
   double *a0 = nullptr;  // initial matrices (use_A0), one per distinct RVE
+  float *f32 = nullptr;  // mixed precision: FP32 matrices and inner-CG vectors
+  BOuter *outer_d = nullptr, *outer_h = nullptr;
   CgState *cg = nullptr;
   std::map<std::tuple<int, int, int>, int *> cols;
   // per-call device arrays and pinned staging
@@ -64,6 +66,10 @@ struct mppd_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int gchunk = 0;
+  // mixed precision: graph of gchunk inner (FP32) CG iterations; counters
+  hipGraph_t graph32 = nullptr;
+  hipGraphExec_t gexec32 = nullptr;
+  long m_outer = 0, m_fallbacks = 0, m_short = 0;
   int spmv_shape = kSpmvDefault;  // tuned SpMV shape (launch_b_spmv)
   double cg_queued = 0, graph_launches = 0, last_ms = 0;
 
@@ -139,6 +145,23 @@ hipError_t queue_iters(mppd_batch *B, int g) {
   return hipSuccess;
 }
 
+// the inner FP32 CG iterations of the mixed-precision solve, gchunk of them
+int capture_graph32(mppd_batch *B) {
+  int rc = sync(B);
+  if (rc) return rc;
+  const BLaunch L = B->launch(M_MAIN);
+  MPP_HIP(hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < B->gchunk && e == hipSuccess; ++i) e = launch_bm_iter(L, i, B->st);
+  hipGraph_t graph = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(B->st, &graph);
+  if (e != hipSuccess) return hip_fail(e, "batch graph capture (mixed)");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+  B->graph32 = graph;
+  MPP_HIP(hipGraphInstantiate(&B->gexec32, graph, nullptr, nullptr, 0));
+  return MPPD_OK;
+}
+
 int capture_graph(mppd_batch *B, int g) {
   if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
@@ -207,6 +230,107 @@ int cg_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &its) {
   return MPPD_OK;
 }
 
+// The linear solves of cg_batch in mixed precision, as mppd_newton_mixed
+// does them for one RVE: FP64 defect correction du += e around an FP32
+// Jacobi-PCG on an FP32 copy of the matrix, stopping on the reference CG's
+// metric ||D^-1 (b - A du)|| < max(cg_abs_tol, cg_rel_tol ||D^-1 b||) in
+// FP64.  An inner CG that gains less than 10% over >= 32 iterations is
+// stopped (its RVE's next outer step restarts from the true residual); an RVE
+// whose inner CG breaks down (non-finite) is solved again by the FP64 CG.
+// its[r] = inner iterations (+ FP64 fallback iterations).  Leaves the main
+// mask = on and du in x.
+int cg_batch_mixed(mppd_batch *B, const std::vector<int> &on, std::vector<int> &its) {
+  const mppd_newton_params &p = B->prm.newton;
+  constexpr double kInnerTol = 1e-5;
+  constexpr int kMaxOuter = 10;
+  const int n = B->n, G = B->gchunk, max_inner = p.cg_max_its;
+  for (int r = 0; r < n; ++r) its[r] = 0;
+  int rc = set_mask(B, M_MAIN, on);
+  if (rc) return rc;
+  const BLaunch L = B->launch(M_MAIN);
+  MPP_HIP(launch_bm_prep(L, B->st));
+  std::vector<int> live(on), broke(n, 0);
+  for (int outer = 0;; ++outer) {
+    if ((rc = set_mask(B, M_MAIN, live))) return rc;
+    MPP_HIP(launch_bm_resid(L, outer == 0, p.cg_rel_tol, p.cg_abs_tol, kInnerTol, max_inner, B->outer_d, B->st));
+    MPP_HIP(hipMemcpyAsync(B->outer_h, B->outer_d, sizeof(BOuter) * n, hipMemcpyDeviceToHost, B->st));
+    if ((rc = read_cg(B))) return rc;
+    bool any = false;
+    for (int r = 0; r < n; ++r) {
+      if (!live[r]) continue;
+      if (!std::isfinite(B->outer_h[r].pn)) {
+        live[r] = 0;
+        broke[r] = 1;
+      } else if (B->outer_h[r].conv) {
+        live[r] = 0;
+      } else {
+        any = true;
+      }
+    }
+    if (!any) break;
+    if (outer >= kMaxOuter) {
+      for (int r = 0; r < n; ++r) B->m_short += live[r];
+      break;
+    }
+    ++B->m_outer;
+    // the inner CGs (the outer kernel started those of the other RVEs done),
+    // gchunk iterations per graph launch; each live RVE's ||z|| is checked at
+    // every read for breakdown and stall
+    std::vector<double> pn_mark(n, 0.0);
+    std::vector<int> its_mark(n, 0);
+    for (int r = 0; r < n; ++r) pn_mark[r] = B->cg_h[2 * r].pn;
+    int queued = 0;
+    bool all = false;
+    while (!all && queued < max_inner) {
+      MPP_HIP(hipGraphLaunch(B->gexec32, B->st));
+      queued += G;
+      B->cg_queued += G;
+      B->graph_launches += 1;
+      if ((rc = read_cg(B))) return rc;
+      all = true;
+      for (int r = 0; r < n; ++r) {
+        CgState &c = B->cg_h[2 * r + (queued & 1)];
+        if (!live[r] || c.done) continue;
+        bool stop = false;
+        if (!std::isfinite(c.pn)) {
+          stop = true;
+          broke[r] = 1;
+        } else if (c.its - its_mark[r] >= 32) {
+          if (c.pn > 0.9 * pn_mark[r]) stop = true;  // stalled: restart from the true residual
+          its_mark[r] = c.its;
+          pn_mark[r] = c.pn;
+        }
+        if (stop) {
+          c.done = 1;
+          MPP_HIP(hipMemcpyAsync(B->cg + 2 * r + (queued & 1), &c, sizeof(CgState), hipMemcpyHostToDevice, B->st));
+        } else {
+          all = false;
+        }
+      }
+    }
+    if ((rc = sync(B))) return rc;
+    for (int r = 0; r < n; ++r) {
+      if (!live[r]) continue;
+      its[r] += B->cg_h[2 * r + (queued & 1)].its;
+      if (broke[r]) live[r] = 0;
+    }
+    if ((rc = set_mask(B, M_MAIN, live))) return rc;
+    MPP_HIP(launch_bm_accum(L, B->st));
+  }
+  bool any_broke = false;
+  for (int r = 0; r < n; ++r) any_broke |= broke[r] != 0;
+  if (any_broke) {  // the FP64 Jacobi-PCG on the same systems, from du = 0
+    std::vector<int> fb(n, 0);
+    if ((rc = cg_batch(B, broke, fb))) return rc;
+    for (int r = 0; r < n; ++r)
+      if (broke[r]) {
+        its[r] += fb[r];
+        ++B->m_fallbacks;
+      }
+  }
+  return set_mask(B, M_MAIN, on);
+}
+
 // The tangent of Newton iteration its[r] of the RVEs in `on`: element
 // stiffness + vals (phase-mode RVEs: once), or, with use_A0, the initial
 // matrix for the first its_with_A0 iterations (src/solve.cpp:50-66).
@@ -265,7 +389,8 @@ int newton_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &it
     }
     if (!any) break;
     if ((rc = mat_batch(B, run, its))) return rc;
-    if ((rc = cg_batch(B, run, cg_its))) return rc;  // leaves the main mask = run
+    rc = B->prm.mixed_precision ? cg_batch_mixed(B, run, cg_its) : cg_batch(B, run, cg_its);
+    if (rc) return rc;  // leaves the main mask = run
     for (int r = 0; r < n; ++r) sits[r] += cg_its[r];
     MPP_HIP(launch_b_axpy(L, B->st));
     MPP_HIP(launch_b_rhs(L, B->norms_d, B->st));
@@ -414,12 +539,14 @@ void destroy(mppd_batch *B) {
   if (B->st) (void)hipStreamSynchronize(B->st);
   if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
+  if (B->gexec32) (void)hipGraphExecDestroy(B->gexec32);
+  if (B->graph32) (void)hipGraphDestroy(B->graph32);
   for (int k = 0; k < 3; ++k) {
     dfree(B->map_d[k]);
     dfree(B->blk0_d[k]);
   }
-  for (void *p : {(void *)B->rv, (void *)B->vec, (void *)B->vals, (void *)B->a0, (void *)B->ke, (void *)B->be,
-                  (void *)B->vars, (void *)B->part, (void *)B->cg, (void *)B->flags_d, (void *)B->eps_d, (void *)B->norms_d,
+  for (void *p : {(void *)B->rv, (void *)B->vec, (void *)B->vals, (void *)B->a0, (void *)B->f32,
+                  (void *)B->outer_d, (void *)B->ke, (void *)B->be, (void *)B->vars, (void *)B->part, (void *)B->cg, (void *)B->flags_d, (void *)B->eps_d, (void *)B->norms_d,
                   (void *)B->sig_d})
     dfree(p);
   for (auto &kv : B->cols) dfree(kv.second);
@@ -427,7 +554,8 @@ void destroy(mppd_batch *B) {
     dfree(B->mask_d[k]);
     if (B->mask_h[k]) (void)hipHostFree(B->mask_h[k]);
   }
-  for (void *p : {(void *)B->flags_h, (void *)B->eps_h, (void *)B->norms_h, (void *)B->sig_h, (void *)B->cg_h})
+  for (void *p : {(void *)B->flags_h, (void *)B->eps_h, (void *)B->norms_h, (void *)B->sig_h, (void *)B->cg_h,
+                  (void *)B->outer_h})
     if (p) (void)hipHostFree(p);
   for (mppd_rve *r : B->rves) mppd_rve_destroy(r);
   if (B->st) (void)hipStreamDestroy(B->st);
@@ -588,6 +716,21 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
     o_be += (size_t)g.nelem * 24;
     o_part += std::max<size_t>(3 * (size_t)R.nb_row, 6 * (size_t)R.nb_elem);
   }
+  if (B->prm.mixed_precision) {
+    constexpr int kVecs32 = 6;  // x32 r32 z32 p32 Ap32 kinv32
+    if ((rc = dalloc(&B->f32, slots + rows * kVecs32))) return rc;
+    size_t os = 0, orow = 0;
+    for (int r = 0; r < n; ++r) {
+      BRve &R = B->h[r];
+      R.vals32 = B->f32 + os;
+      float **v[kVecs32] = {&R.x32, &R.r32, &R.z32, &R.p32, &R.Ap32, &R.kinv32};
+      for (int q = 0; q < kVecs32; ++q) *v[q] = B->f32 + slots + rows * q + orow;
+      os += (size_t)R.g.nrow_pad * kSlots;
+      orow += R.g.nrow_pad;
+    }
+    if ((rc = dalloc(&B->outer_d, n))) return rc;
+    if ((rc = pinned(&B->outer_h, n))) return rc;
+  }
   for (int k = 0; k < M_NUM; ++k) {
     if ((rc = dalloc(&B->mask_d[k], n))) return rc;
     if ((rc = pinned(&B->mask_h[k], n))) return rc;
@@ -619,6 +762,7 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
   else
     rc = autotune(B);
   if (rc) return rc;
+  if (B->prm.mixed_precision && (rc = capture_graph32(B))) return rc;
   if ((rc = sync(B))) return rc;
   // micropp constructor: ctan of every GP from the linear solves
   bool any_fe = false;
@@ -650,6 +794,7 @@ void mppd_batch_params_default(mppd_batch_params *p) {
   p->spmv_chunk = 0;
   p->block_order = -1;
   p->use_A0 = 0;
+  p->mixed_precision = 0;
 }
 
 int mppd_batch_create(mppd_batch **out, int nrve, const mppd_batch_rve *rves, const mppd_batch_params *p) {
@@ -872,6 +1017,14 @@ int mppd_batch_stats(const mppd_batch *b, double stats[4]) {
   stats[1] = b->gchunk;
   stats[2] = b->graph_launches;
   stats[3] = b->last_ms;
+  return MPPD_OK;
+}
+
+int mppd_batch_mixed_stats(const mppd_batch *b, long out[3]) {
+  if (!b || !out) return fail(MPPD_ERR_ARG, "null argument");
+  out[0] = b->m_outer;
+  out[1] = b->m_fallbacks;
+  out[2] = b->m_short;
   return MPPD_OK;
 }
 

@@ -469,17 +469,18 @@ __device__ __forceinline__ double row_dot(const int *cols, const double *vals, c
 // product for a finite x, and a sum that starts at +0 is unchanged by adding
 // either signed zero.  So the result equals row_dot's bit for bit.  (A matrix
 // uploaded from the host may hold values in those slots: use row_dot.)
-template <int CH = kChunk>
-__device__ __forceinline__ double row_dot_grid(const Grid &g, const double *vals, const double *x, int row) {
+// T = float: the FP32 inner CG of the mixed-precision batch (FP32 sums).
+template <int CH = kChunk, typename T>
+__device__ __forceinline__ T row_dot_grid(const Grid &g, const T *vals, const T *x, int row) {
 #pragma clang fp contract(off)
   static_assert(kSlots % CH == 0 && CH % 3 == 0, "chunks tile the row in whole neighbours");
   const size_t base = (size_t)(row >> 6) * kSlots * 64 + (row & 63);
-  const double *v = vals + base;
+  const T *v = vals + base;
   const int n = row / 3, nxny = g.nx * g.ny, hi = g.nn - 1;
-  double acc = 0;
+  T acc = 0;
 #pragma unroll
   for (int ch = 0; ch < kSlots / CH; ++ch) {
-    double vi[CH], xv[CH];
+    T vi[CH], xv[CH];
 #pragma unroll
     for (int k = 0; k < CH; ++k) vi[k] = __builtin_nontemporal_load(&v[(ch * CH + k) * 64]);
 #pragma unroll
@@ -508,7 +509,10 @@ __device__ __forceinline__ double row_dot_grid(const Grid &g, const double *vals
 // iteration k are read from state[k & 1] and written (by block 0 of step3)
 // to state[(k+1) & 1].  Once the stopping test holds, later iterations
 // return at once and step3 only carries the final state forward.
-// Partials: part1 = part[0 .. nb), part2 = part[nb .. 3 nb).
+// Partials: part1 = part[0 .. nb), part2 = part[nb .. 3 nb).  The steps take
+// vectors (and, in step 1, the matrix) of type T: double for the FP64 CG,
+// float for the inner CG of the mixed-precision batch, whose products and
+// partials are still formed in FP64.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int cg_done(const CgState &c) {
   const bool conv = (c.pn < c.min_err) || (c.pn < c.pn0 * c.rel_err);
@@ -562,23 +566,28 @@ __device__ __forceinline__ void cg_init_state_body(const double *part, int npart
 
 // STENCIL: the matrix has nonzeros only on in-grid stencil slots (assembled
 // here), so the columns are computed (row_dot_grid); otherwise they are read.
-template <int CH = kChunk, bool STENCIL = true>
-__device__ __forceinline__ void cg_step1_body(const Grid &g, int lb, const int *cols, const double *vals,
-                                              const double *p, double *Ap, double *part1, const CgState *cur) {
+template <int CH = kChunk, bool STENCIL = true, typename T>
+__device__ __forceinline__ void cg_step1_body(const Grid &g, int lb, const int *cols, const T *vals, const T *p,
+                                              T *Ap, double *part1, const CgState *cur) {
 #pragma clang fp contract(off)
   __shared__ double lds[4];
   if (cur->done) return;
   const int row = lb * 256 + threadIdx.x;
-  const double y = STENCIL ? row_dot_grid<CH>(g, vals, p, row) : row_dot<CH>(cols, vals, p, row);
+  T y;
+  if constexpr (STENCIL)
+    y = row_dot_grid<CH>(g, vals, p, row);
+  else
+    y = row_dot<CH>(cols, vals, p, row);
   Ap[row] = y;
-  double v[1] = {p[row] * y};
+  double v[1] = {(double)p[row] * (double)y};
   block_sum<1, 256>(v, lds);
   if (threadIdx.x == 0) part1[lb] = v[0];
 }
 
-__device__ __forceinline__ void cg_step2_body(int lb, int nb, double *x, double *r, double *z, const double *p,
-                                              const double *Ap, const double *kinv, const double *part1,
-                                              double *part2, const CgState *cur) {
+template <typename T>
+__device__ __forceinline__ void cg_step2_body(int lb, int nb, T *x, T *r, T *z, const T *p, const T *Ap,
+                                              const T *kinv, const double *part1, double *part2,
+                                              const CgState *cur) {
 #pragma clang fp contract(off)
   __shared__ double lds[4 * 2];
   if (cur->done) return;
@@ -587,11 +596,11 @@ __device__ __forceinline__ void cg_step2_body(int lb, int nb, double *x, double 
   const double alpha = cur->rz / pap[0];
   const int row = lb * 256 + threadIdx.x;
   x[row] += alpha * p[row];
-  const double rv = r[row] - alpha * Ap[row];
+  const T rv = r[row] - alpha * Ap[row];
   r[row] = rv;
-  const double zv = kinv[row] * rv;
+  const T zv = kinv[row] * rv;
   z[row] = zv;
-  double v[2] = {zv * zv, rv * zv};
+  double v[2] = {(double)zv * zv, (double)rv * zv};
   block_sum<2, 256>(v, lds);
   if (threadIdx.x == 0) {
     part2[(size_t)lb * 2 + 0] = v[0];
@@ -599,7 +608,8 @@ __device__ __forceinline__ void cg_step2_body(int lb, int nb, double *x, double 
   }
 }
 
-__device__ __forceinline__ void cg_step3_body(int lb, int nb, const double *z, double *p, const double *part1,
+template <typename T>
+__device__ __forceinline__ void cg_step3_body(int lb, int nb, const T *z, T *p, const double *part1,
                                               const double *part2, const CgState *cur, CgState *nxt) {
 #pragma clang fp contract(off)
   if (cur->done) {

@@ -1,0 +1,63 @@
+"""GPU: batches of RVEs with mixed-precision linear solves (mppd_batch_params::mixed_precision).
+
+Every Newton linear solve of every RVE runs as FP64 defect correction around an
+FP32 Jacobi-PCG on an FP32 copy of its matrix, to the reference CG's metric in
+FP64 (the rules of mppd_newton_mixed).  The bar is the FP64 batch, itself equal
+bit for bit to micropp<3> instances and within 1e-6 of the oracle:
+- sigma within 1e-6 relative and ctan within 1e-4 (ctan is a forward difference
+  over a 1e-8 strain step, so it multiplies solve-level differences by ~1e8 /
+  |sigma|) for the elastic, J2 and elastic-regime RVEs;
+- 1e-4 / 1e-2 for the damaged RVE, whose FD Newton tangent is nonsymmetric and
+  ill-conditioned, so that its state carries the linear solves' error (the
+  bars of the single-RVE mixed Newton);
+- the same nonlinear (history) and convergence flags.
+"""
+import numpy as np
+import pytest
+
+from test_gpu_batch import KW, SPECS, mixed_batch, rel, run_batch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mp():
+    import micropp_amd
+
+    micropp_amd.load()
+    if not micropp_amd.gpu_available():
+        pytest.skip("no HIP device")
+    return micropp_amd
+
+
+@pytest.mark.parametrize("its_with_A0", [0, 1])
+def test_mixed_batch_matches_fp64_batch(mp, its_with_A0):
+    descs = mixed_batch(mp)
+    kw = dict(KW, use_A0=1, its_with_A0=its_with_A0) if its_with_A0 else dict(KW)
+    steps = 2
+    r64, s64 = run_batch(mp, descs, steps, **kw)
+    r32, s32 = run_batch(mp, descs, steps, mixed_precision=1, **kw)
+    print(f"CG iterations queued: fp64 {s64[0]:.0f}, mixed {s32[0]:.0f}", flush=True)
+    for t in range(steps):
+        sig64, ctan64, info64 = r64[t]
+        sig32, ctan32, info32 = r32[t]
+        for i, d in enumerate(descs):
+            ts, tc = (1e-4, 1e-2) if d["kind"] == "damage" else (1e-6, 1e-4)
+            es, ec = rel(sig32[i], sig64[i]), rel(ctan32[i], ctan64[i])
+            print(f"step {t} rve {i} ({d['kind']}): info mixed {tuple(info32[i])} fp64 {tuple(info64[i])}; "
+                  f"sigma {es:.2e} ctan {ec:.2e}", flush=True)
+            assert es < ts and ec < tc
+            assert info32[i][0] == info64[i][0] and info32[i][2] == info64[i][2]
+
+
+def test_mixed_batch_stats(mp):
+    """An elastic batch: every solve converges in FP32 steps (no FP64 fallback, no short solve)."""
+    descs = [d for d in mixed_batch(mp) if d["kind"] == "elastic"]
+    b = mp.Batch([dict(size=d["size"], micro_type=d["micro_type"], geo=d["geo"],
+                       materials=[mp.material(*s) for s in SPECS[d["kind"]]], coupling=d["coupling"])
+                  for d in descs], mixed_precision=1, **KW)
+    b.set_strains(np.tile([1e-3, -2e-4, 3e-4, 5e-4, 0.0, -1e-4], (len(descs), 1)))
+    b.homogenize()
+    outer, fallbacks, short = b.mixed_stats()
+    print(f"outer steps {outer}, fallbacks {fallbacks}, short {short}", flush=True)
+    assert outer > 0 and fallbacks == 0 and short == 0
