@@ -307,11 +307,11 @@ def newton_mixed_workload(nrve, steps):
 
 
 def gp_concurrent_workload(nrve, steps):
-    """One micropp<3> instance with mixed-precision Newton (which does not run as a batch): nrve FE_ONE_WAY
-    Gauss points of a 30^3 z-fibre RVE with a J2 matrix (E=1e6, nu=0.3, Sy=1e4, Ka=1e5), each with its own
-    seeded strain ramp (peak components U[-3e-3, 3e-3]).  The same instance type with MPPD_GP_WORKERS=1 (one
-    solver, GPs one after another) and =4 (four solvers on their own streams, GPs concurrently); the results
-    must be equal bit for bit."""
+    """One micropp<3> instance with mixed-precision Newton: nrve FE_ONE_WAY Gauss points of a 30^3 z-fibre RVE
+    with a J2 matrix (E=1e6, nu=0.3, Sy=1e4, Ka=1e5), each with its own seeded strain ramp (peak components
+    U[-3e-3, 3e-3]).  With the batch off (MPPD_GP_BATCH=0): MPPD_GP_WORKERS=1 (one solver, GPs one after
+    another) and =4 (four solvers on their own streams, GPs concurrently), whose results must be equal bit for
+    bit; then the default route, the GPs as one batch with mixed-precision solves."""
     import micropp_amd as mp
 
     nrve = nrve or 16
@@ -322,8 +322,12 @@ def gp_concurrent_workload(nrve, steps):
     rng = np.random.default_rng(SEED)
     d = rng.uniform(-3e-3, 3e-3, (nrve, 6))
     res = {}
-    for workers in (1, 4):
-        os.environ["MPPD_GP_WORKERS"] = str(workers)
+    for workers in (1, 4, "batch"):
+        if workers == "batch":
+            os.environ.pop("MPPD_GP_BATCH")
+        else:
+            os.environ["MPPD_GP_BATCH"] = "0"
+            os.environ["MPPD_GP_WORKERS"] = str(workers)
         m = mp.Micropp3(31, mp.MIC_CILI_FIB_Z, (0.2, 0, 0, 0), mats, ngp=nrve, coupling=[mp.FE_ONE_WAY] * nrve,
                         **kw)
         sig, t = [], 0.0
@@ -339,11 +343,15 @@ def gp_concurrent_workload(nrve, steps):
             print(f"gp_concurrent: workers={workers} step {k} done", file=sys.stderr, flush=True)
         res[workers] = (np.array(sig), t, sum(m.is_non_linear(g) for g in range(nrve)))
     os.environ.pop("MPPD_GP_WORKERS")
+    s4, sb = res[4][0], res["batch"][0]
     return {"metric": "Gauss-point solves/s, mixed-precision Newton at 30^3 (one vs four concurrent solvers)",
             "value": nrve * steps / res[4][1], "unit": "GP solves/s (4 solvers)",
             "workload": f"{nrve} GPs of a 30^3 z-fibre RVE, J2 matrix, {steps} load steps",
             "one_solver_gp_solves_per_s": nrve * steps / res[1][1], "speedup": res[1][1] / res[4][1],
-            "nonlinear_gps": int(res[4][2]), "bit_equal": bool(np.array_equal(res[1][0], res[4][0]))}
+            "nonlinear_gps": int(res[4][2]), "bit_equal": bool(np.array_equal(res[1][0], res[4][0])),
+            "batched_gp_solves_per_s": nrve * steps / res["batch"][1],
+            "batched_vs_4_solvers": res[4][1] / res["batch"][1],
+            "batched_sigma_max_rel_err_vs_4_solvers": float(np.abs(sb - s4).max() / np.abs(s4).max())}
 
 
 def pmc_traffic(summary):

@@ -212,13 +212,13 @@ struct micropp<tdim>::impl {
   // The batch is made at the first homogenize, sized to half the free device
   // memory; MPPD_GP_BATCH=0 in the environment keeps the one-GP-at-a-time
   // path, MPPD_GP_BATCH=n caps the slots at n.  With use_A0 the batch keeps
-  // its own copy of the initial matrix.  With mixed precision (the batch
-  // solves in FP64) or with fewer than two FE GPs the one-GP path runs.
+  // its own copy of the initial matrix; with mixed precision it solves in
+  // mixed precision.  With fewer than two FE GPs the one-GP path runs.
   bool batch_ready() {
     if (batch) return true;
     if (batch_tried) return false;
     batch_tried = true;
-    if (mixed || fe_ids.size() < 2) return false;
+    if (fe_ids.size() < 2) return false;
     const char *env = getenv("MPPD_GP_BATCH");
     const long cap = env ? strtol(env, nullptr, 10) : -1;
     if (cap == 0) return false;
@@ -244,6 +244,7 @@ struct micropp<tdim>::impl {
     p.lin_stress = lin_stress;
     p.calc_ctan_lin = 0;  // the instance's ctan_lin_fe is loaded into every slot
     p.use_A0 = use_A0;
+    p.mixed_precision = mixed;
     p.subiterations = subiterations;
     p.nsubiterations = nsubiterations;
     if (mppd_batch_create(&batch, (int)n, ds.data(), &p) != MPPD_OK) {
@@ -306,8 +307,8 @@ struct micropp<tdim>::impl {
     }
   }
 
-  // FE GPs that do not run as a batch (mixed precision,
-  // MPPD_GP_BATCH=0, or no batch could be made) run concurrently, in place of
+  // FE GPs that do not run as a batch (MPPD_GP_BATCH=0, or no batch could be
+  // made) run concurrently, in place of
   // the reference's OpenMP loop over GPs (src/homogenize.cpp:72): up to
   // MPPD_GP_WORKERS (default 4) solvers, each with its own stream, take GPs
   // in turn from host threads.  A GP's kernels and their order do not depend

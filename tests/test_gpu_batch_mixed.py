@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 from test_gpu_batch import KW, SPECS, mixed_batch, rel, run_batch
+import test_gpu_gp_batch as gpb
 
 pytestmark = pytest.mark.gpu
 
@@ -61,3 +62,24 @@ def test_mixed_batch_stats(mp):
     outer, fallbacks, short = b.mixed_stats()
     print(f"outer steps {outer}, fallbacks {fallbacks}, short {short}", flush=True)
     assert outer > 0 and fallbacks == 0 and short == 0
+
+
+def test_micropp_mixed_instance_runs_as_mixed_batch(mp, monkeypatch):
+    """A micropp<3> instance with mixed_precision runs its FE GPs as a batch with mixed-precision solves;
+    against the same instance on concurrent solvers (MPPD_GP_BATCH=0, the node-block FP32 CG of
+    mppd_newton_mixed): sigma within 1e-6, ctan within 1e-4, the same nonlinear and convergence flags, over
+    three J2 load steps with history allocation."""
+    steps = 3
+    bat, seq = (gpb.make(mp, lin_stress=0, mixed_precision=1, **gpb.KW) for _ in range(2))
+    rp = gpb.ramps(bat.ngp, steps)
+    nl = 0
+    for t in range(steps):
+        a = gpb.step(bat, rp, t, monkeypatch, -1)
+        b = gpb.step(seq, rp, t, monkeypatch, 0)
+        for g, (ra, rb) in enumerate(zip(a, b)):
+            es, ec = rel(ra[0], rb[0]), rel(ra[1], rb[1])
+            print(f"step {t} gp {g}: flags batch {ra[2]} solvers {rb[2]}; sigma {es:.2e} ctan {ec:.2e}", flush=True)
+            assert es < 1e-6 and ec < 1e-4
+            assert ra[2][0] == rb[2][0] and ra[2][2] == rb[2][2]
+            nl += ra[2][0]
+    assert nl > 0, "the case must allocate history"

@@ -1,8 +1,7 @@
 """GPU: micropp<3>::homogenize running FE GPs concurrently on several solvers.
 
-When the FE GPs of an instance do not run as one batch (mixed-precision
-solves; MPPD_GP_BATCH=0), the reference's
-OpenMP loop over GPs (src/homogenize.cpp:72) becomes up to MPPD_GP_WORKERS
+When the FE GPs of an instance do not run as one batch (MPPD_GP_BATCH=0), the
+reference's OpenMP loop over GPs (src/homogenize.cpp:72) becomes up to MPPD_GP_WORKERS
 solvers, each with its own HIP stream, taking GPs in turn from host threads.
 A GP's kernels and their order do not depend on which solver runs it, so the
 bar against one solver (MPPD_GP_WORKERS=1) is bit-for-bit equality of sigma,
@@ -49,9 +48,9 @@ def step(m, rp, t, monkeypatch, workers, batch=None):
 
 
 CASES = {
-    "mixed": dict(mixed_precision=1),
-    "use_A0": dict(use_A0=1, its_with_A0=1, batch=0),  # an instance with use_A0 otherwise runs as a batch
-    "mixed_A0": dict(mixed_precision=1, use_A0=1, its_with_A0=1),
+    "mixed": dict(mixed_precision=1, batch=0),
+    "use_A0": dict(use_A0=1, its_with_A0=1, batch=0),
+    "mixed_A0": dict(mixed_precision=1, use_A0=1, its_with_A0=1, batch=0),
     "fp64_no_batch": dict(batch=0),
 }
 
