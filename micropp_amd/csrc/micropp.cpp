@@ -224,6 +224,9 @@ struct micropp<tdim>::impl {
     if (cap == 0) return false;
     size_t per = 0, fr = 0, tot = 0;
     check(mppd_batch_slot_bytes(rve, &per), "batch sizing");
+    // mixed precision adds an FP32 copy of the values and six FP32 vectors,
+    // less than half of a slot's FP64 bytes
+    if (mixed) per += per / 2;
     check(mppd_mem_info(&fr, &tot), "device memory");
     size_t n = std::min(fe_ids.size(), fr / 2 / std::max<size_t>(per, 1));
     if (cap > 0) n = std::min<size_t>(n, (size_t)cap);
