@@ -277,11 +277,16 @@ __global__ void __launch_bounds__(256) kb_m_outer(const BRve *rv, const int *act
   }
 }
 
-__global__ void __launch_bounds__(256) kb_m_step1(const BRve *rv, const int *map, const int *blk0, int parity) {
-  const int rid = map[blockIdx.x];
+// shapes as kb_cg_step1
+template <int CH, bool XCD>
+__global__ void __launch_bounds__(256) kb_m_step1(const BRve *rv, const int *map, const int *blk0, int nb, int per,
+                                                  int parity) {
+  const int b = XCD ? (int)((blockIdx.x & 7) * per + (blockIdx.x >> 3)) : (int)blockIdx.x;
+  if (b >= nb) return;
+  const int rid = map[b];
   const BRve &R = rv[rid];
-  cg_step1_body<kChunk, true>(R.g, blockIdx.x - blk0[rid], nullptr, (const float *)R.vals32,
-                              (const float *)R.p32, R.Ap32, R.part, R.cg + (parity & 1));
+  cg_step1_body<CH, true>(R.g, b - blk0[rid], nullptr, (const float *)R.vals32, (const float *)R.p32, R.Ap32,
+                          R.part, R.cg + (parity & 1));
 }
 
 __global__ void __launch_bounds__(256) kb_m_step2(const BRve *rv, const int *map, const int *blk0, int parity) {
@@ -383,8 +388,23 @@ hipError_t launch_bm_resid(const BLaunch &L, int first, double rel, double abs_t
                      max_inner, out);
   return hipGetLastError();
 }
+hipError_t launch_bm_spmv(const BLaunch &L, int shape, int parity, hipStream_t st) {
+  const int nb = L.row.nb, per = (nb + 7) / 8;
+  const dim3 plain(nb), xcd(8 * per);
+  switch (shape) {
+    case 0: hipLaunchKernelGGL((kb_m_step1<9, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 1: hipLaunchKernelGGL((kb_m_step1<27, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 2: hipLaunchKernelGGL((kb_m_step1<81, false>), plain, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 3: hipLaunchKernelGGL((kb_m_step1<9, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 4: hipLaunchKernelGGL((kb_m_step1<27, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    case 5: hipLaunchKernelGGL((kb_m_step1<81, true>), xcd, dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nb, per, parity); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 hipError_t launch_bm_iter(const BLaunch &L, int parity, hipStream_t st) {
-  hipLaunchKernelGGL(kb_m_step1, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
+  const hipError_t e = launch_bm_spmv(L, L.spmv_shape32, parity, st);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kb_m_step2, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
   hipLaunchKernelGGL(kb_m_step3, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
   return hipGetLastError();

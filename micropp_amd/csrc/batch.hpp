@@ -56,7 +56,8 @@ struct BLaunch {
   const int *act;  // per-RVE activity mask of this launch
   const int *a0on;  // per RVE: the CG solves with a0 instead of vals
   BShape row, elem, ke;
-  int spmv_shape = kSpmvDefault;  // see launch_b_spmv
+  int spmv_shape = kSpmvDefault;    // see launch_b_spmv
+  int spmv_shape32 = kSpmvDefault;  // the same choice for the inner FP32 SpMV (launch_bm_spmv)
 };
 
 hipError_t launch_b_set_bc(const BLaunch &L, const double *eps, hipStream_t st);
@@ -93,6 +94,8 @@ hipError_t launch_bm_resid(const BLaunch &L, int first, double rel, double abs_t
                            BOuter *out, hipStream_t st);
 // one inner CG iteration of every RVE (done RVEs return at once)
 hipError_t launch_bm_iter(const BLaunch &L, int parity, hipStream_t st);
+// the inner FP32 SpMV alone in one of kSpmvShapes shapes, as launch_b_spmv
+hipError_t launch_bm_spmv(const BLaunch &L, int shape, int parity, hipStream_t st);
 // du += x32
 hipError_t launch_bm_accum(const BLaunch &L, hipStream_t st);
 

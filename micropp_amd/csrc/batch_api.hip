@@ -70,7 +70,8 @@ struct mppd_batch {
   hipGraph_t graph32 = nullptr;
   hipGraphExec_t gexec32 = nullptr;
   long m_outer = 0, m_fallbacks = 0, m_short = 0;
-  int spmv_shape = kSpmvDefault;  // tuned SpMV shape (launch_b_spmv)
+  int spmv_shape = kSpmvDefault;    // tuned SpMV shape (launch_b_spmv)
+  int spmv_shape32 = kSpmvDefault;  // tuned inner FP32 SpMV shape (launch_bm_spmv)
   double cg_queued = 0, graph_launches = 0, last_ms = 0;
 
   BLaunch launch(int mask) const {
@@ -83,6 +84,7 @@ struct mppd_batch {
     L.elem = {nb[1], map_d[1], blk0_d[1]};
     L.ke = {nb[2], map_d[2], blk0_d[2]};
     L.spmv_shape = spmv_shape;
+    L.spmv_shape32 = spmv_shape32;
     return L;
   }
 };
@@ -466,7 +468,8 @@ int time_chunk(mppd_batch *B, int g, double *ms_per_it) {
 // the parameters) timed on this batch's own matrices with every RVE's CG
 // state live; the fastest is kept.  The shapes differ only in load
 // scheduling and block order, so the results do not depend on the choice.
-int tune_spmv(mppd_batch *B) {
+// fp32: the same for the inner FP32 SpMV of the mixed-precision solves.
+int tune_spmv(mppd_batch *B, bool fp32 = false) {
   const int chunks[3] = {9, 27, 81};
   std::vector<int> cand;
   for (int x = 0; x < 2; ++x)
@@ -474,8 +477,9 @@ int tune_spmv(mppd_batch *B) {
       if ((B->prm.spmv_chunk == 0 || B->prm.spmv_chunk == chunks[c]) &&
           (B->prm.block_order < 0 || B->prm.block_order == x))
         cand.push_back(3 * x + c);
+  int &pick = fp32 ? B->spmv_shape32 : B->spmv_shape;
   if (cand.size() == 1) {
-    B->spmv_shape = cand[0];
+    pick = cand[0];
     return MPPD_OK;
   }
   int rc = sync(B);
@@ -494,9 +498,10 @@ int tune_spmv(mppd_batch *B) {
   hipError_t e = hipSuccess;
   for (int shape : cand) {
     constexpr int kReps = 10;
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_b_spmv(L, shape, 0, B->st);  // warm
+    auto spmv = [&]() { return fp32 ? launch_bm_spmv(L, shape, 0, B->st) : launch_b_spmv(L, shape, 0, B->st); };
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = spmv();  // warm
     if (e == hipSuccess) e = hipEventRecord(e0, B->st);
-    for (int i = 0; i < kReps && e == hipSuccess; ++i) e = launch_b_spmv(L, shape, 0, B->st);
+    for (int i = 0; i < kReps && e == hipSuccess; ++i) e = spmv();
     if (e == hipSuccess) e = hipEventRecord(e1, B->st);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.f;
@@ -504,7 +509,7 @@ int tune_spmv(mppd_batch *B) {
     if (e != hipSuccess) break;
     if (shape == cand[0] || ms < best) {
       best = ms;
-      B->spmv_shape = shape;
+      pick = shape;
     }
   }
   (void)hipEventDestroy(e0);
@@ -762,7 +767,7 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
   else
     rc = autotune(B);
   if (rc) return rc;
-  if (B->prm.mixed_precision && (rc = capture_graph32(B))) return rc;
+  if (B->prm.mixed_precision && ((rc = tune_spmv(B, true)) || (rc = capture_graph32(B)))) return rc;
   if ((rc = sync(B))) return rc;
   // micropp constructor: ctan of every GP from the linear solves
   bool any_fe = false;

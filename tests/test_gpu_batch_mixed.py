@@ -51,6 +51,16 @@ def test_mixed_batch_matches_fp64_batch(mp, its_with_A0):
             assert info32[i][0] == info64[i][0] and info32[i][2] == info64[i][2]
 
 
+def test_mixed_batch_spmv_shape_does_not_change_results(mp):
+    """The inner FP32 SpMV's tuned shapes (slots in flight, block order) change only load scheduling: the
+    mixed batch's results are the same bit for bit in the extreme shapes."""
+    descs = mixed_batch(mp)[:4]
+    ra, _ = run_batch(mp, descs, 1, mixed_precision=1, spmv_chunk=9, block_order=0, **KW)
+    rb, _ = run_batch(mp, descs, 1, mixed_precision=1, spmv_chunk=81, block_order=1, **KW)
+    for x, y in zip(ra[0], rb[0]):
+        assert np.array_equal(x, y)
+
+
 def test_mixed_batch_stats(mp):
     """An elastic batch: every solve converges in FP32 steps (no FP64 fallback, no short solve)."""
     descs = [d for d in mixed_batch(mp) if d["kind"] == "elastic"]
