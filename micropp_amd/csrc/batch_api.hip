@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -70,6 +71,10 @@ struct mppd_batch {
   hipGraph_t graph32 = nullptr;
   hipGraphExec_t gexec32 = nullptr;
   long m_outer = 0, m_fallbacks = 0, m_short = 0;
+  // test hook (MPPD_BATCH_FORCE_FALLBACK=1 at mppd_batch_create): every FP32
+  // inner solve counts as broken down, so each linear solve is redone by the
+  // FP64 CG from du = 0, which exercises the fallback path
+  bool force_fallback = false;
   int spmv_shape = kSpmvDefault;    // tuned SpMV shape (launch_b_spmv)
   int spmv_shape32 = kSpmvDefault;  // tuned inner FP32 SpMV shape (launch_bm_spmv)
   double cg_queued = 0, graph_launches = 0, last_ms = 0;
@@ -269,6 +274,13 @@ int cg_batch_mixed(mppd_batch *B, const std::vector<int> &on, std::vector<int> &
         any = true;
       }
     }
+    if (B->force_fallback)
+      for (int r = 0; r < n; ++r)
+        if (live[r]) {
+          live[r] = 0;
+          broke[r] = 1;
+          any = false;
+        }
     if (!any) break;
     if (outer >= kMaxOuter) {
       for (int r = 0; r < n; ++r) B->m_short += live[r];
@@ -815,6 +827,8 @@ int mppd_batch_create(mppd_batch **out, int nrve, const mppd_batch_rve *rves, co
   mppd_batch *B = new mppd_batch();
   B->n = nrve;
   B->prm = *p;
+  const char *ff = getenv("MPPD_BATCH_FORCE_FALLBACK");
+  B->force_fallback = ff && atoi(ff) != 0;
   const int rc = setup(B, rves);
   if (rc) {
     destroy(B);
