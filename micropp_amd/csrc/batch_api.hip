@@ -71,7 +71,7 @@ struct mppd_batch {
   hipGraph_t graph32 = nullptr;
   hipGraphExec_t gexec32 = nullptr;
   long m_outer = 0, m_fallbacks = 0, m_short = 0;
-  // test hook (MPPD_BATCH_FORCE_FALLBACK=1 at mppd_batch_create): every FP32
+  // test hook (MPPD_FORCE_FP64_FALLBACK=1 at mppd_batch_create): every FP32
   // inner solve counts as broken down, so each linear solve is redone by the
   // FP64 CG from du = 0, which exercises the fallback path
   bool force_fallback = false;
@@ -827,7 +827,7 @@ int mppd_batch_create(mppd_batch **out, int nrve, const mppd_batch_rve *rves, co
   mppd_batch *B = new mppd_batch();
   B->n = nrve;
   B->prm = *p;
-  const char *ff = getenv("MPPD_BATCH_FORCE_FALLBACK");
+  const char *ff = getenv("MPPD_FORCE_FP64_FALLBACK");
   B->force_fallback = ff && atoi(ff) != 0;
   const int rc = setup(B, rves);
   if (rc) {

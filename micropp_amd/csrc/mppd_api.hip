@@ -7,6 +7,7 @@
 // 6-component averages.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -935,8 +936,11 @@ int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6]
       copied = A0;
     }
     MPP_HIP(hipMemsetAsync(s->du, 0, sizeof(double) * g.nrow_pad, s->st));
-    bool broke = false;
-    for (int outer = 0;; ++outer) {
+    // test hook: MPPD_FORCE_FP64_FALLBACK=1 counts every FP32 solve as broken
+    // down, so each linear solve is redone by the FP64 CG (the fallback path)
+    const char *ff = getenv("MPPD_FORCE_FP64_FALLBACK");
+    bool broke = ff && atoi(ff) != 0;
+    for (int outer = 0; !broke; ++outer) {
       MPP_HIP(launch_m_resid_ell(g, K, A->cols, A->vals, s->b, s->du, ml->kinv64, ml->kinv, ml->R, ml->X, ml->P,
                                  ml->part, outer == 0, p->cg_rel_tol, p->cg_abs_tol, inner_tol, max_inner,
                                  ml->state, s->st));

@@ -62,12 +62,12 @@ def test_mixed_batch_spmv_shape_does_not_change_results(mp):
 
 
 def test_mixed_batch_fp64_fallback_equals_fp64_batch(mp, monkeypatch):
-    """The fallback path, forced from the host (MPPD_BATCH_FORCE_FALLBACK=1: every FP32 inner solve counts as
+    """The fallback path, forced from the host (MPPD_FORCE_FP64_FALLBACK=1: every FP32 inner solve counts as
     broken down): each linear solve is redone by the FP64 batch CG from du = 0, so sigma, ctan, cost and flags
     equal the FP64 batch's bit for bit."""
     descs = mixed_batch(mp)
     r64, _ = run_batch(mp, descs, 2, **KW)
-    monkeypatch.setenv("MPPD_BATCH_FORCE_FALLBACK", "1")
+    monkeypatch.setenv("MPPD_FORCE_FP64_FALLBACK", "1")
     rfb, _ = run_batch(mp, descs, 2, mixed_precision=1, **KW)
     for t in range(2):
         for x, y in zip(r64[t], rfb[t]):

@@ -160,6 +160,24 @@ def test_solver_newton_mixed_solves_with_A0(mp):
     assert rel(d64, down / 3) < 1e-6  # the stiffer RVE has the same Poisson ratios: its tangent is 3x
 
 
+def test_newton_mixed_fp64_fallback_equals_fp64(mp, monkeypatch):
+    """The FP64 fallback, forced from the host (MPPD_FORCE_FP64_FALLBACK=1: every FP32 solve counts as broken
+    down): each linear solve is redone by mppd_newton's FP64 CG from du = 0, so the Newton counts and u equal
+    mppd_newton's bit for bit, with one fallback per Newton step."""
+    mats = [mp.material(*s) for s in LAWS["plastic"]]
+    r = mp.Rve(13, mp.MIC3D_RANDOM_SPHERES, (11.0, 6.0, 0.05, 0.2), mats)
+    s = mp.Solver(r)
+    eps = np.array([4e-3, -1e-3, 5e-4, 2e-3, 0.0, -1e-3])
+    u64, u32 = mp.DeviceArray(r.nrow), mp.DeviceArray(r.nrow)
+    kw = dict(nr_max_its=6, nr_rel_tol=1e-8, cg_rel_tol=1e-10, cg_max_its=4000)
+    out64 = s.newton(eps, u64, **kw)
+    monkeypatch.setenv("MPPD_FORCE_FP64_FALLBACK", "1")
+    out32, stats = s.newton_mixed(eps, u32, **kw)
+    print(f"fp64 {out64}, forced fallback {out32}, stats {stats}", flush=True)
+    assert out32 == out64 and stats[2] == out64[0] and stats[0] == 0
+    assert np.array_equal(u32.numpy(), u64.numpy())
+
+
 def test_newton_mixed_rejects_bad_arguments(mp):
     r = mp.Rve(5, mp.MIC_SPHERE, (0.2, 0, 0, 0), [mp.material(0, 1e6, 0.3)] * 3)
     s = mp.Solver(r)
