@@ -178,6 +178,24 @@ def test_newton_mixed_fp64_fallback_equals_fp64(mp, monkeypatch):
     assert np.array_equal(u32.numpy(), u64.numpy())
 
 
+def test_newton_mixed_outer_cap_counts_short_solves(mp):
+    """max_outer = 1 with a 1e-12 CG tolerance: each linear solve stops after one FP32 inner solve (about 1e-5
+    below its start), short of the tolerance, and is counted in stats[3]; the Newton state stays finite and
+    near the FP64 one."""
+    mats = [mp.material(*s) for s in LAWS["plastic"]]
+    r = mp.Rve(13, mp.MIC3D_RANDOM_SPHERES, (11.0, 6.0, 0.05, 0.2), mats)
+    s = mp.Solver(r)
+    eps = np.array([1e-4, -2e-5, 3e-5, 5e-5, 0.0, -2e-5])  # elastic: one Newton step
+    u64, u32 = mp.DeviceArray(r.nrow), mp.DeviceArray(r.nrow)
+    kw = dict(nr_max_its=1, cg_rel_tol=1e-12, cg_max_its=4000)
+    s.newton(eps, u64, **kw)
+    out, stats = s.newton_mixed(eps, u32, max_outer=1, **kw)
+    e = rel(s.ave_stress(u32), s.ave_stress(u64))
+    print(f"out {out}, stats {stats}, sigma rel {e:.2e}", flush=True)
+    assert stats[3] == 1 and stats[0] == 1 and stats[2] == 0
+    assert np.isfinite(u32.numpy()).all() and e < 1e-3
+
+
 def test_newton_mixed_rejects_bad_arguments(mp):
     r = mp.Rve(5, mp.MIC_SPHERE, (0.2, 0, 0, 0), [mp.material(0, 1e6, 0.3)] * 3)
     s = mp.Solver(r)
