@@ -193,7 +193,8 @@ int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *it
  * stats (may be NULL) = {outer steps, iterations, FP64 fallbacks, linear
  * solves stopped by max_outer short of the tolerance}.  A0 (may be NULL) as
  * in mppd_newton: the first p->its_with_A0 Newton steps solve with A0's
- * matrix instead of assembling (micropp<3> with use_A0 and mixed precision);
+ * matrix instead of assembling (micropp<3> with use_A0 and mixed precision,
+ * for GPs that do not run as a batch);
  * A0 must be assembled (mppd_assembly_mat), not uploaded. */
 int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6], const double *vars_old_d,
                       double *u_d, const mppd_newton_params *p, const mppd_mixed_params *mp, int out[3],
