@@ -288,20 +288,25 @@ int cg_batch_mixed(mppd_batch *B, const std::vector<int> &on, std::vector<int> &
     }
     ++B->m_outer;
     // the inner CGs (the outer kernel started those of the other RVEs done),
-    // gchunk iterations per graph launch; each live RVE's ||z|| is checked at
-    // every read for breakdown and stall
-    std::vector<double> pn_mark(n, 0.0);
-    std::vector<int> its_mark(n, 0);
-    for (int r = 0; r < n; ++r) pn_mark[r] = B->cg_h[2 * r].pn;
-    int queued = 0;
+    // in graphs of gchunk iterations, as many between reads as the slowest
+    // live RVE's observed rate needs to reach its inner target (at most
+    // kMaxChunk, which bounds what a stalled RVE runs before it is stopped);
+    // each live RVE's ||z|| is checked at every read for breakdown and stall
+    constexpr int kMaxChunk = 64;
+    std::vector<double> pn_mark(n, 0.0), pn_prev(n, 0.0);
+    std::vector<int> its_mark(n, 0), its_prev(n, 0);
+    for (int r = 0; r < n; ++r) pn_mark[r] = pn_prev[r] = B->cg_h[2 * r].pn;
+    int queued = 0, chunk = 2 * G;
     bool all = false;
     while (!all && queued < max_inner) {
-      MPP_HIP(hipGraphLaunch(B->gexec32, B->st));
-      queued += G;
-      B->cg_queued += G;
-      B->graph_launches += 1;
+      const int reps = std::max(1, (std::min(chunk, max_inner - queued) + G - 1) / G);
+      for (int i = 0; i < reps; ++i) MPP_HIP(hipGraphLaunch(B->gexec32, B->st));
+      queued += reps * G;
+      B->cg_queued += reps * G;
+      B->graph_launches += reps;
       if ((rc = read_cg(B))) return rc;
       all = true;
+      int next = 0;
       for (int r = 0; r < n; ++r) {
         CgState &c = B->cg_h[2 * r + (queued & 1)];
         if (!live[r] || c.done) continue;
@@ -317,10 +322,21 @@ int cg_batch_mixed(mppd_batch *B, const std::vector<int> &on, std::vector<int> &
         if (stop) {
           c.done = 1;
           MPP_HIP(hipMemcpyAsync(B->cg + 2 * r + (queued & 1), &c, sizeof(CgState), hipMemcpyHostToDevice, B->st));
-        } else {
-          all = false;
+          continue;
         }
+        all = false;
+        int need = 2 * chunk;
+        const int dit = c.its - its_prev[r];
+        if (dit > 0 && c.pn > 0 && pn_prev[r] > 0 && c.pn < pn_prev[r]) {
+          const double rate = std::log(c.pn / pn_prev[r]) / dit;
+          const double est = std::log(c.pn0 * c.rel_err / c.pn) / rate;
+          if (std::isfinite(est)) need = (int)std::ceil(est) + 2;
+        }
+        next = std::max(next, need);
+        pn_prev[r] = c.pn;
+        its_prev[r] = c.its;
       }
+      chunk = std::max(G, std::min(kMaxChunk, next));
     }
     if ((rc = sync(B))) return rc;
     for (int r = 0; r < n; ++r) {
