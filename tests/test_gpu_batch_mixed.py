@@ -87,17 +87,19 @@ def test_mixed_batch_stats(mp):
     assert outer > 0 and fallbacks == 0 and short == 0
 
 
-def test_micropp_mixed_instance_runs_as_mixed_batch(mp, monkeypatch):
-    """A micropp<3> instance with mixed_precision runs its FE GPs as a batch with mixed-precision solves;
-    against the same instance on concurrent solvers (MPPD_GP_BATCH=0, the node-block FP32 CG of
-    mppd_newton_mixed): sigma within 1e-6, ctan within 1e-4, the same nonlinear and convergence flags, over
-    three J2 load steps with history allocation."""
+@pytest.mark.parametrize("slots,lin_stress", [(-1, 0), (2, 0), (2, 1)])
+def test_micropp_mixed_instance_runs_as_mixed_batch(mp, monkeypatch, slots, lin_stress):
+    """A micropp<3> instance with mixed_precision runs its FE GPs as a batch with mixed-precision solves (all
+    5 FE GPs in one batch, or groups of 2 with an idle slot in the last); against the same instance on
+    concurrent solvers (MPPD_GP_BATCH=0, the node-block FP32 CG of mppd_newton_mixed): sigma within 1e-6,
+    ctan within 1e-4, the same nonlinear and convergence flags, over three J2 load steps with history
+    allocation."""
     steps = 3
-    bat, seq = (gpb.make(mp, lin_stress=0, mixed_precision=1, **gpb.KW) for _ in range(2))
+    bat, seq = (gpb.make(mp, lin_stress=lin_stress, mixed_precision=1, **gpb.KW) for _ in range(2))
     rp = gpb.ramps(bat.ngp, steps)
     nl = 0
     for t in range(steps):
-        a = gpb.step(bat, rp, t, monkeypatch, -1)
+        a = gpb.step(bat, rp, t, monkeypatch, slots)
         b = gpb.step(seq, rp, t, monkeypatch, 0)
         for g, (ra, rb) in enumerate(zip(a, b)):
             es, ec = rel(ra[0], rb[0]), rel(ra[1], rb[1])
