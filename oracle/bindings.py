@@ -70,12 +70,13 @@ def material(type_, E, nu, Ka=0.0, Sy=0.0, Xt=0.0):
     return m
 
 
-def ensure_built(ref: bool = True) -> None:
-    """Build the oracle (and the reference, when its source tree is here)."""
+def ensure_built(ref: bool = True, force: bool = False) -> None:
+    """Build the oracle (and the reference, when its source tree is here); force: recompile everything."""
     targets = ["oracle"]
     if ref and os.path.isdir(os.path.join(REF_SRC, "src")):
         targets.append("ref")
-    subprocess.run(["make", "-s", "-C", HERE, f"REF={REF_SRC}"] + targets, check=True)
+    subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", HERE, f"REF={REF_SRC}"] + targets,
+                   check=True)
 
 
 def ref_available() -> bool:
