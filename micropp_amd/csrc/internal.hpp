@@ -14,6 +14,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 
 #include "material_base.h"
@@ -82,6 +83,16 @@ inline int dalloc(T **p, size_t n, bool zero = true) {
 
 inline void dfree(void *p) {
   if (p) (void)hipFree(p);
+}
+
+// Test hook: with MPPD_FAIL_ALLOC=1 in the environment, the buffers an entry
+// point makes on its first call (mppd_newton_mixed, mppd_msolver_adjoint) are
+// freed again after they are allocated and the call fails with
+// MPPD_ERR_NOMEM, as an out-of-memory failure would; a test then checks that
+// the next call allocates afresh and gives the normal result.
+inline bool alloc_fault_injected() {
+  const char *v = std::getenv("MPPD_FAIL_ALLOC");
+  return v && std::atoi(v) != 0;
 }
 
 // material_base with the derived constants of the reference's constructors

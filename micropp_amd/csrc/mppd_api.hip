@@ -834,11 +834,27 @@ int mppd_msolver_adjoint(mppd_msolver *m, int n, const double *eps_h, const mppd
   const int K = m->K;
   int rc = MPPD_OK;
   if (!m->F) {
-    if ((rc = dalloc(&m->F, mixed_vector_len(g, K)))) return rc;
-    if ((rc = dalloc(&m->G, 3 * 24 * 6))) return rc;
-    if ((rc = dalloc(&m->apart, mixed_adj_partials(g, K - 6)))) return rc;
-    if ((rc = dalloc(&m->adj_d, 36 * (size_t)(K - 5)))) return rc;
-    MPP_HIP(hipHostMalloc((void **)&m->adj_h, 36 * (K - 5) * sizeof(double), hipHostMallocDefault));
+    // all five buffers or none: m->F, tested above, is set last
+    double *F = nullptr, *G = nullptr, *apart = nullptr, *adj_d = nullptr, *adj_h = nullptr;
+    do {
+      if ((rc = dalloc(&F, mixed_vector_len(g, K)))) break;
+      if ((rc = dalloc(&G, 3 * 24 * 6))) break;
+      if ((rc = dalloc(&apart, mixed_adj_partials(g, K - 6)))) break;
+      if ((rc = dalloc(&adj_d, 36 * (size_t)(K - 5)))) break;
+      const hipError_t e = hipHostMalloc((void **)&adj_h, 36 * (K - 5) * sizeof(double), hipHostMallocDefault);
+      if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc");
+    } while (0);
+    if (!rc && alloc_fault_injected()) rc = fail(MPPD_ERR_NOMEM, "injected allocation failure (MPPD_FAIL_ALLOC)");
+    if (rc) {
+      for (void *q : {(void *)F, (void *)G, (void *)apart, (void *)adj_d}) dfree(q);
+      if (adj_h) (void)hipHostFree(adj_h);
+      return rc;
+    }
+    m->G = G;
+    m->apart = apart;
+    m->adj_d = adj_d;
+    m->adj_h = adj_h;
+    m->F = F;
   }
   std::vector<double> pn, pn0;
   int all_conv = 0;
@@ -897,19 +913,31 @@ int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6]
   if (max_inner < 1 || max_outer < 1) return fail(MPPD_ERR_ARG, "bad mixed parameters");
   int rc = MPPD_OK;
   if (!s->ml) {
+    // built in full before the solver holds it: a failed allocation leaves
+    // s->ml null, so the next call allocates again instead of launching on
+    // a half-built set of buffers
     MixedLin *ml = new MixedLin();
-    s->ml = ml;
     const size_t nv = mixed_vector_len(g, K), NP = nv / (3 * K);
-    if ((rc = dalloc(&ml->A, mixed_matrix_floats(g)))) return rc;
-    for (float **v : {&ml->R, &ml->X, &ml->P, &ml->AP})
-      if ((rc = dalloc(v, nv))) return rc;
-    if ((rc = dalloc(&ml->kinv, 3 * NP))) return rc;
-    if ((rc = dalloc(&ml->kinv64, 3 * NP))) return rc;
-    if ((rc = dalloc(&ml->part, mixed_partials(g, K)))) return rc;
-    char *st = nullptr;
-    if ((rc = dalloc(&st, mixed_state_bytes()))) return rc;
-    ml->state = st;
-    MPP_HIP(hipHostMalloc(&ml->state_h, mixed_state_bytes(), hipHostMallocDefault));
+    do {
+      if ((rc = dalloc(&ml->A, mixed_matrix_floats(g)))) break;
+      for (float **v : {&ml->R, &ml->X, &ml->P, &ml->AP})
+        if ((rc = dalloc(v, nv))) break;
+      if (rc) break;
+      if ((rc = dalloc(&ml->kinv, 3 * NP))) break;
+      if ((rc = dalloc(&ml->kinv64, 3 * NP))) break;
+      if ((rc = dalloc(&ml->part, mixed_partials(g, K)))) break;
+      char *st = nullptr;
+      if ((rc = dalloc(&st, mixed_state_bytes()))) break;
+      ml->state = st;
+      const hipError_t e = hipHostMalloc(&ml->state_h, mixed_state_bytes(), hipHostMallocDefault);
+      if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc");
+    } while (0);
+    if (!rc && alloc_fault_injected()) rc = fail(MPPD_ERR_NOMEM, "injected allocation failure (MPPD_FAIL_ALLOC)");
+    if (rc) {
+      mixed_lin_free(ml);
+      return rc;
+    }
+    s->ml = ml;
   }
   MixedLin *ml = s->ml;
   int its = 0, solver_its = 0, conv = 0, outers = 0, fallbacks = 0, short_solves = 0;

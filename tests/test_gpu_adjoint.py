@@ -118,3 +118,21 @@ def test_adjoint_argument_checks(mp):
     r = mp.Rve(5, mp.MIC_SPHERE, (0.2, 0, 0, 0), [mp.material(*s) for s in SPEC])
     with pytest.raises(mp.MppdError):
         mp.MSolver(r, 8).adjoint(np.zeros((3, 6)))  # 6 + 3 > K = 8
+
+
+def test_adjoint_failed_first_allocation_is_not_kept(mp, monkeypatch):
+    """The adjoint buffers of an msolver are made on the first mppd_msolver_adjoint call.  When that allocation
+    fails (forced from the host with MPPD_FAIL_ALLOC=1), none of them may stay behind: the next call allocates
+    afresh and equals, bit for bit, the adjoint of an msolver that never failed."""
+    r = mp.Rve(9, mp.MIC3D_RANDOM_SPHERES, GEO, [mp.material(*s) for s in SPEC])
+    strains = np.array([[2e-3, -1e-3, 5e-4, 1e-3, 0.0, -5e-4]])
+    good, ms = mp.MSolver(r, 8), mp.MSolver(r, 8)
+    ref = good.adjoint(strains, rel_tol=1e-10)
+    monkeypatch.setenv("MPPD_FAIL_ALLOC", "1")
+    with pytest.raises(mp.MppdError, match="allocation"):
+        ms.adjoint(strains, rel_tol=1e-10)
+    monkeypatch.delenv("MPPD_FAIL_ALLOC")
+    got = ms.adjoint(strains, rel_tol=1e-10)
+    assert got[5] and ref[5]
+    for a, b in zip(got[:3], ref[:3]):
+        assert np.array_equal(a, b)

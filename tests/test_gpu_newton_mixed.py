@@ -212,3 +212,26 @@ def test_newton_mixed_rejects_bad_arguments(mp):
     up.upload_vals(s.ell()[1])
     with pytest.raises(mp.MppdError):  # uploaded values: the mixed solve needs an assembled A0
         s.newton_mixed(np.zeros(6), u, A0=up)
+
+
+def test_newton_mixed_failed_first_allocation_is_not_kept(mp, monkeypatch):
+    """The FP32 buffers of mppd_newton_mixed are made on its first call.  When that allocation fails (forced
+    from the host: MPPD_FAIL_ALLOC=1 frees them again and returns MPPD_ERR_NOMEM, as an out-of-memory failure
+    would), the solver must keep no part of them: the next call allocates afresh and gives the same Newton
+    counts and u, bit for bit, as a solver that never failed."""
+    mats = [mp.material(*s) for s in LAWS["plastic"]]
+    r = mp.Rve(9, mp.MIC3D_RANDOM_SPHERES, (11.0, 6.0, 0.05, 0.2), mats)
+    eps = np.array([4e-3, -1e-3, 5e-4, 2e-3, 0.0, -1e-3])
+    kw = dict(nr_max_its=6, nr_rel_tol=1e-8, cg_rel_tol=1e-10, cg_max_its=4000)
+    good, s = mp.Solver(r), mp.Solver(r)
+    u_good, u = mp.DeviceArray(r.nrow), mp.DeviceArray(r.nrow)
+    out_good, _ = good.newton_mixed(eps, u_good, **kw)
+    monkeypatch.setenv("MPPD_FAIL_ALLOC", "1")
+    with pytest.raises(mp.MppdError, match="allocation"):
+        s.newton_mixed(eps, u, **kw)
+    monkeypatch.delenv("MPPD_FAIL_ALLOC")
+    mp.load().mppd_memset(u.ptr, 0, u.nbytes)
+    out, _ = s.newton_mixed(eps, u, **kw)
+    print(f"after the failed allocation {out}, never failed {out_good}", flush=True)
+    assert out == out_good
+    assert np.array_equal(u.numpy(), u_good.numpy())
