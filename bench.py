@@ -196,13 +196,23 @@ def _batch_run(mp, descs, ramps, params, label):
                 batched_equals_one_by_one=bool(np.array_equal(np.array(sig_s), sig_b[:nseq])))
 
 
-def batch_j2(nrve, steps, mixed=False):
-    """RVEs of 30^3 hexes, a z-fibre of radius U[0.1, 0.3], J2 matrix (E 1e6, nu 0.3, Sy 1e4, Ka 1e5), elastic
-    fibre; cyclic uniaxial + shear strain of amplitude 2e-3."""
-    import micropp_amd as mp
+J2_PARAMS = dict(lin_stress=0, nr_max_its=4, nr_rel_tol=1e-3, cg_rel_tol=1e-8, cg_max_its=2000)
+DAMAGE_PARAMS = dict(lin_stress=0, nr_max_its=10, nr_rel_tol=1e-6, cg_rel_tol=1e-8, cg_max_its=2000)
+# The reference damage law (src/material.cpp:206-269) leaves the linear range when r = sqrt(sigma_lin : eps)
+# passes Xt / sqrt(E), and its hardening law holds q = 0 (D = 1) below r0 = Ey / sqrt(E) with Ey = 1e5 fixed
+# in the law.  Xt = 1e5 = Ey puts the threshold at r0 (the reference's own benchmark-damage choice).  With
+# E = 1e6, r0 = 100 and r ~ 1.2e3 |eps|: a ramp to |eps| = 1e-2 stays linear; one to 0.15 damages after its
+# fourth step.
+DAMAGE_XT = 1e5
+DAMAGE_PEAK = 0.15
 
-    nrve, steps = nrve or 32, steps or 2
-    rng = np.random.default_rng(3003)
+
+def j2_fibre_workload(mp, nrve, steps, seed=3003):
+    """RVEs of 30^3 hexes, a z-fibre of radius U[0.1, 0.3], J2 matrix (E 1e6, nu 0.3, Sy 1e4, Ka 1e5), elastic
+    fibre (E 1e7, nu 0.25), FE_FULL; cyclic uniaxial + shear strain of amplitude 2e-3 (20 steps a cycle).
+    The workload names no Newton tolerance: the reference defaults (4 iterations, 1e-3), CG 1e-8.
+    Returns (RVE descriptors, per-RVE strain lists, batch parameters)."""
+    rng = np.random.default_rng(seed)
     mats = [mp.material(1, 1e6, 0.3, 1e5, 1e4), mp.material(0, 1e7, 0.25), mp.material(0, 1e7, 0.25)]
     descs = [dict(size=31, micro_type=mp.MIC_CILI_FIB_Z, geo=(rng.uniform(0.1, 0.3), 0, 0, 0), materials=mats,
                   coupling=mp.FE_FULL) for _ in range(nrve)]
@@ -211,9 +221,44 @@ def batch_j2(nrve, steps, mixed=False):
         d = np.zeros(6)
         d[0], d[3] = 2e-3, rng.uniform(-2e-3, 2e-3)
         ramps.append([d * np.sin(2 * np.pi * (t + 1) / 20) / np.sin(2 * np.pi / 20) for t in range(steps)])
-    # the workload names no Newton tolerance: the reference defaults (4 iterations, 1e-3)
-    params = dict(lin_stress=0, nr_max_its=4, nr_rel_tol=1e-3, cg_rel_tol=1e-8, cg_max_its=2000,
-                  mixed_precision=int(mixed))
+    return descs, ramps, dict(J2_PARAMS)
+
+
+def damage_layer_workload(mp, nrve, steps, seed=2002, peak=DAMAGE_PEAK, ramp_steps=10):
+    """RVEs of n in {20, 30, 40, 50} hexes (seeded mix), a y-layer of width U[0.1, 0.3], damage matrix (E 1e6,
+    nu 0.3, Xt 1e5), elastic fibre (E 1e7, nu 0.25), FE_FULL; each RVE a seeded strain direction ramped
+    linearly to |eps| = peak over ramp_steps steps (the first `steps` of them); Newton tolerance 1e-6, CG 1e-8.
+    Returns (RVE descriptors, per-RVE strain lists, batch parameters)."""
+    rng = np.random.default_rng(seed)
+    mats = [mp.material(2, 1e6, 0.3, 0, 0, DAMAGE_XT), mp.material(0, 1e7, 0.25), mp.material(0, 1e7, 0.25)]
+    descs = [dict(size=int(rng.choice([20, 30, 40, 50])) + 1, micro_type=mp.MIC_LAYER_Y,
+                  geo=(rng.uniform(0.1, 0.3), 0, 0, 0), materials=mats, coupling=mp.FE_FULL) for _ in range(nrve)]
+    ramps = []
+    for _ in range(nrve):
+        d = rng.normal(size=6)
+        d *= peak / np.linalg.norm(d)
+        ramps.append([d * (t + 1) / ramp_steps for t in range(steps)])
+    return descs, ramps, dict(DAMAGE_PARAMS)
+
+
+def sphere_rves(nrve, seed=4004):
+    """Geometry of the adjoint workload: 40^3 hexes, 1-10 seeded spheres with radii U[0.05, 0.2]
+    (MIC3D_RANDOM_SPHERES geo_params), and two macro strains U[-5e-3, 5e-3] per RVE."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nrve):
+        geo = (float(rng.integers(1, 1 << 30)), float(rng.integers(1, 11)), 0.05, 0.2)
+        out.append((geo, rng.uniform(-5e-3, 5e-3, (2, 6))))
+    return out
+
+
+def batch_j2(nrve, steps, mixed=False):
+    """The batched J2 workload (j2_fibre_workload): batched vs one-by-one, or mixed vs FP64 batch."""
+    import micropp_amd as mp
+
+    nrve, steps = nrve or 32, steps or 2
+    descs, ramps, params = j2_fibre_workload(mp, nrve, steps)
+    params["mixed_precision"] = int(mixed)
     out = {"metric": "batched J2 RVEs (cylindrical fibre), RVE solves/s", "workload":
            f"{nrve} RVEs of 30^3 hexes, z-fibre radius U[0.1,0.3], J2 matrix, {steps} cyclic load steps"}
     out.update(_batch_run(mp, descs, ramps, params, "batch_j2"))
@@ -221,25 +266,15 @@ def batch_j2(nrve, steps, mixed=False):
 
 
 def batch_damage(nrve, steps, mixed=False):
-    """RVEs of n in {20, 30, 40, 50} hexes (seeded mix), a y-layer of width U[0.1, 0.3], damage matrix (E 1e6,
-    nu 0.3, Xt = 1e3 sqrt(E)), elastic fibre; a seeded strain ramp to |eps| = 1e-2 over 10 steps."""
+    """The batched damage workload (damage_layer_workload): batched vs one-by-one, or mixed vs FP64 batch."""
     import micropp_amd as mp
 
     nrve, steps = nrve or 16, steps or 2
-    rng = np.random.default_rng(2002)
-    mats = [mp.material(2, 1e6, 0.3, 0, 0, 1e3 * np.sqrt(1e6)), mp.material(0, 1e7, 0.25),
-            mp.material(0, 1e7, 0.25)]
-    descs = [dict(size=int(rng.choice([20, 30, 40, 50])) + 1, micro_type=mp.MIC_LAYER_Y,
-                  geo=(rng.uniform(0.1, 0.3), 0, 0, 0), materials=mats, coupling=mp.FE_FULL) for _ in range(nrve)]
-    ramps = []
-    for _ in range(nrve):
-        d = rng.normal(size=6)
-        d *= 1e-2 / np.linalg.norm(d)
-        ramps.append([d * (t + 1) / 10 for t in range(steps)])
-    params = dict(lin_stress=0, nr_max_its=10, nr_rel_tol=1e-6, cg_rel_tol=1e-8, cg_max_its=2000,
-                  mixed_precision=int(mixed))
+    descs, ramps, params = damage_layer_workload(mp, nrve, steps)
+    params["mixed_precision"] = int(mixed)
     out = {"metric": "batched damage RVEs (fibre layer, uneven sizes), RVE solves/s", "workload":
-           f"{nrve} RVEs of 20..50^3 hexes, y-layer width U[0.1,0.3], damage matrix, {steps} ramp steps"}
+           f"{nrve} RVEs of 20..50^3 hexes, y-layer width U[0.1,0.3], damage matrix, {steps} of 10 ramp steps "
+           f"to |eps| = {DAMAGE_PEAK}"}
     out.update(_batch_run(mp, descs, ramps, params, "batch_damage"))
     return out
 
@@ -251,13 +286,10 @@ def adjoint_workload(nrve, _steps):
     import micropp_amd as mp
 
     nrve = nrve or 4
-    rng = np.random.default_rng(4004)
     spec = [(0, 1e6, 0.3), (0, 1e7, 0.25), (0, 1e7, 0.25)]
     t_adj = t_fwd = 0.0
     err_ctan = err_fd = 0.0
-    for i in range(nrve):
-        geo = (float(rng.integers(1, 1 << 30)), float(rng.integers(1, 11)), 0.05, 0.2)
-        strains = rng.uniform(-5e-3, 5e-3, (2, 6))
+    for i, (geo, strains) in enumerate(sphere_rves(nrve)):
         r = mp.Rve(41, mp.MIC3D_RANDOM_SPHERES, geo, [mp.material(*s) for s in spec])
         ms = mp.MSolver(r, 8)
         t0 = time.perf_counter()

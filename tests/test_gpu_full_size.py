@@ -1,0 +1,201 @@
+"""GPU: the batched workloads at their real RVE sizes.
+
+- Batched damage: RVEs of 20, 30, 40 and 50 hexes per side in one batch,
+  fibre layers, the reference damage law on a strain ramp that crosses its
+  threshold (|eps| 0.04, 0.08, 0.12).
+- Batched J2: RVEs of 30^3 hexes with a z-fibre, J2 matrix, 3 cyclic steps.
+- Adjoint: elastic RVEs of 40^3 hexes with 1-10 random spheres.
+
+Each is checked three ways:
+- against the oracle (the CPU restatement of the reference) on the one or
+  two smallest RVEs, FE_ONE_WAY (sigma by calc_ave_stress): sigma within
+  1e-6 relative, the same nonlinear / convergence flags, the cost within the
+  +-1 CG iteration per linear solve that a different reduction order allows;
+- the rest by bit equality with micropp<3> instances run one by one;
+- the batch with mixed-precision solves against the FP64 batch.
+The adjoint: ctan against the forward six-solve ctan and the oracle's
+calc_ctan_lin_fe_models, d sigma / d (E, nu) against central differences.
+
+The oracle runs in worker threads (its ctypes calls release the GIL) while
+the GPU batches run.
+"""
+import concurrent.futures as cf
+
+import numpy as np
+import pytest
+
+import bench
+from oracle import bindings as ob
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 3
+
+
+@pytest.fixture(scope="module")
+def mp():
+    import micropp_amd
+
+    micropp_amd.load()
+    if not micropp_amd.gpu_available():
+        pytest.skip("no HIP device")
+    return micropp_amd
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def orc_materials(d):
+    return [ob.material(m.type, m.E, m.nu, m.Ka, m.Sy, m.Xt) for m in d["materials"]]
+
+
+def oracle_run(oracle, d, ramp, params):
+    """One RVE as an FE_ONE_WAY oracle instance over the ramp: [(sigma, (nonlinear, cost, converged))]."""
+    p, k = ob.make_params(d["size"], d["micro_type"], d["geo"], orc_materials(d), ngp=1,
+                          coupling=[ob.FE_ONE_WAY], **params)
+    m = oracle.micro(p, k)
+    out = []
+    for e in ramp:
+        m.set_strain(0, e)
+        m.homogenize()
+        out.append((m.get_stress(0), (m.is_non_linear(0), m.get_cost(0), int(m.has_converged(0)))))
+        m.update_vars()
+    m.close()
+    return out
+
+
+def batch_run(mp, descs, ramps, params):
+    b = mp.Batch(descs, **params)
+    rec = []
+    for t in range(len(ramps[0])):
+        b.set_strains(np.array([r[t] for r in ramps]))
+        b.homogenize()
+        rec.append((b.stress(), b.ctan(), b.info()))
+        b.update_vars()
+    out = (rec, b.stats(), b.mixed_stats() if params.get("mixed_precision") else None)
+    del b
+    return out
+
+
+def single_run(mp, d, ramp, params):
+    m = mp.Micropp3(d["size"], d["micro_type"], d["geo"], d["materials"], ngp=1, coupling=[d["coupling"]],
+                    **params)
+    rec = []
+    for e in ramp:
+        m.set_strain(0, e)
+        m.homogenize()
+        rec.append((m.get_stress(0), m.get_ctan(0), (m.is_non_linear(0), m.get_cost(0), int(m.has_converged(0)),
+                                                      int(m.has_subiterated(0)))))
+        m.update_vars()
+    m.close()
+    return rec
+
+
+def check_workload(mp, oracle, descs, ramps, params, orc_ix, single_ix, ts_mixed, label):
+    """The three checks of the module docstring; returns the FP64 batch records."""
+    for i in orc_ix:
+        descs[i] = dict(descs[i], coupling=mp.FE_ONE_WAY)
+    print(f"{label}: {len(descs)} RVEs, sizes {[d['size'] for d in descs]}; oracle RVEs {orc_ix}, "
+          f"one-by-one RVEs {single_ix}", flush=True)
+    with cf.ThreadPoolExecutor(max_workers=len(orc_ix)) as ex:
+        futs = {i: ex.submit(oracle_run, oracle, descs[i], ramps[i], params) for i in orc_ix}
+        rec, stats, _ = batch_run(mp, descs, ramps, params)
+        print(f"{label}: FP64 batch done, stats {stats.tolist()}", flush=True)
+        rec32, stats32, mstats = batch_run(mp, descs, ramps, dict(params, mixed_precision=1))
+        print(f"{label}: mixed batch done, stats {stats32.tolist()}, mixed {mstats}", flush=True)
+        singles = {i: single_run(mp, descs[i], ramps[i], params) for i in single_ix}
+        print(f"{label}: one-by-one instances done", flush=True)
+        orc = {i: f.result() for i, f in futs.items()}
+    nl_total = 0
+    for t in range(len(ramps[0])):
+        sig, ctan, info = rec[t]
+        sig32, ctan32, info32 = rec32[t]
+        nl_total += int(info[:, 0].sum())
+        for i, o in orc.items():
+            s1, (nl1, cost1, conv1) = o[t]
+            nlin = params["nr_max_its"]
+            print(f"{label} step {t} rve {i} vs oracle: sigma {rel(sig[i], s1):.2e}, info {tuple(info[i][:3])} vs "
+                  f"{(nl1, cost1, conv1)}", flush=True)
+            assert rel(sig[i], s1) < 1e-6
+            assert (int(info[i][0]), int(info[i][2])) == (nl1, conv1)
+            assert abs(int(info[i][1]) - cost1) <= nlin
+        for i, srec in singles.items():
+            s1, c1, f1 = srec[t]
+            assert np.array_equal(sig[i], s1), f"step {t} rve {i}"
+            assert np.array_equal(ctan[i], c1), f"step {t} rve {i}"
+            assert tuple(int(v) for v in info[i]) == tuple(int(v) for v in f1), f"step {t} rve {i}"
+        es = [rel(sig32[i], sig[i]) for i in range(len(descs))]
+        ec = [rel(ctan32[i], ctan[i]) for i in range(len(descs))]
+        print(f"{label} step {t}: mixed vs FP64 batch, sigma max {max(es):.2e}, ctan max {max(ec):.2e}; "
+              f"nonlinear RVEs {int(info[:, 0].sum())}, converged {int(info[:, 2].sum())}", flush=True)
+        assert max(es) < ts_mixed
+        assert np.array_equal(info32[:, 0], info[:, 0]) and np.array_equal(info32[:, 2], info[:, 2])
+    return rec, nl_total
+
+
+def test_batched_damage_real_sizes(mp, oracle):
+    """16 RVEs of 20..50^3 (seeded mix, all four sizes present), damage matrix with Xt = 1e5 (threshold at the
+    hardening law's r0 = Ey / sqrt(E) = 100), ramp |eps| 0.04, 0.08, 0.12: the matrix damages in the last two
+    steps.  Oracle on the first 20^3 and the first 30^3 RVE; one-by-one on a 50^3, a 40^3 and a 20^3 RVE."""
+    descs, ramps, params = bench.damage_layer_workload(mp, 16, STEPS, peak=0.12, ramp_steps=STEPS)
+    sizes = [d["size"] for d in descs]
+    assert set(sizes) == {21, 31, 41, 51}
+    orc_ix = [sizes.index(21), sizes.index(31)]
+    single_ix = [sizes.index(51), sizes.index(41), [i for i, s in enumerate(sizes) if s == 21][1]]
+    rec, nl_total = check_workload(mp, oracle, descs, ramps, params, orc_ix, single_ix, 1e-6, "damage")
+    assert int(rec[-1][2][:, 0].sum()) >= len(descs) // 2, "the ramp must damage most RVEs"
+
+
+def test_batched_j2_real_size(mp, oracle):
+    """16 RVEs of 30^3 with a z-fibre, J2 matrix, 3 cyclic steps (the matrix yields from the first step).
+    Oracle on two RVEs; one-by-one on three FE_FULL RVEs."""
+    descs, ramps, params = bench.j2_fibre_workload(mp, 16, STEPS)
+    rec, nl_total = check_workload(mp, oracle, descs, ramps, params, [0, 1], [2, 3, 4], 1e-6, "j2")
+    assert int(rec[0][2][:, 0].sum()) == len(descs), "every RVE yields in the first step"
+
+
+def test_adjoint_real_size(mp, oracle):
+    """Elastic RVEs of 40^3 with 1-10 random spheres: the adjoint ctan against the forward six-solve ctan (all
+    four RVEs) and the oracle's calc_ctan_lin_fe_models (one RVE); d sigma / d (E, nu) against central
+    differences with a 1e-6 relative step (two RVEs)."""
+    spec = [(0, 1e6, 0.3), (0, 1e7, 0.25), (0, 1e7, 0.25)]
+    rves = bench.sphere_rves(4)
+    with cf.ThreadPoolExecutor(max_workers=1) as ex:
+        geo0 = rves[0][0]
+        p, k = ob.make_params(41, ob.MIC3D_RANDOM_SPHERES, geo0, [ob.material(*s) for s in spec], ngp=1,
+                              coupling=[ob.FE_FULL], cg_rel_tol=1e-10, cg_max_its=4000)
+        fut = ex.submit(lambda: oracle.micro(p, k).ctan_lin())
+        ctans = []
+        for i, (geo, strains) in enumerate(rves):
+            r = mp.Rve(41, mp.MIC3D_RANDOM_SPHERES, geo, [mp.material(*s) for s in spec])
+            ctan, sig, dmat, stats, resid, ok = mp.MSolver(r, 8).adjoint(strains, rel_tol=1e-10)
+            assert ok
+            sg, ctan_f, _, _, ok2 = mp.homogenize_mixed(mp.MSolver(r, 12), strains, rel_tol=1e-10)
+            assert ok2
+            print(f"rve {i} ({int(geo[1])} spheres): adjoint vs forward ctan {rel(ctan, ctan_f):.2e}, sigma "
+                  f"{rel(sig, sg):.2e}; stats {stats}", flush=True)
+            assert rel(ctan, ctan_f) < 1e-6 and rel(sig, sg) < 1e-6
+            assert rel(ctan, ctan.T) < 1e-7
+            ctans.append(ctan)
+            if i < 2:
+                fd = np.zeros((2, 6, 2, 2))
+                for ph in range(2):
+                    for w in (1, 2):
+                        h = spec[ph][w] * 1e-6
+                        res = []
+                        for sgn in (1, -1):
+                            sp = [list(s) for s in spec]
+                            sp[ph][w] += sgn * h
+                            rr = mp.Rve(41, mp.MIC3D_RANDOM_SPHERES, geo, [mp.material(*s) for s in sp])
+                            sfd, _, _, okf = mp.MSolver(rr, 8).solve(strains, rel_tol=1e-13, max_outer=30)
+                            assert okf
+                            res.append(sfd)
+                        fd[:, :, ph, w - 1] = (res[0] - res[1]) / (2 * h)
+                print(f"rve {i}: d sigma / d(E, nu) vs central differences {rel(dmat[:, :, :2, :], fd):.2e}",
+                      flush=True)
+                assert rel(dmat[:, :, :2, :], fd) < 1e-5
+        octan = fut.result()
+    print(f"rve 0: adjoint ctan vs oracle {rel(ctans[0], octan):.2e}", flush=True)
+    assert rel(ctans[0], octan) < 1e-6
