@@ -88,11 +88,6 @@ typedef struct {
   int its_with_A0 = 1;
   bool lin_stress = true;
   bool write_log = false;
-  // additive (no reference counterpart): Newton linear solves as FP64 defect
-  // correction around FP32 Jacobi-PCG (mppd_newton_mixed); the residual,
-  // return mapping, damage state, u and the averages stay FP64 (use_A0 is
-  // not applied to these solves)
-  bool mixed_precision = false;
 
   void print() {
     cout << "ngp  : " << ngp << endl;
@@ -110,9 +105,23 @@ typedef struct {
     cout << "its_with_A0 : " << its_with_A0 << endl;
     cout << "lin_stress : " << lin_stress << endl;
     cout << "write_log : " << write_log << endl;
-    cout << "mixed_precision : " << mixed_precision << endl;
   }
 } micropp_params_t;
+
+/* Additive (no reference counterpart): choices of this implementation that
+ * micropp_params_t, kept to the reference's layout, does not carry.  An
+ * instance made with the reference's constructor takes them from the
+ * environment (MICROPP_MIXED_PRECISION=1). */
+typedef struct {
+  /* every Newton linear solve as FP64 defect correction around FP32
+   * Jacobi-PCG, to the reference CG's metric evaluated in FP64; the residual,
+   * return mapping, damage state, u and the averages stay FP64.  An RVE whose
+   * phases are all linear elastic solves the constructor's six ctan problems
+   * in one multi-right-hand-side pass and the FE Gauss points of homogenize
+   * in passes of up to 16; other RVEs solve their GPs as a batch (use_A0 is
+   * applied in both). */
+  bool mixed_precision = false;
+} micropp_options_t;
 
 /* reference include/types.hpp:85-99, plus the seeded sphere list */
 enum {
@@ -161,6 +170,8 @@ class micropp {
  public:
   micropp() = delete;
   micropp(const micropp_params_t &params);
+  /* additive: the reference constructor with this implementation's options */
+  micropp(const micropp_params_t &params, const micropp_options_t &options);
   ~micropp();
   micropp(const micropp &) = delete;
   micropp &operator=(const micropp &) = delete;

@@ -74,7 +74,7 @@ struct micropp3_params {
   int its_with_A0;
   int lin_stress;
   int write_log;
-  int mixed_precision; /* additive: micropp_params_t::mixed_precision */
+  int mixed_precision; /* micropp_options_t::mixed_precision */
 };
 void micropp3_params_default(struct micropp3_params *p);
 /* self->ptr is NULL on failure (message on stderr) */

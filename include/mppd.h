@@ -162,6 +162,23 @@ int mppd_msolver_solve(mppd_msolver *m, int n, const double *eps_h, const mppd_m
                        int stats[2], double *rel_h);
 /* displacement field k (nrow doubles, reference DOF numbering) of the last solve */
 int mppd_msolver_get_u(mppd_msolver *m, int k, double *u_h);
+/* newton_raphson (src/solve.cpp:28-82) of n <= K Gauss points of the
+ * msolver's RVE at once (the reference runs one per GP, src/homogenize.cpp:
+ * 111-186, OpenMP over the GPs).  GP k starts from u_in[k] (nrow doubles on
+ * the device, reference DOF numbering; u_in or u_in[k] NULL: zero) with the
+ * boundary values of eps_h[k * 6 ..]; the Newton loop of each GP is the
+ * reference's (stop at nr_max_its unconverged, or when ||b||_2 < max(
+ * nr_max_tol, nr_rel_tol ||b_0||), tested at the top of an iteration), each
+ * correction solved to cg_rel_tol of the current ||D^-1 b|| by FP64 defect
+ * correction around the FP32 inner CG of all live GPs together.  Outputs:
+ * out[k * 3 ..] = {its, cost, converged} with cost = the inner iterations of
+ * the passes the GP took part in; sig_h (may be NULL) n x 6 FP64 volume
+ * averages (calc_ave_stress); u_out[k] (may be NULL) the final displacement
+ * (may equal u_in[k]); stats (may be NULL) {outer steps, inner iterations}.
+ * mp: inner_tol, max_inner_its, max_outer (NULL: 1e-5, cg_max_its, 10). */
+int mppd_msolver_newton(mppd_msolver *m, int n, const double *eps_h, const double *const *u_in,
+                        double *const *u_out, const mppd_newton_params *p, const mppd_mixed_params *mp,
+                        double *sig_h, int *out, int stats[2]);
 /* Adjoint derivatives of the volume-averaged stress (no reference
  * counterpart: the reference differentiates by six perturbed solves,
  * calc_ctan_lin_fe_models src/micropp.cpp:255-284 and the FE_FULL loop

@@ -146,6 +146,7 @@ SIGNATURES = {
     "mppd_msolver_get_u": (_I, [_VP, _I, _PD]),
     "mppd_msolver_adjoint": (_I, [_VP, _I, _PD, C.c_void_p, _PD, _PD, _PD, _PI, _PD]),
     "mppd_msolver_profile": (_I, [_VP, _I, _PD, _PD]),
+    "mppd_msolver_newton": (_I, [_VP, _I, _PD, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _PD, _PI, _PI]),
     "mppd_newton_mixed": (_I, [_VP, _VP, _PD, _VP, _VP, C.POINTER(NewtonParams), _VP, _PI, _PI]),
     "mppd_batch_params_default": (None, [_VP]),
     "mppd_batch_create": (_I, [C.POINTER(C.c_void_p), _I, _VP, _VP]),
@@ -596,6 +597,26 @@ class MSolver:
         out = np.zeros(self.rve.nrow)
         _chk(load().mppd_msolver_get_u(self.h, k, _dp(out)), "mppd_msolver_get_u")
         return out
+
+    def newton(self, strains, u_in=None, u_out=None, nr_max_its=4, nr_max_tol=1e-10, nr_rel_tol=1e-3,
+               cg_max_its=1000, cg_abs_tol=1e-50, cg_rel_tol=1e-5, mixed=None):
+        """newton_raphson of n <= K Gauss points at once (mppd_msolver_newton): u_in / u_out are lists of
+        DeviceArray or None (zero start / no output).  mixed: (inner_tol, max_inner_its, max_outer) or None.
+        Returns (sigma n x 6, out n x 3 = its, cost, converged, stats (outer steps, inner iterations))."""
+        e = np.ascontiguousarray(strains, dtype=np.float64).reshape(-1, 6)
+        n = e.shape[0]
+        p = NewtonParams(nr_max_its, nr_max_tol, nr_rel_tol, cg_max_its, cg_abs_tol, cg_rel_tol, 1)
+        ins = (C.c_void_p * n)(*[None if a is None else a.ptr for a in (u_in or [None] * n)])
+        outs = (C.c_void_p * n)(*[None if a is None else a.ptr for a in (u_out or [None] * n)])
+        mpar = None if mixed is None else MixedParams(cg_rel_tol, cg_abs_tol, *mixed)
+        sig = np.zeros((n, 6))
+        out = np.zeros((n, 3), dtype=np.int32)
+        stats = np.zeros(2, dtype=np.int32)
+        _chk(load().mppd_msolver_newton(self.h, n, _dp(e), C.cast(ins, C.c_void_p), C.cast(outs, C.c_void_p),
+                                        C.cast(C.byref(p), C.c_void_p),
+                                        None if mpar is None else C.cast(C.byref(mpar), C.c_void_p), _dp(sig),
+                                        _ip(out), _ip(stats)), "mppd_msolver_newton")
+        return sig, out, (int(stats[0]), int(stats[1]))
 
     def profile(self, iters):
         a, t = C.c_double(), C.c_double()

@@ -162,7 +162,11 @@ hipError_t launch_m_setbc(const Grid &g, int K, const double *eps, double *U, hi
 // F: adjoint sources (FP64, the layout of U) or null
 hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke, const double *U, const double *F,
                           const float *kinv, float *R, float *X, float *P, double *part, int first, double rel,
-                          double abs_tol, double tau_min, int max_inner, void *state, hipStream_t st);
+                          double abs_tol, double tau_min, int max_inner, void *state, hipStream_t st,
+                          const int *active = nullptr);
+// field f of U from / into a displacement vector in the reference numbering (node * 3 + d)
+hipError_t launch_m_load_u(const Grid &g, int K, int f, const double *u, double *U, hipStream_t st);
+hipError_t launch_m_store_u(const Grid &g, int K, int f, const double *U, double *u, hipStream_t st);
 // adjoint: G (3 x 24 x 6 doubles) and the sources F of fields 0 .. nadj-1
 hipError_t launch_m_adj_src(const Grid &g, int K, int nadj, const double *bmat, const material_base *mats_d,
                             const int *et, double *G, double *F, hipStream_t st);
@@ -214,5 +218,7 @@ void mixed_state_view(const void *state_h, int parity, int *its, int *all_done);
 // residual norm, *target the norm at which it stops
 int mixed_state_field(const void *state_h, int parity, int k, double *pn, double *target);
 void mixed_outer_view(const void *state_h, int K, int *all_conv, double *pn, double *pn0);
+// per field: converged flag of the last outer test, ||r||_2 now and at the last `first` test
+void mixed_outer_norms(const void *state_h, int K, int *conv, double *rn, double *rn0);
 
 }  // namespace mpp

@@ -93,8 +93,13 @@ struct micropp<tdim>::impl {
   std::vector<worker> wk;  // wk[0] = {sol, u, vars_aux}; more for concurrent GPs
   bool wk_tried = false;
   std::mutex err_mu;
+  // mixed precision on an all-elastic RVE: the multi-right-hand-side solver
+  // with K fields, for the constructor's ctan and the FE GPs of homogenize
+  mppd_msolver *ms = nullptr;
+  int ms_K = 0;
 
   ~impl() {
+    mppd_msolver_destroy(ms);
     mppd_batch_destroy(batch);
     for (size_t k = 1; k < wk.size(); ++k) {
       mppd_free(wk[k].u);
@@ -134,6 +139,16 @@ struct micropp<tdim>::impl {
 
   // calc_ctan_lin_fe_models (src/micropp.cpp:255-284)
   void ctan_lin_fe_models() {
+    if (ms) {  // the six solves from u = 0 as one multi-right-hand-side Newton pass
+      double eps[36] = {0}, sig[36];
+      int out[18];
+      for (int i = 0; i < 6; ++i) eps[i * 6 + i] = D_EPS_CTAN_AVE;
+      check(mppd_msolver_newton(ms, 6, eps, nullptr, nullptr, &np, nullptr, sig, out, nullptr),
+            "calc_ctan_lin_fe_models (multi-RHS)");
+      for (int i = 0; i < 6; ++i)
+        for (int v = 0; v < 6; ++v) ctan_lin_fe[v * 6 + i] = sig[i * 6 + v] / D_EPS_CTAN_AVE;
+      return;
+    }
     worker &w = wk[0];
     for (int i = 0; i < 6; ++i) {
       check(mppd_solver_fill(w.sol, w.u, 0, sizeof(double) * nndim), "device fill");
@@ -200,6 +215,55 @@ struct micropp<tdim>::impl {
         q.cost += n.solver_its;  // the reference adds the sigma solve's count again
         ave_stress(w, w.u, q.vars_n, sig1);
         for (int v = 0; v < 6; ++v) q.ctan[v * 6 + i] = (sig1[v] - sig0[v]) / D_EPS_CTAN_AVE;
+      }
+    }
+  }
+
+  // The FE GPs of an all-elastic RVE with mixed precision run through the
+  // multi-right-hand-side solver, ms_K GPs per pass, in place of the
+  // reference's OpenMP loop over GPs (src/homogenize.cpp:72): each GP's
+  // Newton starts from its u_n with its strain's boundary values (as
+  // homogenize_fe_one_way does, :111-186) and writes u_k; sigma is ctan.eps
+  // (lin_stress) or the pass's FP64 volume average.  The GPs of a pass share
+  // one FP32 matrix stream per inner iteration.  An elastic RVE never
+  // allocates history, so calc_vars_new and the FE_FULL perturbations have
+  // nothing to do.  A GP whose Newton does not converge, with subiterations
+  // on, is solved again on the one-GP path, which subiterates.
+  void fe_multi() {
+    const int nfe = (int)fe_ids.size(), K = ms_K;
+    std::vector<double> eps(6 * (size_t)K), sig(6 * (size_t)K);
+    std::vector<const double *> uin(K);
+    std::vector<double *> uout(K);
+    std::vector<int> out(3 * (size_t)K);
+    for (int c0 = 0; c0 < nfe; c0 += K) {
+      const int m = std::min(K, nfe - c0);
+      try {
+        for (int r = 0; r < m; ++r) {
+          const gp_state &q = gp[fe_ids[c0 + r]];
+          memcpy(&eps[6 * r], q.strain, sizeof(q.strain));
+          uin[r] = q.u_n;
+          uout[r] = q.u_k;
+        }
+        check(mppd_msolver_newton(ms, m, eps.data(), uin.data(), uout.data(), &np, nullptr, sig.data(), out.data(),
+                                  nullptr),
+              "newton_raphson (multi-RHS)");
+        for (int r = 0; r < m; ++r) {
+          gp_state &q = gp[fe_ids[c0 + r]];
+          q.cost = out[3 * r + 1];
+          q.converged = out[3 * r + 2] != 0;
+          q.subiterated = false;
+          if (!q.converged && subiterations) {
+            fe(wk[0], q, q.coupling == FE_FULL);
+            continue;
+          }
+          if (lin_stress)
+            linear(q);
+          else
+            memcpy(q.stress, &sig[6 * r], sizeof(q.stress));
+        }
+      } catch (const std::exception &ex) {
+        for (int r = 0; r < m; ++r) gp[fe_ids[c0 + r]].converged = false;
+        cerr << ex.what() << " (gps " << fe_ids[c0] << " .. " << fe_ids[c0 + m - 1] << ")" << endl;
       }
     }
   }
@@ -409,9 +473,24 @@ void micropp<tdim>::impl::chamis(double c[36]) const {
   for (int i = 3; i < 6; ++i) c[i * 6 + i] = G12;
 }
 
+namespace {
+
+// options of an instance made with the reference's constructor
+micropp_options_t env_options() {
+  micropp_options_t o;
+  const char *v = getenv("MICROPP_MIXED_PRECISION");
+  o.mixed_precision = v && strtol(v, nullptr, 10) != 0;
+  return o;
+}
+
+}  // namespace
+
+template <int tdim>
+micropp<tdim>::micropp(const micropp_params_t &params) : micropp(params, env_options()) {}
+
 // Constructor (src/micropp.cpp:29-183)
 template <int tdim>
-micropp<tdim>::micropp(const micropp_params_t &params) : p_(new impl()) {
+micropp<tdim>::micropp(const micropp_params_t &params, const micropp_options_t &options) : p_(new impl()) {
   impl &s = *p_;
   s.ngp = params.ngp;
   s.nx = params.size[0];
@@ -438,7 +517,7 @@ micropp<tdim>::micropp(const micropp_params_t &params) : p_(new impl()) {
   s.use_A0 = params.use_A0;
   s.lin_stress = params.lin_stress;
   s.write_log_flag = params.write_log;
-  s.mixed = params.mixed_precision;
+  s.mixed = options.mixed_precision;
   try {
     if (s.ngp < 0) throw std::invalid_argument("micropp_amd: ngp must be >= 0");
     if (s.micro_type < 0 || s.micro_type > MIC3D_RANDOM_SPHERES)
@@ -467,6 +546,18 @@ micropp<tdim>::micropp(const micropp_params_t &params) : p_(new impl()) {
     if (s.use_A0) {
       check(mppd_solver_create(&s.A0, s.rve), "A0 set-up");
       check(mppd_assembly_mat(s.A0, s.u, nullptr), "A0 assembly");
+    }
+    // mixed precision on an all-elastic RVE: the multi-RHS solver, with
+    // fields for the six ctan problems or up to 16 FE GPs per pass
+    // (MPPD_MSOLVER_ROUTE=0 in the environment keeps the per-GP paths)
+    bool elastic = true;
+    for (int i = 0; i < MAX_MATERIALS; ++i) elastic &= s.mats[i].type == MATERIAL_ELASTIC;
+    const char *route = getenv("MPPD_MSOLVER_ROUTE");
+    if (s.mixed && elastic && (!route || strtol(route, nullptr, 10) != 0) &&
+        (n_fe > 0 && (s.calc_ctan_lin_flag || !s.fe_ids.empty()))) {
+      const int want = std::max<int>(s.calc_ctan_lin_flag ? 6 : 1, std::min<int>(16, (int)s.fe_ids.size()));
+      s.ms_K = (want + 3) / 4 * 4;
+      check(mppd_msolver_create(&s.ms, s.rve, s.ms_K), "multi-RHS solver set-up");
     }
     if (s.calc_ctan_lin_flag && n_fe > 0) s.ctan_lin_fe_models();
     for (auto &q : s.gp) {
@@ -522,12 +613,15 @@ void micropp<tdim>::homogenize() {
   impl &s = *p_;
   bool batched = false;
   try {
-    batched = s.batch_ready();
+    batched = !s.ms && s.batch_ready();
   } catch (const std::exception &ex) {
     cerr << ex.what() << "; GPs run one at a time" << endl;
   }
   bool parallel = false;
-  if (batched) {
+  const bool multi = s.ms && !s.fe_ids.empty();
+  if (multi) {
+    s.fe_multi();
+  } else if (batched) {
     s.fe_batched();
   } else if (s.fe_ids.size() > 1) {
     try {
@@ -542,7 +636,7 @@ void micropp<tdim>::homogenize() {
     try {
       if (q.coupling == FE_LINEAR || q.coupling == MIX_RULE_CHAMIS)
         impl::linear(q);
-      else if (batched || parallel)
+      else if (multi || batched || parallel)
         continue;
       else if (q.coupling == FE_ONE_WAY)
         s.fe(s.wk[0], q, false);

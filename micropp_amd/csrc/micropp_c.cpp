@@ -108,7 +108,7 @@ void micropp3_params_default(struct micropp3_params *p) {
   p->its_with_A0 = d.its_with_A0;
   p->lin_stress = d.lin_stress;
   p->write_log = d.write_log;
-  p->mixed_precision = d.mixed_precision;
+  p->mixed_precision = micropp_options_t{}.mixed_precision;
 }
 
 void micropp3_new_ext(struct micropp3 *self, const struct micropp3_params *p) {
@@ -133,10 +133,11 @@ void micropp3_new_ext(struct micropp3 *self, const struct micropp3_params *p) {
   params.its_with_A0 = p->its_with_A0;
   params.lin_stress = p->lin_stress != 0;
   params.write_log = p->write_log != 0;
-  params.mixed_precision = p->mixed_precision != 0;
+  micropp_options_t options;
+  options.mixed_precision = p->mixed_precision != 0;
   self->ptr = nullptr;
   try {
-    self->ptr = new micropp<3>(params);
+    self->ptr = new micropp<3>(params, options);
   } catch (const std::exception &e) {
     fprintf(stderr, "micropp3_new_ext: %s\n", e.what());
   }

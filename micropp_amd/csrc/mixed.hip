@@ -59,8 +59,11 @@ struct MState {
   int its, max_its, all_done, pad;
 };
 
+// outer (defect-correction) test per field: pn = ||kinv r|| (the reference
+// CG's metric), rn = ||r||_2 over the interior DOFs (assembly_rhs's norm:
+// its Dirichlet rows are zero), and their values at the last `first` call
 struct MOuter {
-  double pn[kMaxRhs], pn0[kMaxRhs], target[kMaxRhs];
+  double pn[kMaxRhs], pn0[kMaxRhs], target[kMaxRhs], rn[kMaxRhs], rn0[kMaxRhs];
   int conv[kMaxRhs];
   int all_conv, pad[3];
 };
@@ -230,6 +233,24 @@ __global__ void __launch_bounds__(256) k_m_setbc(Grid g, int NP, const double *e
 }
 
 // ---------------------------------------------------------------------------
+// One field of U from / into a displacement vector in the reference DOF
+// numbering (u[node * 3 + d]); padding nodes of U are zero after a load.
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ void __launch_bounds__(256) k_m_load_u(Grid g, int NP, int f, const double *u, double *U) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= NP) return;
+  for (int d = 0; d < 3; ++d) U[(size_t)(d * K + f) * NP + n] = (n < g.nn) ? u[(size_t)n * 3 + d] : 0.0;
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) k_m_store_u(Grid g, int NP, int f, const double *U, double *u) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nn) return;
+  for (int d = 0; d < 3; ++d) u[(size_t)n * 3 + d] = U[(size_t)(d * K + f) * NP + n];
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint sources.  For sigma_v = (1/V) sum_e sum_gp wg [C B u_e]_v the
 // derivative with respect to the displacements is assembled from, per phase
 // p, G[p][q][v] = wg sum_m C_p[v][m] sum_gp B_gp[m][q] (q < 24 element DOFs).
@@ -296,7 +317,8 @@ __global__ void __launch_bounds__(256) k_m_adj_src(Grid g, int NP, int nadj, con
 // phase table, summed per stencil neighbour; F = 0 unless adjoint sources are
 // given), rounded into the FP32 CG start: R = r, X = 0, P = kinv R.
 // Partials: part[q * gridDim.x + block], q < K: ||kinv r||^2 (FP64) of field
-// q; q >= K: (R, P) of field q - K.
+// q; K <= q < 2K: (R, P) of field q - K; 2K <= q: ||r||^2 (FP64) of field
+// q - 2K.
 // ---------------------------------------------------------------------------
 template <int K>
 __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const int *et, const double *ke,
@@ -304,14 +326,14 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const 
                                                  float4 *X, float4 *P, double *part) {
   constexpr int KG = K / 4;
   __shared__ double ke_s[3 * 576];
-  __shared__ double lds[4 * 2 * K];
+  __shared__ double lds[4 * 3 * K];
   for (int t = threadIdx.x; t < 3 * 576; t += blockDim.x) ke_s[t] = ke[t];
   __syncthreads();
   const int lb = xcd_block(per);
   const int n = lb * 256 + threadIdx.x;
-  double zz[K], rz[K];
+  double zz[K], rz[K], rr[K];
 #pragma unroll
-  for (int r = 0; r < K; ++r) zz[r] = rz[r] = 0.0;
+  for (int r = 0; r < K; ++r) zz[r] = rz[r] = rr[r] = 0.0;
   if (n < NP) {
     const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
     const bool interior = n < g.nn && !node_on_boundary(g, i, j, k);
@@ -373,58 +395,64 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const 
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const float kv = kinv[(size_t)d * NP + n];
-        float rr[4], pp[4];
+        float rf[4], pp[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const size_t idx = (size_t)(d * K + kg + q) * NP + n;
           const double r64 = (F ? F[idx] : 0.0) - acc[d][kg + q];
-          rr[q] = (float)r64;
-          pp[q] = kv * rr[q];
+          rf[q] = (float)r64;
+          pp[q] = kv * rf[q];
           const double z64 = (double)kv * r64;
           zz[kg + q] += z64 * z64;
-          rz[kg + q] += (double)rr[q] * (double)pp[q];
+          rz[kg + q] += (double)rf[q] * (double)pp[q];
+          rr[kg + q] += r64 * r64;
         }
         const size_t o = (size_t)(d * KG + kg / 4) * NP + n;
-        R[o] = make_float4(rr[0], rr[1], rr[2], rr[3]);
+        R[o] = make_float4(rf[0], rf[1], rf[2], rf[3]);
         X[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         P[o] = make_float4(pp[0], pp[1], pp[2], pp[3]);
       }
     }
   }
-  double v[2 * K];
+  double v[3 * K];
 #pragma unroll
   for (int r = 0; r < K; ++r) {
     v[r] = zz[r];
     v[K + r] = rz[r];
+    v[2 * K + r] = rr[r];
   }
-  block_sum<2 * K, 256>(v, lds);
+  block_sum<3 * K, 256>(v, lds);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < 2 * K; ++q) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
+    for (int q = 0; q < 3 * K; ++q) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
 }
 
-// one block: outer convergence test and the inner CG start state
+// one block: outer convergence test and the inner CG start state.  active
+// (may be null): fields whose entry is 0 count as converged (they take no
+// part in the solve).
 template <int K>
 __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const double *part, int first, double rel,
-                                                         double abs_tol, double tau_min, int max_inner, MOuter *o,
-                                                         MState *s0) {
-  __shared__ double t[2 * kMaxRhs];
+                                                         double abs_tol, double tau_min, int max_inner,
+                                                         const int *active, MOuter *o, MState *s0) {
+  __shared__ double t[3 * kMaxRhs];
   __shared__ int conv_s[kMaxRhs];
-  const double s = rows_sum<2 * K>(part, nparts, nparts);
-  if (threadIdx.x < 2 * K) t[threadIdx.x] = s;
+  const double s = rows_sum<3 * K>(part, nparts, nparts);
+  if (threadIdx.x < 3 * K) t[threadIdx.x] = s;
   __syncthreads();
   // one thread per field; every read of the old state comes before any write
   const int r = threadIdx.x;
   if (r < K) {
-    const double pn = sqrt(t[r]);
+    const double pn = sqrt(t[r]), rn = sqrt(t[2 * K + r]);
     const double target = first ? fmax(abs_tol, rel * pn) : o->target[r];
-    const int conv = pn < target || pn == 0.0;
+    const int conv = pn < target || pn == 0.0 || (active && !active[r]);
     conv_s[r] = conv;
     if (first) {
       o->pn0[r] = pn;
+      o->rn0[r] = rn;
       o->target[r] = target;
     }
     o->pn[r] = pn;
+    o->rn[r] = rn;
     o->conv[r] = conv;
     s0->rz[r] = t[K + r];
     s0->pn[r] = pn;
@@ -1141,16 +1169,16 @@ __global__ void __launch_bounds__(256) k_m_from_ell(Grid g, int NP, const double
 // boundary), one thread per row; the FP32 CG start R = r, X = 0, P = kinv R
 // in field 0, zero in the other fields; partials as k_m_resid
 // (part[q * gridDim.x + block]: q < K the FP64 ||kinv r||^2 of field q,
-// q >= K the (R, P) of field q - K)
+// K <= q < 2K the (R, P) of field q - K, 2K <= q the FP64 ||r||^2)
 template <int K>
 __global__ void __launch_bounds__(256) k_m_resid_ell(Grid g, int NP, const int *cols, const double *vals,
                                                      const double *b, const double *du, const double *kinv64,
                                                      const float *kinv, float4 *R, float4 *X, float4 *P,
                                                      double *part) {
   constexpr int KG = K / 4;
-  __shared__ double lds[4 * 2 * K];
+  __shared__ double lds[4 * 3 * K];
   const int row = blockIdx.x * 256 + threadIdx.x;
-  double zz = 0.0, rz = 0.0;
+  double zz = 0.0, rz = 0.0, rr = 0.0;
   if (row < g.nrow) {
     const int n = row / 3, d = row % 3;
     const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
@@ -1162,6 +1190,7 @@ __global__ void __launch_bounds__(256) k_m_resid_ell(Grid g, int NP, const int *
       const double z64 = kinv64[(size_t)d * NP + n] * r64;
       zz = z64 * z64;
       rz = (double)r32 * (double)p32;
+      rr = r64 * r64;
     }
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
@@ -1172,15 +1201,16 @@ __global__ void __launch_bounds__(256) k_m_resid_ell(Grid g, int NP, const int *
       P[o] = gq ? zero : make_float4(p32, 0.0f, 0.0f, 0.0f);
     }
   }
-  double v[2 * K];
+  double v[3 * K];
 #pragma unroll
-  for (int q = 0; q < 2 * K; ++q) v[q] = 0.0;
+  for (int q = 0; q < 3 * K; ++q) v[q] = 0.0;
   v[0] = zz;
   v[K] = rz;
-  block_sum<2 * K, 256>(v, lds);
+  v[2 * K] = rr;
+  block_sum<3 * K, 256>(v, lds);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < 2 * K; ++q) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
+    for (int q = 0; q < 3 * K; ++q) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
 }
 
 // du += X (field 0), FP64, one thread per row
@@ -1220,7 +1250,7 @@ size_t mixed_matrix_floats(const Grid &g) { return (size_t)mshape(g).NP * kNodeQ
 size_t mixed_vector_len(const Grid &g, int K) { return (size_t)mshape(g).NP * 3 * K; }
 size_t mixed_partials(const Grid &g, int K) {
   const MLaunch s = mshape(g);
-  const size_t resid = std::max((size_t)s.nb, (size_t)g.nrow_pad / 256) * 2 * K;  // k_m_resid / k_m_resid_ell
+  const size_t resid = std::max((size_t)s.nb, (size_t)g.nrow_pad / 256) * 3 * K;  // k_m_resid / k_m_resid_ell
   const size_t ave = (size_t)nblk(g.nelem, 256) * 6 * K;
   const size_t iter = (size_t)K * s.nsl + 2 * (size_t)3 * K * (s.NP / 256);
   return std::max(std::max(resid, ave), iter);
@@ -1263,10 +1293,23 @@ hipError_t launch_m_setbc(const Grid &g, int K, const double *eps, double *U, hi
   return hipGetLastError();
 }
 
+hipError_t launch_m_load_u(const Grid &g, int K, int f, const double *u, double *U, hipStream_t st) {
+  const MLaunch s = mshape(g);
+  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_load_u<KK>, dim3(s.NP / 256), dim3(256), 0, st, g, s.NP, f, u, U));
+  return hipGetLastError();
+}
+
+hipError_t launch_m_store_u(const Grid &g, int K, int f, const double *U, double *u, hipStream_t st) {
+  const MLaunch s = mshape(g);
+  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_store_u<KK>, dim3(nblk(g.nn, 256)), dim3(256), 0, st, g, s.NP, f, U, u));
+  return hipGetLastError();
+}
+
 // residual + outer test + inner start state
 hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke, const double *U, const double *F,
                           const float *kinv, float *R, float *X, float *P, double *part, int first, double rel,
-                          double abs_tol, double tau_min, int max_inner, void *state, hipStream_t st) {
+                          double abs_tol, double tau_min, int max_inner, void *state, hipStream_t st,
+                          const int *active) {
   const MLaunch s = mshape(g);
   MState *ms = static_cast<MState *>(state);
   MOuter *mo = reinterpret_cast<MOuter *>(ms + 2);
@@ -1275,7 +1318,7 @@ hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke,
                        reinterpret_cast<float4 *>(R), reinterpret_cast<float4 *>(X), reinterpret_cast<float4 *>(P),
                        part);
     hipLaunchKernelGGL(k_m_outer<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nb, part, first, rel, abs_tol, tau_min,
-                       max_inner, mo, ms);
+                       max_inner, active, mo, ms);
   });
   return hipGetLastError();
 }
@@ -1334,7 +1377,7 @@ hipError_t launch_m_resid_ell(const Grid &g, int K, const int *cols, const doubl
                        reinterpret_cast<float4 *>(R), reinterpret_cast<float4 *>(X), reinterpret_cast<float4 *>(P),
                        part);
     hipLaunchKernelGGL(k_m_outer<KK>, dim3(1), dim3(kRedThreads), 0, st, nbr, part, first, rel, abs_tol, tau_min,
-                       max_inner, mo, ms);
+                       max_inner, nullptr, mo, ms);
   });
   return hipGetLastError();
 }
@@ -1408,6 +1451,14 @@ void mixed_outer_view(const void *state_h, int K, int *all_conv, double *pn, dou
   for (int r = 0; r < K; ++r) {
     pn[r] = mo->pn[r];
     pn0[r] = mo->pn0[r];
+  }
+}
+void mixed_outer_norms(const void *state_h, int K, int *conv, double *rn, double *rn0) {
+  const MOuter *mo = reinterpret_cast<const MOuter *>(static_cast<const MState *>(state_h) + 2);
+  for (int r = 0; r < K; ++r) {
+    conv[r] = mo->conv[r];
+    rn[r] = mo->rn[r];
+    rn0[r] = mo->rn0[r];
   }
 }
 

@@ -617,6 +617,8 @@ int mppd_cg_profile(mppd_solver *s, const double *b_d, int iters, double *spmv_m
 // ---------------------------------------------------------------------------
 // mixed-precision multi-RHS solver
 // ---------------------------------------------------------------------------
+constexpr int kMaxRhsFields = 16;  // largest K of an msolver
+
 struct mppd_msolver {
   mppd_rve *r = nullptr;
   int K = 0;
@@ -628,6 +630,7 @@ struct mppd_msolver {
   double *U = nullptr, *eps_d = nullptr, *part = nullptr, *sig_d = nullptr;
   void *state = nullptr, *state_h = nullptr;
   double *eps_h = nullptr, *sig_h = nullptr;  // pinned staging
+  int *act_d = nullptr, *act_h = nullptr;     // fields taking part in a Newton iteration (device, pinned)
   // adjoint (allocated on first use): sources, per-phase source table,
   // partials, results (device and pinned)
   double *F = nullptr, *G = nullptr, *apart = nullptr, *adj_d = nullptr, *adj_h = nullptr;
@@ -665,7 +668,9 @@ int mppd_msolver_create(mppd_msolver **out, mppd_rve *r, int nrhs) {
     if ((rc = dalloc(&m->sig_d, 16 * 6))) break;
     if ((rc = dalloc(&m->part, mixed_partials(g, nrhs)))) break;
     if ((rc = dalloc((char **)&m->state, mixed_state_bytes()))) break;
+    if ((rc = dalloc(&m->act_d, kMaxRhsFields))) break;
     e = hipHostMalloc(&m->state_h, mixed_state_bytes(), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&m->act_h, kMaxRhsFields * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&m->eps_h, 16 * 6 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&m->sig_h, 16 * 6 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemcpy(m->mats_d, r->mats, 3 * sizeof(material_base), hipMemcpyHostToDevice);
@@ -689,9 +694,9 @@ int mppd_msolver_destroy(mppd_msolver *m) {
   if (m->st) (void)hipStreamSynchronize(m->st);
   for (void *p : {(void *)m->ke, (void *)m->mats_d, (void *)m->A, (void *)m->kinv, (void *)m->X, (void *)m->R,
                   (void *)m->P, (void *)m->AP, (void *)m->U, (void *)m->eps_d, (void *)m->sig_d, (void *)m->part,
-                  m->state, (void *)m->F, (void *)m->G, (void *)m->apart, (void *)m->adj_d})
+                  m->state, (void *)m->act_d, (void *)m->F, (void *)m->G, (void *)m->apart, (void *)m->adj_d})
     dfree(p);
-  for (void *p : {m->state_h, (void *)m->eps_h, (void *)m->sig_h, (void *)m->adj_h})
+  for (void *p : {m->state_h, (void *)m->eps_h, (void *)m->sig_h, (void *)m->act_h, (void *)m->adj_h})
     if (p) (void)hipHostFree(p);
   if (m->st) (void)hipStreamDestroy(m->st);
   delete m;
@@ -881,6 +886,110 @@ int mppd_msolver_get_u(mppd_msolver *m, int k, double *u_h) {
   const size_t np = U.size() / (3 * (size_t)m->K);
   for (int n = 0; n < g.nn; ++n)
     for (int d = 0; d < 3; ++d) u_h[(size_t)n * 3 + d] = U[((size_t)d * m->K + k) * np + n];
+  return MPPD_OK;
+}
+
+// newton_raphson (src/solve.cpp:28-82) of n Gauss points of the msolver's
+// (all-elastic) RVE at once.  Field k starts from u_in[k] (zero when the
+// pointer is null) with the boundary values of eps_h[k]; the Newton loop of
+// every field follows the reference: at the top of an iteration it stops at
+// nr_max_its (unconverged) or when ||b||_2 (the FP64 interior residual, the
+// norm of assembly_rhs) is below max(nr_max_tol, nr_rel_tol ||b_0||); else it
+// solves for the correction to the CG's relative tolerance of the current
+// ||D^-1 b|| -- FP64 defect correction around the FP32 inner CG of all live
+// fields together -- and adds it.  The averages are FP64 (calc_ave_stress).
+int mppd_msolver_newton(mppd_msolver *m, int n, const double *eps_h, const double *const *u_in,
+                        double *const *u_out, const mppd_newton_params *p, const mppd_mixed_params *mp,
+                        double *sig_h, int *out, int stats[2]) {
+  if (!m || !eps_h || !p || !out || n < 1 || n > m->K) return fail(MPPD_ERR_ARG, "bad msolver_newton arguments");
+  const Grid &g = m->r->g;
+  const int K = m->K;
+  const double inner_tol = mp ? mp->inner_tol : 1e-5;
+  const int max_inner = mp ? mp->max_inner_its : p->cg_max_its;
+  const int max_outer = mp ? mp->max_outer : 10;
+  if (max_inner < 1 || max_outer < 1) return fail(MPPD_ERR_ARG, "bad mixed parameters");
+  for (int i = 0; i < K * 6; ++i) m->eps_h[i] = (i / 6 < n) ? eps_h[i] : 0.0;
+  MPP_HIP(hipMemcpyAsync(m->eps_d, m->eps_h, K * 6 * sizeof(double), hipMemcpyHostToDevice, m->st));
+  MPP_HIP(hipMemsetAsync(m->U, 0, mixed_vector_len(g, K) * sizeof(double), m->st));
+  for (int k = 0; k < n; ++k)
+    if (u_in && u_in[k]) MPP_HIP(launch_m_load_u(g, K, k, u_in[k], m->U, m->st));
+  MPP_HIP(launch_m_setbc(g, K, m->eps_d, m->U, m->st));
+  std::vector<int> its(K, 0), run(K, 0), conv(K, 0), cost(K, 0), cflag(K, 0);
+  std::vector<double> rn(K), rn0(K), norm0(K), pn(K), pn0(K);
+  for (int k = 0; k < n; ++k) run[k] = 1;
+  int outer_total = 0, inner_total = 0;
+  // the residual of the current U for the fields in run, with the outer test
+  // and the inner start state (first: targets from this residual)
+  auto resid = [&](int first) -> int {
+    MPP_HIP(hipStreamSynchronize(m->st));  // act_h may still feed an earlier copy
+    for (int k = 0; k < K; ++k) m->act_h[k] = run[k];
+    MPP_HIP(hipMemcpyAsync(m->act_d, m->act_h, K * sizeof(int), hipMemcpyHostToDevice, m->st));
+    MPP_HIP(launch_m_resid(g, K, m->r->et, m->ke, m->U, nullptr, m->kinv, m->R, m->X, m->P, m->part, first,
+                           p->cg_rel_tol, p->cg_abs_tol, inner_tol, max_inner, m->state, m->st, m->act_d));
+    MPP_HIP(hipMemcpyAsync(m->state_h, m->state, mixed_state_bytes(), hipMemcpyDeviceToHost, m->st));
+    MPP_HIP(hipStreamSynchronize(m->st));
+    mixed_outer_norms(m->state_h, K, cflag.data(), rn.data(), rn0.data());
+    return MPPD_OK;
+  };
+  int rc = resid(1);
+  if (rc) return rc;
+  for (int k = 0; k < K; ++k) norm0[k] = rn[k];
+  bool fresh = true;  // the last residual set the targets of a Newton iteration
+  for (;;) {
+    // Newton tests of the fields still running (at the top of an iteration)
+    bool any = false, changed = false;
+    for (int k = 0; k < n; ++k) {
+      if (!run[k]) continue;
+      if (its[k] >= p->nr_max_its) {
+        run[k] = 0;
+      } else if (rn[k] < p->nr_max_tol || rn[k] < norm0[k] * p->nr_rel_tol) {
+        conv[k] = 1;
+        run[k] = 0;
+      } else {
+        any = true;
+        continue;
+      }
+      changed = true;
+    }
+    if (!any) break;
+    // one Newton iteration of those fields: its linear solve targets the
+    // current residual (the last one computed is that of the current U)
+    if ((changed || !fresh) && (rc = resid(1))) return rc;
+    fresh = false;
+    for (int outer = 0;; ++outer) {
+      if (outer > 0 && (rc = resid(0))) return rc;
+      int all_conv = 0;
+      mixed_outer_view(m->state_h, K, &all_conv, pn.data(), pn0.data());
+      if (all_conv || outer >= max_outer) break;
+      int inner = 0;
+      if ((rc = mixed_inner(g, K, m->A, m->kinv, m->R, m->X, m->P, m->AP, m->part, m->state, m->state_h, max_inner,
+                            m->st, &inner)))
+        return rc;
+      inner_total += inner;
+      ++outer_total;
+      for (int k = 0; k < n; ++k) cost[k] += run[k] ? inner : 0;
+      MPP_HIP(launch_m_accum(g, K, m->X, m->U, m->st));
+    }
+    // every exit of the loop above follows a residual of the current U, whose
+    // norms the next tests read
+    for (int k = 0; k < n; ++k) its[k] += run[k];
+  }
+  if (sig_h) {
+    if ((rc = msolver_averages(m))) return rc;
+    for (int i = 0; i < n * 6; ++i) sig_h[i] = m->sig_h[i];
+  }
+  for (int k = 0; k < n; ++k)
+    if (u_out && u_out[k]) MPP_HIP(launch_m_store_u(g, K, k, m->U, u_out[k], m->st));
+  MPP_HIP(hipStreamSynchronize(m->st));
+  for (int k = 0; k < n; ++k) {
+    out[3 * k] = its[k];
+    out[3 * k + 1] = cost[k];
+    out[3 * k + 2] = conv[k];
+  }
+  if (stats) {
+    stats[0] = outer_total;
+    stats[1] = inner_total;
+  }
   return MPPD_OK;
 }
 
