@@ -110,6 +110,21 @@ int mppd_ell_mvp(mppd_solver *s, const double *x_d, double *y_d);
  * at exit, as in the reference.  Convergence is decided on the device. */
 int mppd_cg(mppd_solver *s, const mppd_solver *A0, const double *b_d, double *x_d, double min_err, double rel_err,
             int max_its, int *its, double *err);
+/* mppd_cg without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same Jacobi-PCG as
+ * mppd_cg, with the same arguments and results, but its working set (x, r,
+ * z, p, Ap, D^-1, the partial sums and the CG state) lives in a fixed device
+ * array of the library (about 19 MB on each GPU the library runs on, held
+ * until the process ends) instead of the solver's buffers.  RVEs of more
+ * than MPPD_NOALLOC_MAX_ROWS DOF rows (after padding to a multiple of 256;
+ * 51^3 nodes fit) fail with MPPD_ERR_ARG.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (several host threads, or several
+ * solvers' streams) race and must be serialised by the caller.  Runs on the
+ * solver's stream and is complete on return. */
+#define MPPD_NOALLOC_MAX_ROWS 398080
+int mppd_cg_noalloc(mppd_solver *s, const mppd_solver *A0, const double *b_d, double *x_d, double min_err,
+                    double rel_err, int max_its, int *its, double *err);
 /* calc_ave_stress (src/average.cpp:57-82) */
 int mppd_ave_stress(mppd_solver *s, const double *u_d, const double *vars_d, double sig[6]);
 /* calc_vars_new (src/update.cpp:32-56); *nonlinear = OR of the evolute flags */

@@ -135,6 +135,7 @@ SIGNATURES = {
     "mppd_assembly_mat": (_I, [_VP, _VP, _VP]),
     "mppd_ell_mvp": (_I, [_VP, _VP, _VP]),
     "mppd_cg": (_I, [_VP, _VP, _VP, _VP, _D, _D, _I, _PI, _PD]),
+    "mppd_cg_noalloc": (_I, [_VP, _VP, _VP, _VP, _D, _D, _I, _PI, _PD]),
     "mppd_ave_stress": (_I, [_VP, _VP, _VP, _PD]),
     "mppd_vars_new": (_I, [_VP, _VP, _VP, _VP, _PI]),
     "mppd_elem_fields": (_I, [_VP, _VP, _VP, _PD, _PD]),

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -447,30 +448,37 @@ int mppd_ell_mvp(mppd_solver *s, const double *x_d, double *y_d) {
   return MPPD_OK;
 }
 
+// The working set of one Jacobi-PCG: vectors of nrow_pad doubles, the
+// partial table (3 * nrow_pad / 256 doubles), the ping-pong state pair on the
+// device and one host (pinned) state for the reads between chunks.
+struct CgWork {
+  double *x, *r, *z, *p, *Ap, *kinv, *part;
+  CgState *state, *state_h;
+};
+
 // CG on the device; the host queues iterations in chunks sized from the
 // observed convergence rate and reads the 64-byte state between chunks.
 // After `queued` iterations the current state is state[queued & 1].
-static int cg_run(mppd_solver *s, const mppd_solver *A, const double *b_d, double min_err, double rel_err,
-                  int max_its, int *its_out, double *err_out) {
-  const Grid &g = s->r->g;
-  MPP_HIP(launch_cg_init(g, A->vals, b_d, s->x, s->rr, s->z, s->p, s->kinv, s->partials, min_err, rel_err, max_its,
-                         s->cg, s->st));
+static int cg_run_on(const Grid &g, const mppd_solver *A, const CgWork &w, hipStream_t st, const double *b_d,
+                     double min_err, double rel_err, int max_its, int *its_out, double *err_out) {
+  MPP_HIP(launch_cg_init(g, A->vals, b_d, w.x, w.r, w.z, w.p, w.kinv, w.part, min_err, rel_err, max_its, w.state,
+                         st));
   int queued = 0;
-  MPP_HIP(hipMemcpyAsync(s->cg_h, s->cg, sizeof(CgState), hipMemcpyDeviceToHost, s->st));
-  MPP_HIP(hipStreamSynchronize(s->st));
-  double pn_prev = s->cg_h->pn;
+  MPP_HIP(hipMemcpyAsync(w.state_h, w.state, sizeof(CgState), hipMemcpyDeviceToHost, st));
+  MPP_HIP(hipStreamSynchronize(st));
+  double pn_prev = w.state_h->pn;
   int its_prev = 0;
   int chunk = 32;
-  while (!s->cg_h->done) {
+  while (!w.state_h->done) {
     const int n = std::min(chunk, max_its - queued);
     if (n <= 0) break;
     for (int i = 0; i < n; ++i)
-      MPP_HIP(launch_cg_iter(g, A->vals_stencil ? nullptr : A->cols, A->vals, s->x, s->rr, s->z, s->p, s->Ap, s->kinv, s->partials, s->cg,
-                             queued + i, s->st));
+      MPP_HIP(launch_cg_iter(g, A->vals_stencil ? nullptr : A->cols, A->vals, w.x, w.r, w.z, w.p, w.Ap, w.kinv,
+                             w.part, w.state, queued + i, st));
     queued += n;
-    MPP_HIP(hipMemcpyAsync(s->cg_h, s->cg + (queued & 1), sizeof(CgState), hipMemcpyDeviceToHost, s->st));
-    MPP_HIP(hipStreamSynchronize(s->st));
-    const CgState &c = *s->cg_h;
+    MPP_HIP(hipMemcpyAsync(w.state_h, w.state + (queued & 1), sizeof(CgState), hipMemcpyDeviceToHost, st));
+    MPP_HIP(hipStreamSynchronize(st));
+    const CgState &c = *w.state_h;
     if (c.done) break;
     // next chunk: iterations still needed at the observed rate, within [4, 256]
     const double target = std::max(c.min_err, c.pn0 * c.rel_err);
@@ -485,9 +493,15 @@ static int cg_run(mppd_solver *s, const mppd_solver *A, const double *b_d, doubl
     pn_prev = c.pn;
     its_prev = c.its;
   }
-  if (its_out) *its_out = s->cg_h->its;
-  if (err_out) *err_out = s->cg_h->rz;
+  if (its_out) *its_out = w.state_h->its;
+  if (err_out) *err_out = w.state_h->rz;
   return MPPD_OK;
+}
+
+static int cg_run(mppd_solver *s, const mppd_solver *A, const double *b_d, double min_err, double rel_err,
+                  int max_its, int *its_out, double *err_out) {
+  const CgWork w{s->x, s->rr, s->z, s->p, s->Ap, s->kinv, s->partials, s->cg, s->cg_h};
+  return cg_run_on(s->r->g, A, w, s->st, b_d, min_err, rel_err, max_its, its_out, err_out);
 }
 
 // A0's matrix is read with s's row layout: it must be a solver of an RVE grid
@@ -507,6 +521,72 @@ int mppd_cg(mppd_solver *s, const mppd_solver *A0, const double *b_d, double *x_
   int rc = cg_run(s, A, b_d, min_err, rel_err, max_its, its, err);
   if (rc) return rc;
   MPP_HIP(launch_copy(s->r->g.nrow, s->x, x_d, s->st));
+  MPP_HIP(hipStreamSynchronize(s->st));
+  return MPPD_OK;
+}
+
+// mppd_cg_noalloc: the working set of the CG (x, r, z, p, Ap, D^-1, the
+// partial table and the state pair) lives in `scratch`, a fixed device array
+// of this file, instead of the solver's own buffers.  Layout (doubles): the
+// state pair in the first 256 bytes, then 6 vectors of kNoAllocRows, then the
+// partial table.  x is a scratch vector too: the CG kernels write all
+// nrow_pad rows, the caller's x_d holds nrow.  On each device the array's address comes from
+// hipGetSymbolAddress (once per device, with that device current); in host
+// code the array's name is not a device address and is never passed on.
+constexpr size_t kNoAllocRows = MPPD_NOALLOC_MAX_ROWS;
+static_assert(kNoAllocRows % 256 == 0, "whole row blocks");
+constexpr size_t kNoAllocHead = 256 / sizeof(double);
+constexpr size_t kNoAllocParts = 3 * (kNoAllocRows / 256);
+constexpr size_t kNoAllocDoubles = kNoAllocHead + 6 * kNoAllocRows + kNoAllocParts;
+static_assert(2 * sizeof(CgState) <= 256, "the state pair fits the head");
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static double *g_noalloc_ptr[kNoAllocMaxDevices];
+
+// the current device's copy of `scratch` (looked up on the first call there)
+static int noalloc_scratch(double **out) {
+  int dev = 0;
+  MPP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(MPPD_ERR_ARG, "mppd_cg_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(MPPD_ERR_HIP, "hipGetSymbolAddress(scratch) gave no address");
+    g_noalloc_ptr[dev] = static_cast<double *>(ptr);
+  }
+  *out = g_noalloc_ptr[dev];
+  return MPPD_OK;
+}
+
+int mppd_cg_noalloc(mppd_solver *s, const mppd_solver *A0, const double *b_d, double *x_d, double min_err,
+                    double rel_err, int max_its, int *its, double *err) {
+  if (!s || !b_d || !x_d) return fail(MPPD_ERR_ARG, "null argument");
+  if (int rc = check_A0(s, A0)) return rc;
+  const Grid &g = s->r->g;
+  if ((size_t)g.nrow_pad > kNoAllocRows)
+    return fail(MPPD_ERR_ARG, "mppd_cg_noalloc: " + std::to_string(g.nrow) + " rows exceed the scratch maximum of " +
+                                  std::to_string(kNoAllocRows) + " (MPPD_NOALLOC_MAX_ROWS)");
+  double *ptr = nullptr;
+  if (int rc = noalloc_scratch(&ptr)) return rc;
+  // every entry the CG reads is written by its initialisation (all nrow_pad
+  // rows of the vectors, every partial, state[0]) or by an earlier kernel of
+  // the same solve, so nothing left by the previous call needs clearing
+  double *v = ptr + kNoAllocHead;
+  const CgWork w{v,
+                 v + kNoAllocRows,
+                 v + 2 * kNoAllocRows,
+                 v + 3 * kNoAllocRows,
+                 v + 4 * kNoAllocRows,
+                 v + 5 * kNoAllocRows,
+                 v + 6 * kNoAllocRows,
+                 reinterpret_cast<CgState *>(ptr),
+                 s->cg_h};
+  if (int rc = cg_run_on(g, A0 ? A0 : s, w, s->st, b_d, min_err, rel_err, max_its, its, err)) return rc;
+  MPP_HIP(launch_copy(g.nrow, w.x, x_d, s->st));
   MPP_HIP(hipStreamSynchronize(s->st));
   return MPPD_OK;
 }
