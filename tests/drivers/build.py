@@ -1,6 +1,7 @@
 """Build the C and Fortran drivers of the micropp3_* interface into tests/drivers/bin/.
 
-c_driver.c (gcc) and f_driver.f90 (ROCm flang, when present) link
+c_driver.c (gcc) and f_driver.f90 (ROCm flang, when present, with the
+drop-in modules micropp_amd/fortran/libmaterial.f95 and libmicropp.f95) link
 micropp_amd/libmicropp_amd.so with an rpath relative to the binary, so the
 built tree can move.  Run after the library is built:
 
@@ -16,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 BIN = os.path.join(HERE, "bin")
 LIBDIR = os.path.join(ROOT, "micropp_amd")
+FORTRAN = os.path.join(LIBDIR, "fortran")
 RPATH = "-Wl,-rpath,$ORIGIN/../../../micropp_amd"
 
 
@@ -35,9 +37,11 @@ def build() -> dict:
     out["c"] = os.path.join(BIN, "c_driver")
     fc = flang()
     if fc:
-        cmd = [fc, "-O2", "-o", os.path.join(BIN, "f_driver"), os.path.join(HERE, "f_driver.f90"),
-               f"-module-dir", BIN, f"-L{LIBDIR}", "-lmicropp_amd", RPATH]
-        subprocess.run(cmd, check=True)
+        # the drop-in modules first (their .mod files into bin/), then the driver that uses them
+        mods = [os.path.join(FORTRAN, f) for f in ("libmaterial.f95", "libmicropp.f95")]
+        cmd = [fc, "-O2", "-o", os.path.join(BIN, "f_driver")] + mods + [os.path.join(HERE, "f_driver.f90"),
+               "-module-dir", BIN, f"-L{LIBDIR}", "-lmicropp_amd", RPATH]
+        subprocess.run(cmd, check=True, cwd=BIN)
         out["fortran"] = os.path.join(BIN, "f_driver")
     return out
 

@@ -1,88 +1,11 @@
-! f_driver.f90 -- a Fortran caller of the micropp3_* C interface through
-! iso_c_binding, the way a Fortran macro code binds it (the struct layouts and
-! signatures of include/micropp_c.h and include/material_base.h).  Same
-! problem and output as c_driver.c.
-module micropp3_binding
-  use iso_c_binding
-  implicit none
-  type, bind(C) :: material_base
-    real(c_double) :: E, nu, Ka, Sy, k, mu, lambda, Xt
-    integer(c_int) :: mtype
-  end type
-  type, bind(C) :: micropp3
-    type(c_ptr) :: ptr
-  end type
-  interface
-    subroutine material_set(self, mtype, E, nu, Ka, Sy, Xt) bind(C, name="material_set")
-      import :: material_base, c_int, c_double
-      type(material_base), intent(out) :: self
-      integer(c_int), value :: mtype
-      real(c_double), value :: E, nu, Ka, Sy, Xt
-    end subroutine
-    subroutine micropp3_new(self, ngp, nsize, micro_type, micro_params, materials, coupling, nsub, rank) &
-        bind(C, name="micropp3_new")
-      import :: micropp3, material_base, c_int, c_double
-      type(micropp3), intent(out) :: self
-      integer(c_int), value :: ngp, micro_type, nsub, rank
-      integer(c_int), intent(in) :: nsize(3), coupling(*)
-      real(c_double), intent(in) :: micro_params(4)
-      type(material_base), intent(in) :: materials(3)
-    end subroutine
-    subroutine micropp3_free(self) bind(C, name="micropp3_free")
-      import :: micropp3
-      type(micropp3), intent(inout) :: self
-    end subroutine
-    subroutine micropp3_set_strain(self, gp, strain) bind(C, name="micropp3_set_strain")
-      import :: micropp3, c_int, c_double
-      type(micropp3), intent(inout) :: self
-      integer(c_int), value :: gp
-      real(c_double), intent(in) :: strain(6)
-    end subroutine
-    subroutine micropp3_get_stress(self, gp, stress) bind(C, name="micropp3_get_stress")
-      import :: micropp3, c_int, c_double
-      type(micropp3), intent(in) :: self
-      integer(c_int), value :: gp
-      real(c_double), intent(out) :: stress(6)
-    end subroutine
-    subroutine micropp3_get_ctan(self, gp, ctan) bind(C, name="micropp3_get_ctan")
-      import :: micropp3, c_int, c_double
-      type(micropp3), intent(in) :: self
-      integer(c_int), value :: gp
-      real(c_double), intent(out) :: ctan(36)
-    end subroutine
-    subroutine micropp3_homogenize(self) bind(C, name="micropp3_homogenize")
-      import :: micropp3
-      type(micropp3), intent(inout) :: self
-    end subroutine
-    subroutine micropp3_update_vars(self) bind(C, name="micropp3_update_vars")
-      import :: micropp3
-      type(micropp3), intent(inout) :: self
-    end subroutine
-    logical(c_bool) function micropp3_is_non_linear(self, gp) bind(C, name="micropp3_is_non_linear")
-      import :: micropp3, c_int, c_bool
-      type(micropp3), intent(in) :: self
-      integer(c_int), value :: gp
-    end function
-    integer(c_int) function micropp3_get_cost(self, gp) bind(C, name="micropp3_get_cost")
-      import :: micropp3, c_int
-      type(micropp3), intent(in) :: self
-      integer(c_int), value :: gp
-    end function
-    logical(c_bool) function micropp3_has_converged(self, gp) bind(C, name="micropp3_has_converged")
-      import :: micropp3, c_int, c_bool
-      type(micropp3), intent(in) :: self
-      integer(c_int), value :: gp
-    end function
-    logical(c_bool) function micropp3_has_subiterated(self, gp) bind(C, name="micropp3_has_subiterated")
-      import :: micropp3, c_int, c_bool
-      type(micropp3), intent(in) :: self
-      integer(c_int), value :: gp
-    end function
-  end interface
-end module
-
+! f_driver.f90 -- a Fortran caller of micropp_amd through the drop-in
+! modules `libmicropp` / `libmaterial` (micropp_amd/fortran/), the way a
+! Fortran macro code uses the reference's modules.  Same problem and output
+! as c_driver.c: one instance with two Gauss points (FE_FULL and FE_ONE_WAY)
+! of a 6x6x6-node sphere RVE with a J2-plastic matrix, three load steps.
 program f_driver
-  use micropp3_binding
+  use, intrinsic :: iso_c_binding, only: c_int, c_double, c_associated
+  use libmicropp
   implicit none
   integer(c_int), parameter :: ngp = 2, steps = 3
   integer(c_int) :: nsize(3) = [6, 6, 6], coupling(2) = [2, 1]
@@ -116,4 +39,4 @@ program f_driver
     call micropp3_update_vars(m)
   end do
   call micropp3_free(m)
-end program
+end program f_driver
