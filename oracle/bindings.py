@@ -429,6 +429,9 @@ class Ref(_Lib):
             ("vars_new", C.c_int, [_VP, _D, PD, _D]),
             ("newton", None, [_VP, _D, PD, _D, C.c_double, C.c_double, C.c_int, _I]),
             ("write_vtu", C.c_int, [_VP, _D, PD, C.c_char_p]),
+            ("micro_new_rank", _VP, [C.POINTER(Params), C.c_int, C.c_int]),
+            ("write_restart", None, [_VP, C.c_int]),
+            ("read_restart", None, [_VP, C.c_int]),
         ]:
             f = getattr(L, "ref_" + n)
             f.restype = res
@@ -443,9 +446,34 @@ class Ref(_Lib):
     def micro(self, params, keep=None):
         return RefMicro(self, params, keep)
 
+    def micro_rank(self, params, keep=None, rank=0, write_log=0):
+        """An instance of MPI rank `rank` (the restart / log file names), writing the profiling log
+        micropp-profiling-<rank>.log into the current directory when write_log is set."""
+        return RefMicro(self, params, keep, rank=rank, write_log=write_log)
+
 
 class RefMicro(Micro):
     """A reference micropp<3> instance, plus the protected members the shim exposes."""
+
+    def __init__(self, lib, params, keep=None, rank=None, write_log=0):
+        if rank is None:
+            super().__init__(lib, params, keep)
+            return
+        self.lib, self._keep, self.params = lib, keep, params
+        self.h = lib.lib.ref_micro_new_rank(C.byref(params), int(rank), int(write_log))
+        if not self.h:
+            raise ValueError("micro_new_rank failed")
+        self.ngp = params.ngp
+        nx, ny, nz = params.size
+        self.nndim = nx * ny * nz * 3
+        self.nvars = (nx - 1) * (ny - 1) * (nz - 1) * 8 * NVAR
+
+    def write_restart(self, restart_id):
+        """micropp-restart-<rank>-<id>.bin in the current directory."""
+        self.lib.lib.ref_write_restart(self.h, int(restart_id))
+
+    def read_restart(self, restart_id):
+        self.lib.lib.ref_read_restart(self.h, int(restart_id))
 
     def bmat(self):
         b = np.zeros(8 * 6 * 24)

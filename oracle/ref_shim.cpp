@@ -93,6 +93,15 @@ class probe : public micropp<3> {
 
 probe *mk(const orc_params *p) { return new probe(to_params(p)); }
 
+// an instance that writes the profiling log (write_log, src/output.cpp:264-282)
+// as MPI rank `rank`; its files go to the current directory
+probe *mk_log(const orc_params *p, int rank, int write_log) {
+  micropp_params_t q = to_params(p);
+  q.mpi_rank = rank;
+  q.write_log = write_log != 0;
+  return new probe(q);
+}
+
 }  // namespace
 
 extern "C" {
@@ -183,6 +192,11 @@ void ref_ell2d_build(int nx, int ny, const double Ae[16], int *cols, double *val
 }
 
 void *ref_micro_new(const orc_params *p) { return mk(p); }
+void *ref_micro_new_rank(const orc_params *p, int rank, int write_log) { return mk_log(p, rank, write_log); }
+/* micropp<3>::write_restart / read_restart (src/output.cpp:216-262): the
+ * file micropp-restart-<rank>-<id>.bin in the current directory */
+void ref_write_restart(void *m, int id) { static_cast<probe *>(m)->write_restart(id); }
+void ref_read_restart(void *m, int id) { static_cast<probe *>(m)->read_restart(id); }
 void ref_micro_free(void *m) { delete static_cast<probe *>(m); }
 int ref_nrow(void *m) { return static_cast<probe *>(m)->nrow(); }
 int ref_nelem(void *m) { return static_cast<probe *>(m)->ne(); }

@@ -487,3 +487,80 @@ def test_vtu_bytewise(oracle, ref, kind, tmp_path):
         es, ss = orve.calc_fields(u, v)
         sig = orve.ave_stress(u, v)
         assert np.allclose(ss.mean(axis=0), sig, rtol=1e-12, atol=1e-12 * np.abs(sig).max())
+
+
+# ---------------------------------------------------------------------------
+# restart file and profiling log: fixtures written by the compiled reference
+# (tests/golden/make_restart_log.py)
+# ---------------------------------------------------------------------------
+def restart_fixture():
+    import json
+    import os
+
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    with open(os.path.join(d, "restart_log_ref.json")) as f:
+        doc = json.load(f)
+    with open(os.path.join(d, doc["restart_file"]), "rb") as f:
+        raw = f.read()
+    return doc, raw
+
+
+def parse_restart(raw, ngp, nvars, nndim):
+    """The reference restart layout (include/gp.hpp:107-124): per GP a 1-byte bool, then vars_n and u_n
+    (doubles, native order) when it is set.  Returns [(allocated, vars_n, u_n)]."""
+    out, o = [], 0
+    for _ in range(ngp):
+        a = raw[o] != 0
+        o += 1
+        if a:
+            v = np.frombuffer(raw, np.float64, nvars, o)
+            o += 8 * nvars
+            u = np.frombuffer(raw, np.float64, nndim, o)
+            o += 8 * nndim
+            out.append((True, v, u))
+        else:
+            out.append((False, None, None))
+    assert o == len(raw)
+    return out
+
+
+def restart_params(doc):
+    from oracle import bindings as ob
+
+    return ob.make_params(doc["size"], doc["micro_type"], doc["geo"], [ob.material(*m) for m in doc["materials"]],
+                          ngp=len(doc["coupling"]), coupling=doc["coupling"])
+
+
+def test_oracle_reproduces_reference_log_and_restart(oracle):
+    """The oracle over the fixture's three steps: sigma, ctan, cost (the FE_FULL stale addition included,
+    src/homogenize.cpp:267-269) and flags equal the compiled reference's bit for bit, and its state after the
+    second step equals the restart file the reference wrote then, byte for byte."""
+    doc, raw = restart_fixture()
+    p, k = restart_params(doc)
+    m = oracle.micro(p, k)
+    ngp = len(doc["coupling"])
+    for t, rec in enumerate(doc["steps"]):
+        for g in range(ngp):
+            m.set_strain(g, doc["strains"][t][g])
+        m.homogenize()
+        for g in range(ngp):
+            r = rec[g]
+            assert m.get_stress(g).tolist() == r["sigma"], (t, g)
+            assert m.get_ctan(g).ravel().tolist() == r["ctan"], (t, g)
+            assert (m.get_cost(g), m.is_non_linear(g), int(m.has_converged(g))) == (r["cost"], r["non_linear"],
+                                                                                     r["converged"]), (t, g)
+        m.update_vars()
+        if t == doc["restart_after_step"]:
+            for g, (a, v, u) in enumerate(parse_restart(raw, ngp, m.nvars, m.nndim)):
+                assert a == bool(m.is_non_linear(g))
+                if a:
+                    assert np.array_equal(m.state(g, 2), v) and np.array_equal(m.state(g, 0), u)
+    # the log the reference wrote: a header, then per homogenize "log_id : i" and one line per GP
+    lines = doc["log"].splitlines()
+    assert lines[0] == "#<gp_id>  <non-linear>  <cost>  <converged>"
+    assert len(lines) == 1 + len(doc["steps"]) * (1 + ngp)
+    for t, rec in enumerate(doc["steps"]):
+        blk = lines[1 + t * (1 + ngp):1 + (t + 1) * (1 + ngp)]
+        assert blk[0] == f"log_id : {t}"
+        for g in range(ngp):
+            assert blk[1 + g] == f"\t{g}\t{rec[g]['non_linear']}\t{rec[g]['cost']}\t{rec[g]['converged']}"
