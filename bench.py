@@ -19,6 +19,13 @@ Paths
 roofline: the dominant kernel of the mixed path (the FP32 multi-RHS SpMV),
   timed with HIP events on its own stream; algorithmic bytes per node =
   252 x 4 (coefficients incl. the 9 row pads) + 2 x 3K x 4 (p in, Ap out).
+Secondary lines (not value; --no-batch skips them):
+  drop_in: the same workload through the micropp3_* interface (one mixed-
+        precision micropp<3> instance with 6 Gauss points; its constructor
+        and one homogenize per step).
+  batched: the batched J2 (64 RVEs of 30^3, 2 cyclic steps) and damage (16
+        RVEs of 20..50^3, 3 damaging ramp steps) workloads, mixed precision
+        against the FP64 batch, and the adjoint workload (4 RVEs of 40^3).
 cpu_baseline: the oracle (C restatement of the reference; OpenMP over the
   Gauss points as the reference's homogenize) on this host: one sigma solve
   per thread at once, and one solve on one thread; an RVE solve counts as 2
@@ -158,6 +165,54 @@ def rel(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
 
 
+def drop_in(mp, geo, strains, steps):
+    """The headline workload through the drop-in interface (micropp3_new_ext / micropp3_set_strain /
+    micropp3_homogenize / micropp3_get_stress / micropp3_get_ctan): one micropp<3> instance with mixed
+    precision and 6 FE_ONE_WAY Gauss points sharing the 50^3 RVE.  Its constructor runs the six ctan unit-strain
+    solves (calc_ctan_lin_fe_models) as one multi-RHS pass; homogenize runs the 6 Gauss points' Newton solves
+    (sigma by calc_ave_stress) as another.  One step = a fresh instance + one homogenize: 6 RVE solves (sigma +
+    ctan), the same 12 linear solves as a step of `value`.  Returns (dict, sigma, ctan of the last step)."""
+    mats = [mp.material(0, 1e6, 0.3), mp.material(0, 1e7, 0.25), mp.material(0, 1e7, 0.25)]
+    kw = dict(lin_stress=0, cg_rel_tol=CG_TOL, cg_max_its=CG_MAX, mixed_precision=1)
+    ngp = len(strains)
+    t_new, t_hom = [], []
+    sig = ctan = None
+    for s in range(steps + 1):  # the first is untimed
+        mp.load().mppd_synchronize()
+        t0 = time.perf_counter()
+        m = mp.Micropp3(SIZE, mp.MIC3D_RANDOM_SPHERES, geo, mats, ngp=ngp, coupling=[mp.FE_ONE_WAY] * ngp, **kw)
+        t1 = time.perf_counter()
+        for g in range(ngp):
+            m.set_strain(g, strains[g])
+        m.homogenize()
+        sig = np.array([m.get_stress(g) for g in range(ngp)])
+        ctan = m.get_ctan(0)
+        t2 = time.perf_counter()
+        conv = all(m.has_converged(g) for g in range(ngp))
+        m.close()
+        if not conv:
+            raise RuntimeError("drop-in route: a Gauss point did not converge")
+        if s:
+            t_new.append(t1 - t0)
+            t_hom.append(t2 - t1)
+        print(f"drop-in step {s} done", file=sys.stderr, flush=True)
+    ctor, hom = float(np.mean(t_new)), float(np.mean(t_hom))
+    return {"value": ngp / (ctor + hom), "unit": "RVE solves/s",
+            "path": "micropp3_new_ext (mixed_precision) + micropp3_homogenize, 6 FE_ONE_WAY GPs of one 50^3 RVE",
+            "steps": steps, "constructor_ms": 1e3 * ctor, "homogenize_ms": 1e3 * hom,
+            "homogenize_gp_solves_per_s": ngp / hom}, sig, ctan
+
+
+def batched_lines():
+    """Secondary lines (not `value`): the batched nonlinear workloads at their real RVE sizes with reduced RVE
+    and step counts, mixed precision against the FP64 batch, and the adjoint workload."""
+    out = {}
+    out["batch_j2"] = batch_j2(64, 2, mixed=True)
+    out["batch_damage"] = batch_damage(16, 3, mixed=True)
+    out["adjoint"] = adjoint_workload(4, 0)
+    return out
+
+
 # ---------------------------------------------------------------------------
 # secondary workloads (not the headline): batched nonlinear RVEs and adjoint derivatives
 # ---------------------------------------------------------------------------
@@ -172,7 +227,9 @@ def _batch_timed(mp, descs, ramps, params, label):
         b.update_vars()
         print(f"{label} batched step {t} done", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
-    out = (dt, b.info(), b.stats(), b.tuning(), b.stress(), b.mixed_stats() if params.get("mixed_precision") else None)
+    prof = b.spmv_profile(bool(params.get("mixed_precision")))
+    out = (dt, b.info(), b.stats(), b.tuning(), b.stress(), b.mixed_stats() if params.get("mixed_precision") else None,
+           prof)
     del b
     return out
 
@@ -182,20 +239,27 @@ def _batch_run(mp, descs, ramps, params, label):
     subset, for the per-RVE rate); returns the measurement dict.  With mixed_precision, the FP64 batch of the
     same RVEs is timed instead of the one-by-one runs, and sigma is compared with it."""
     nrve, steps = len(descs), len(ramps[0])
-    dt_b, info, stats, (_, spmv_slots, spmv_xcd), sig_b, mstats = _batch_timed(mp, descs, ramps, params, label)
+    dt_b, info, stats, (_, spmv_slots, spmv_xcd), sig_b, mstats, (spmv_ms, spmv_bytes) = _batch_timed(
+        mp, descs, ramps, params, label)
     common = {"rves": nrve, "load_steps": steps, "batched_s": dt_b,
               "nonlinear_rves": int(info[:, 0].sum()), "converged_rves": int(info[:, 2].sum()),
               "graph_chunk": int(stats[1]), "spmv_slots_per_round": spmv_slots, "spmv_blocks_per_xcd": spmv_xcd,
-              "cg_iterations_queued": int(stats[0])}
+              "cg_iterations_queued": int(stats[0]), "host_syncs_last_step": int(stats[4]),
+              "batched_newton_iterations_last_step": int(stats[5]),
+              "spmv": {"precision": "fp32" if mstats is not None else "fp64", "ms": spmv_ms,
+                       "bytes_per_launch": spmv_bytes, "achieved_GBs": spmv_bytes / spmv_ms / 1e6,
+                       "frac_of_hbm_peak": spmv_bytes / spmv_ms / 1e6 / HBM_PEAK_GBS}}
     if mstats is not None:
-        dt64, info64, stats64, _, sig64, _ = _batch_timed(mp, descs, ramps, dict(params, mixed_precision=0),
-                                                           label + " fp64")
+        dt64, info64, stats64, _, sig64, _, (ms64, bytes64) = _batch_timed(
+            mp, descs, ramps, dict(params, mixed_precision=0), label + " fp64")
         err = np.abs(sig_b - sig64).max(axis=1) / np.abs(sig64).max(axis=1)
         return dict(common, value=nrve * steps / dt_b, unit="RVE solves/s (sigma+ctan, FE_FULL)",
                     precision="FP32 inner CG, FP64 defect correction", fp64_batched_rve_solves_per_s=nrve * steps / dt64,
                     speedup_vs_fp64_batch=dt64 / dt_b, sigma_max_rel_err_vs_fp64=float(err.max()),
                     same_flags_as_fp64=bool(np.array_equal(info[:, [0, 2]], info64[:, [0, 2]])),
-                    fp64_cg_iterations_queued=int(stats64[0]),
+                    fp64_cg_iterations_queued=int(stats64[0]), fp64_host_syncs_last_step=int(stats64[4]),
+                    fp64_spmv={"ms": ms64, "bytes_per_launch": bytes64, "achieved_GBs": bytes64 / ms64 / 1e6,
+                               "frac_of_hbm_peak": bytes64 / ms64 / 1e6 / HBM_PEAK_GBS},
                     mixed_outer_steps=mstats[0], mixed_fp64_fallbacks=mstats[1], mixed_short_solves=mstats[2])
     nseq = min(nrve, 4)
     kw = {k: v for k, v in params.items() if k != "graph_chunk"}
@@ -226,6 +290,7 @@ DAMAGE_PARAMS = dict(lin_stress=0, nr_max_its=10, nr_rel_tol=1e-6, cg_rel_tol=1e
 # fourth step.
 DAMAGE_XT = 1e5
 DAMAGE_PEAK = 0.15
+DAMAGE_RAMP_PEAK = 0.12  # the measured ramps: |eps| 0.04, 0.08, 0.12 in 3 steps (damage from the second)
 
 
 def j2_fibre_workload(mp, nrve, steps, seed=3003):
@@ -287,15 +352,16 @@ def batch_j2(nrve, steps, mixed=False):
 
 
 def batch_damage(nrve, steps, mixed=False):
-    """The batched damage workload (damage_layer_workload): batched vs one-by-one, or mixed vs FP64 batch."""
+    """The batched damage workload (damage_layer_workload) on a ramp of `steps` steps to |eps| = 0.12, which
+    damages the matrix from its second step: batched vs one-by-one, or mixed vs FP64 batch."""
     import micropp_amd as mp
 
-    nrve, steps = nrve or 16, steps or 2
-    descs, ramps, params = damage_layer_workload(mp, nrve, steps)
+    nrve, steps = nrve or 16, steps or 3
+    descs, ramps, params = damage_layer_workload(mp, nrve, steps, peak=DAMAGE_RAMP_PEAK, ramp_steps=steps)
     params["mixed_precision"] = int(mixed)
     out = {"metric": "batched damage RVEs (fibre layer, uneven sizes), RVE solves/s", "workload":
-           f"{nrve} RVEs of 20..50^3 hexes, y-layer width U[0.1,0.3], damage matrix, {steps} of 10 ramp steps "
-           f"to |eps| = {DAMAGE_PEAK}"}
+           f"{nrve} RVEs of 20..50^3 hexes, y-layer width U[0.1,0.3], damage matrix, {steps} ramp steps "
+           f"to |eps| = {DAMAGE_RAMP_PEAK}"}
     out.update(_batch_run(mp, descs, ramps, params, "batch_damage"))
     return out
 
@@ -463,6 +529,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the secondary lines (drop-in route, batched J2 / damage, adjoint)")
     ap.add_argument("--profile-iters", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of COPY_PROBE_BYTES (scales HBM counter readings)")
@@ -555,6 +623,17 @@ def main():
                                            "cg_iteration_ms": p_iter_ms}}
         line["speedup_vs_plain_fp64"] = value / (N_STRAINS / dt_plain)
         line["mixed_vs_fp64_rel_err"] = {"sigma": rel(sig_m, sig_p), "ctan": rel(ctan_m, ctan_p)}
+
+    if not args.no_batch:
+        d, sig_d, ctan_d = drop_in(mp, geo, strains, 2)
+        d["vs_value"] = d["value"] / value
+        d["sigma_rel_err_vs_mixed"] = rel(sig_d, sig_m)
+        d["ctan_rel_err_vs_mixed"] = rel(ctan_d, ctan_m)
+        if not args.no_plain:
+            d["sigma_rel_err_vs_fp64"] = rel(sig_d, sig_p)
+            d["ctan_rel_err_vs_fp64"] = rel(ctan_d, ctan_p)
+        line["drop_in"] = d
+        line["batched"] = batched_lines()
 
     if not args.no_cpu:
         line["cpu_baseline"], csig = cpu_baseline(geo, strains)

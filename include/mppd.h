@@ -220,10 +220,13 @@ int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *it
  * FP64.  mp supplies inner_tol, max_inner_its and max_outer (rel_tol /
  * abs_tol unused; NULL: 1e-5, cg_max_its, 10).  A linear solve whose FP32
  * inner CG breaks down (non-finite: the tangent of a softening law can be
- * indefinite) is redone by the FP64 Jacobi-PCG of mppd_newton.  out as
- * mppd_newton, with solver_its = inner (and any FP64 fallback) iterations;
- * stats (may be NULL) = {outer steps, iterations, FP64 fallbacks, linear
- * solves stopped by max_outer short of the tolerance}.  A0 (may be NULL) as
+ * indefinite), or that is still short of the tolerance after max_outer
+ * outer steps (the FP32 CG can stall on such a tangent), is redone by the
+ * FP64 Jacobi-PCG of mppd_newton from 0, so every linear solve meets the
+ * reference CG's tolerance.  out as mppd_newton, with solver_its = inner
+ * (and any FP64 fallback) iterations; stats (may be NULL) = {outer steps,
+ * iterations, FP64 fallbacks (either cause), fallbacks caused by the
+ * max_outer cap}.  A0 (may be NULL) as
  * in mppd_newton: the first p->its_with_A0 Newton steps solve with A0's
  * matrix instead of assembling (micropp<3> with use_A0 and mixed precision,
  * for GPs that do not run as a batch);
@@ -290,12 +293,19 @@ int mppd_batch_get_ctan(const mppd_batch *b, double *ctan);
 /* info: nrve x 4 = {is_non_linear (history allocated), cost, converged, subiterated} */
 int mppd_batch_get_info(const mppd_batch *b, int *info);
 int mppd_batch_update_vars(mppd_batch *b);
-/* stats: {CG iterations queued, graph chunk length, graph launches, device ms of the last homogenize} */
-int mppd_batch_stats(const mppd_batch *b, double stats[4]);
+/* stats: {CG iterations queued, graph chunk length, graph launches, ms of the last homogenize, host
+ * synchronisations of the last homogenize, batched Newton iterations of the last homogenize (each one
+ * launch sequence over all RVEs still iterating)} */
+int mppd_batch_stats(const mppd_batch *b, double stats[6]);
 /* mixed_precision totals since mppd_batch_create: {outer steps, linear solves redone by the FP64 CG
- * after an FP32 breakdown, linear solves stopped by the outer-step cap short of the tolerance}
- * (no reference counterpart) */
+ * (after an FP32 breakdown, or short of the tolerance at the outer-step cap), those of them caused by
+ * the outer-step cap} (no reference counterpart) */
 int mppd_batch_mixed_stats(const mppd_batch *b, long out[3]);
+/* Timing aid (no reference counterpart): the batch's CG SpMV (fp32 = 0: FP64 on vals; fp32 = 1: the
+ * mixed-precision inner FP32 SpMV) in its tuned shape over every RVE, iters launches timed with HIP
+ * events on the batch's stream with every CG live: *ms per launch, *bytes (may be NULL) its algorithmic
+ * bytes per launch (81 values + p in + Ap out per padded row, at the matrix precision). */
+int mppd_batch_spmv_profile(mppd_batch *b, int fp32, int iters, double *ms, double *bytes);
 /* the tuned shape: {graph chunk length, SpMV slots per load round, SpMV block order (1 = per XCD)} */
 int mppd_batch_tuning(const mppd_batch *b, int out[3]);
 

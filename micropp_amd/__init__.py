@@ -162,6 +162,7 @@ SIGNATURES = {
     "mppd_batch_mixed_stats": (_I, [_VP, C.POINTER(C.c_long)]),
     "mppd_batch_slot_bytes": (_I, [_VP, C.POINTER(C.c_size_t)]),
     "mppd_batch_tuning": (_I, [_VP, _PI]),
+    "mppd_batch_spmv_profile": (_I, [_VP, _I, _I, _PD, _PD]),
     "mppd_batch_load": (_I, [_VP, _I, _VP, _VP, _VP]),
     "mppd_batch_store": (_I, [_VP, _I, _VP, _VP, _VP]),
 }
@@ -709,8 +710,9 @@ class Batch:
         _chk(load().mppd_batch_update_vars(self.h), "mppd_batch_update_vars")
 
     def stats(self):
-        """CG iterations queued, graph chunk length, graph launches, ms of the last homogenize."""
-        out = np.zeros(4)
+        """CG iterations queued, graph chunk length, graph launches, ms of the last homogenize, host
+        synchronisations and batched Newton iterations of the last homogenize."""
+        out = np.zeros(6)
         _chk(load().mppd_batch_stats(self.h, _dp(out)), "mppd_batch_stats")
         return out
 
@@ -725,6 +727,13 @@ class Batch:
         out = np.zeros(3, dtype=np.int32)
         _chk(load().mppd_batch_tuning(self.h, _ip(out)), "mppd_batch_tuning")
         return tuple(int(v) for v in out)
+
+    def spmv_profile(self, fp32=False, iters=50):
+        """(ms per launch, algorithmic bytes per launch) of the batch's CG SpMV over all RVEs."""
+        ms, nb = C.c_double(), C.c_double()
+        _chk(load().mppd_batch_spmv_profile(self.h, int(fp32), int(iters), C.byref(ms), C.byref(nb)),
+             "mppd_batch_spmv_profile")
+        return ms.value, nb.value
 
 
 def cg_profile(solver: "Solver", b: DeviceArray, iters: int):

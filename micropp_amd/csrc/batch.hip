@@ -198,6 +198,85 @@ __global__ void __launch_bounds__(256) kb_axpy(const BRve *rv, const int *map, c
   if (row < R.g.nrow) R.uw[row] += 1.0 * R.x[row];
 }
 
+// ---------------- device-side Newton control ----------------
+__device__ __forceinline__ void nt_test(BNewton &s, const BNewtonTest &t) {
+  // the reference tests at the top of an iteration: at the cap it stops
+  // unconverged, even when the last residual passes (src/solve.cpp:43-78)
+  if (s.its >= t.max_its) {
+    s.run = 0;
+  } else if (s.norm < t.max_tol || s.norm < s.norm0 * t.rel_tol) {
+    s.conv = 1;
+    s.run = 0;
+  }
+}
+
+__global__ void kb_nt_start(int n, const int *on, const double *norms, BNewtonTest t, BNewton *nst) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  BNewton s = nst[r];
+  s.its = s.sits = s.conv = s.applied = 0;
+  s.run = on[r] != 0;
+  if (s.run) {
+    s.norm0 = s.norm = norms[r];
+    nt_test(s, t);
+  }
+  nst[r] = s;
+}
+
+__global__ void kb_nt_prepare(int n, const BRve *rv, BNewton *nst, int has_a0, int its_with_A0, int *act, int *ke,
+                              int *gather, int *a0on, int *pend) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  BNewton s = nst[r];
+  const int run = s.run;
+  const int a0sel = run && has_a0 && s.its < its_with_A0;
+  const int phase = rv[r].phase_mode;
+  act[r] = run;
+  if (has_a0) a0on[r] = a0sel;
+  ke[r] = run && !a0sel && !phase;
+  gather[r] = run && !a0sel && (!phase || !s.vals_ready);
+  if (run && !a0sel && phase) s.vals_ready = 1;
+  s.applied = 0;
+  pend[r] = 0;
+  nst[r] = s;
+}
+
+// rows: uw += x of the RVEs whose CG stopped and whose correction is not in uw
+__global__ void __launch_bounds__(256) kb_nt_axpy(const BRve *rv, const int *map, const int *blk0,
+                                                  const BNewton *nst) {
+#pragma clang fp contract(off)
+  const int rid = map[blockIdx.x];
+  const BNewton &s = nst[rid];
+  const BRve &R = rv[rid];
+  if (!s.run || s.applied || !R.cg[0].done) return;
+  const int row = (blockIdx.x - blk0[rid]) * 256 + threadIdx.x;
+  if (row < R.g.nrow) R.uw[row] += 1.0 * R.x[row];
+}
+
+__global__ void kb_nt_mark(int n, const BRve *rv, BNewton *nst, int *pend) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  BNewton s = nst[r];
+  const CgState c = rv[r].cg[0];
+  const int now = s.run && !s.applied && c.done;
+  if (now) {
+    s.applied = 1;
+    s.sits += c.its;
+    nst[r] = s;
+  }
+  pend[r] = now;
+}
+
+__global__ void kb_nt_after(int n, const int *pend, const double *norms, BNewtonTest t, BNewton *nst) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || !pend[r]) return;
+  BNewton s = nst[r];
+  s.its += 1;
+  s.norm = norms[r];
+  nt_test(s, t);
+  nst[r] = s;
+}
+
 // ---------------- mixed precision ----------------
 __global__ void __launch_bounds__(256) kb_m_prep(const BRve *rv, const int *map, const int *act, const int *blk0,
                                                  const int *a0on) {
@@ -239,8 +318,10 @@ __global__ void __launch_bounds__(256) kb_m_resid(const BRve *rv, const int *map
 // one block per RVE: the outer test and the inner CG's start state (an
 // inactive or converged RVE's inner CG starts done)
 __global__ void __launch_bounds__(256) kb_m_outer(const BRve *rv, const int *act, int first, double rel,
-                                                  double abs_tol, double tau_min, int max_inner, BOuter *out) {
+                                                  double abs_tol, double tau_min, int max_inner, const int *tight,
+                                                  double tight_factor, BOuter *out) {
   const int rid = blockIdx.x;
+  if (tight && tight[rid]) rel *= tight_factor;
   const BRve &R = rv[rid];
   if (!act[rid]) {
     if (threadIdx.x == 0) {
@@ -377,15 +458,40 @@ hipError_t launch_b_axpy(const BLaunch &L, hipStream_t st) {
   return hipGetLastError();
 }
 
+static unsigned per_rve_blocks(int n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t launch_b_nt_start(const BLaunch &L, const int *on, const double *norms, BNewtonTest t, BNewton *nst,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(kb_nt_start, dim3(per_rve_blocks(L.nrve)), dim3(256), 0, st, L.nrve, on, norms, t, nst);
+  return hipGetLastError();
+}
+hipError_t launch_b_nt_prepare(const BLaunch &L, BNewton *nst, int has_a0, int its_with_A0, int *act, int *ke,
+                               int *gather, int *a0on, int *pend, hipStream_t st) {
+  hipLaunchKernelGGL(kb_nt_prepare, dim3(per_rve_blocks(L.nrve)), dim3(256), 0, st, L.nrve, L.rv, nst, has_a0,
+                     its_with_A0, act, ke, gather, a0on, pend);
+  return hipGetLastError();
+}
+hipError_t launch_b_nt_finish(const BLaunch &L, BNewtonTest t, BNewton *nst, int *pend, double *norms,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(kb_nt_axpy, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, nst);
+  hipLaunchKernelGGL(kb_nt_mark, dim3(per_rve_blocks(L.nrve)), dim3(256), 0, st, L.nrve, L.rv, nst, pend);
+  BLaunch P = L;
+  P.act = pend;
+  hipError_t e = launch_b_rhs(P, norms, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kb_nt_after, dim3(per_rve_blocks(L.nrve)), dim3(256), 0, st, L.nrve, pend, norms, t, nst);
+  return hipGetLastError();
+}
+
 hipError_t launch_bm_prep(const BLaunch &L, hipStream_t st) {
   hipLaunchKernelGGL(kb_m_prep, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0, L.a0on);
   return hipGetLastError();
 }
 hipError_t launch_bm_resid(const BLaunch &L, int first, double rel, double abs_tol, double tau_min, int max_inner,
-                           BOuter *out, hipStream_t st) {
+                           const int *tight, double tight_factor, BOuter *out, hipStream_t st) {
   hipLaunchKernelGGL(kb_m_resid, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0, L.a0on);
   hipLaunchKernelGGL(kb_m_outer, dim3(L.nrve), dim3(256), 0, st, L.rv, L.act, first, rel, abs_tol, tau_min,
-                     max_inner, out);
+                     max_inner, tight, tight_factor, out);
   return hipGetLastError();
 }
 hipError_t launch_bm_spmv(const BLaunch &L, int shape, int parity, hipStream_t st) {

@@ -50,6 +50,22 @@ struct BOuter {
   int conv, pad;
 };
 
+// The Newton loop of one RVE (newton_raphson, src/solve.cpp:28-82) kept on
+// the device, so that a batched Newton iteration needs no host decision
+// between its kernels: its = Newton iterations, sits = CG iterations, run =
+// still iterating, conv = converged, applied = this iteration's correction
+// is in uw, vals_ready = phase mode: vals assembled once.
+struct BNewton {
+  double norm0, norm;
+  int its, sits, run, conv, applied, vals_ready, pad[2];
+};
+
+// Newton parameters the device-side loop tests (mppd_newton_params)
+struct BNewtonTest {
+  int max_its;
+  double max_tol, rel_tol;
+};
+
 struct BLaunch {
   int nrve;
   const BRve *rv;
@@ -83,15 +99,36 @@ hipError_t launch_b_spmv(const BLaunch &L, int shape, int parity, hipStream_t st
 // uw += x
 hipError_t launch_b_axpy(const BLaunch &L, hipStream_t st);
 
+// Device-side Newton control (one thread per RVE unless said otherwise).
+// start: after the first residual (norms) of the RVEs in `on`, its = sits =
+// 0, norm0 = norm, and the test of the loop's top: run = its < max_its and
+// the norm above max(max_tol, rel_tol norm0), conv when the norm passes.
+hipError_t launch_b_nt_start(const BLaunch &L, const int *on, const double *norms, BNewtonTest t, BNewton *nst,
+                             hipStream_t st);
+// prepare one Newton iteration: act = run; the tangent masks of the
+// iteration (ke: element stiffness, gather: vals, a0on: solve with the
+// initial matrix while its < its_with_A0 when has_a0); applied = pend = 0
+hipError_t launch_b_nt_prepare(const BLaunch &L, BNewton *nst, int has_a0, int its_with_A0, int *act, int *ke,
+                               int *gather, int *a0on, int *pend, hipStream_t st);
+// finish the RVEs whose CG has stopped (state cg[0]: the host queues an even
+// number of iterations) and whose correction is not yet applied: uw += x
+// (rows), then applied = pend = 1 and sits += CG iterations, then (with
+// L.act = pend) the residual b and its norm, then its += 1 and the test of
+// the loop's top for the next iteration.  Idempotent for RVEs already
+// applied, so it can follow every further chunk of CG iterations.
+hipError_t launch_b_nt_finish(const BLaunch &L, BNewtonTest t, BNewton *nst, int *pend, double *norms,
+                              hipStream_t st);
+
 // Mixed precision: FP32 inner Jacobi-PCG with FP64 defect correction.
 // prep: vals32 = the matrix being solved (vals, or a0 where a0on), kinv /
 // kinv32 = 1 / diag, x (du) = 0
 hipError_t launch_bm_prep(const BLaunch &L, hipStream_t st);
 // r = b - A du (FP64) and the outer test into out[rve]; the inner CG starts
 // from r32 = r (x32 = 0) with rel_err = max(tau_min, 0.3 target / pn) and
-// max_inner iterations; first: du = 0, so pn0 and target are set
+// max_inner iterations; first: du = 0, so pn0 and target are set, with rel
+// times tight_factor for the RVEs whose entry in tight (may be null) is set
 hipError_t launch_bm_resid(const BLaunch &L, int first, double rel, double abs_tol, double tau_min, int max_inner,
-                           BOuter *out, hipStream_t st);
+                           const int *tight, double tight_factor, BOuter *out, hipStream_t st);
 // one inner CG iteration of every RVE (done RVEs return at once)
 hipError_t launch_bm_iter(const BLaunch &L, int parity, hipStream_t st);
 // the inner FP32 SpMV alone in one of kSpmvShapes shapes, as launch_b_spmv

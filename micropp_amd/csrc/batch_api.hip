@@ -34,7 +34,7 @@ struct BState {
   int vals_ready = 0;     // phase mode: vals assembled once
 };
 
-enum { M_MAIN = 0, M_KE = 1, M_GATHER = 2, M_A0 = 3, M_NUM = 4 };
+enum { M_MAIN = 0, M_KE = 1, M_GATHER = 2, M_A0 = 3, M_TIGHT = 4, M_NUM = 5 };
 
 }  // namespace
 
@@ -71,10 +71,24 @@ struct mppd_batch {
   hipGraph_t graph32 = nullptr;
   hipGraphExec_t gexec32 = nullptr;
   long m_outer = 0, m_fallbacks = 0, m_short = 0;
+  // device-side Newton control of the FP64 path (newton_batch): per-RVE
+  // state (device, pinned), the per-RVE mask of RVEs whose correction was
+  // applied in the last finishing pass, and the graphs of a Newton
+  // iteration's head (tangent + CG start) and finishing pass
+  BNewton *nst_d = nullptr, *nst_h = nullptr;
+  int *pend_d = nullptr;
+  hipGraph_t graph_head = nullptr, graph_tail = nullptr;
+  hipGraphExec_t gexec_head = nullptr, gexec_tail = nullptr;
+  int cg_est = 0;  // CG iterations queued before the first read of a Newton iteration
+  long syncs = 0, syncs_last = 0, newton_its_last = 0;
   // test hook (MPPD_FORCE_FP64_FALLBACK=1 at mppd_batch_create): every FP32
   // inner solve counts as broken down, so each linear solve is redone by the
   // FP64 CG from du = 0, which exercises the fallback path
   bool force_fallback = false;
+  // test hook (MPPD_BATCH_HOST_NEWTON=1 at mppd_batch_create): the FP64 path
+  // runs the host-driven Newton loop (newton_batch_host) instead of the
+  // device-side one, for comparison
+  bool host_newton = false;
   int spmv_shape = kSpmvDefault;    // tuned SpMV shape (launch_b_spmv)
   int spmv_shape32 = kSpmvDefault;  // tuned inner FP32 SpMV shape (launch_bm_spmv)
   double cg_queued = 0, graph_launches = 0, last_ms = 0;
@@ -98,6 +112,7 @@ namespace {
 
 int sync(mppd_batch *B) {
   MPP_HIP(hipStreamSynchronize(B->st));
+  ++B->syncs;
   return MPPD_OK;
 }
 
@@ -244,9 +259,10 @@ int cg_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &its) {
 // FP64.  An inner CG that gains less than 10% over >= 32 iterations is
 // stopped (its RVE's next outer step restarts from the true residual); an RVE
 // whose inner CG breaks down (non-finite) is solved again by the FP64 CG.
-// its[r] = inner iterations (+ FP64 fallback iterations).  Leaves the main
-// mask = on and du in x.
-int cg_batch_mixed(mppd_batch *B, const std::vector<int> &on, std::vector<int> &its) {
+// its[r] = inner iterations (+ FP64 fallback iterations).  RVEs set in
+// `tight` solve to cg_rel_tol * kMixedLastStepTighten (the last Newton
+// iteration nr_max_its permits).  Leaves the main mask = on and du in x.
+int cg_batch_mixed(mppd_batch *B, const std::vector<int> &on, const std::vector<int> &tight, std::vector<int> &its) {
   const mppd_newton_params &p = B->prm.newton;
   constexpr double kInnerTol = 1e-5;
   constexpr int kMaxOuter = 10;
@@ -254,12 +270,14 @@ int cg_batch_mixed(mppd_batch *B, const std::vector<int> &on, std::vector<int> &
   for (int r = 0; r < n; ++r) its[r] = 0;
   int rc = set_mask(B, M_MAIN, on);
   if (rc) return rc;
+  if ((rc = set_mask(B, M_TIGHT, tight))) return rc;
   const BLaunch L = B->launch(M_MAIN);
   MPP_HIP(launch_bm_prep(L, B->st));
   std::vector<int> live(on), broke(n, 0);
   for (int outer = 0;; ++outer) {
     if ((rc = set_mask(B, M_MAIN, live))) return rc;
-    MPP_HIP(launch_bm_resid(L, outer == 0, p.cg_rel_tol, p.cg_abs_tol, kInnerTol, max_inner, B->outer_d, B->st));
+    MPP_HIP(launch_bm_resid(L, outer == 0, p.cg_rel_tol, p.cg_abs_tol, kInnerTol, max_inner, B->mask_d[M_TIGHT],
+                            kMixedLastStepTighten, B->outer_d, B->st));
     MPP_HIP(hipMemcpyAsync(B->outer_h, B->outer_d, sizeof(BOuter) * n, hipMemcpyDeviceToHost, B->st));
     if ((rc = read_cg(B))) return rc;
     bool any = false;
@@ -282,8 +300,13 @@ int cg_batch_mixed(mppd_batch *B, const std::vector<int> &on, std::vector<int> &
           any = false;
         }
     if (!any) break;
-    if (outer >= kMaxOuter) {
-      for (int r = 0; r < n; ++r) B->m_short += live[r];
+    if (outer >= kMaxOuter) {  // short of the tolerance: redone by the FP64 CG below, like a breakdown
+      for (int r = 0; r < n; ++r)
+        if (live[r]) {
+          ++B->m_short;
+          live[r] = 0;
+          broke[r] = 1;
+        }
       break;
     }
     ++B->m_outer;
@@ -386,9 +409,11 @@ int mat_batch(mppd_batch *B, const std::vector<int> &on, const std::vector<int> 
 }
 
 // newton_raphson (src/solve.cpp:28-82) of every RVE in `on`, on uw with the
-// strains in eps_d: its / solver_its / converged per RVE.
-int newton_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &its, std::vector<int> &sits,
-                 std::vector<int> &conv) {
+// strains in eps_d, the host deciding between kernels (the mixed-precision
+// path, whose linear solves steer from the host): its / solver_its /
+// converged per RVE.
+int newton_batch_host(mppd_batch *B, const std::vector<int> &on, std::vector<int> &its, std::vector<int> &sits,
+                      std::vector<int> &conv) {
   const mppd_newton_params &p = B->prm.newton;
   const int n = B->n;
   int rc = set_mask(B, M_MAIN, on);
@@ -418,14 +443,139 @@ int newton_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &it
       }
     }
     if (!any) break;
+    ++B->newton_its_last;
     if ((rc = mat_batch(B, run, its))) return rc;
-    rc = B->prm.mixed_precision ? cg_batch_mixed(B, run, cg_its) : cg_batch(B, run, cg_its);
+    if (B->prm.mixed_precision) {
+      // the last Newton iteration nr_max_its permits solves to a tighter target: if the Newton stops
+      // there unconverged, this correction is the last term of the state it returns
+      std::vector<int> tight(n, 0);
+      for (int r = 0; r < n; ++r) tight[r] = run[r] && its[r] == p.nr_max_its - 1;
+      rc = cg_batch_mixed(B, run, tight, cg_its);
+    } else {
+      rc = cg_batch(B, run, cg_its);
+    }
     if (rc) return rc;  // leaves the main mask = run
     for (int r = 0; r < n; ++r) sits[r] += cg_its[r];
     MPP_HIP(launch_b_axpy(L, B->st));
     MPP_HIP(launch_b_rhs(L, B->norms_d, B->st));
     if ((rc = read_norms(B))) return rc;
     for (int r = 0; r < n; ++r) its[r] += run[r];
+  }
+  return MPPD_OK;
+}
+
+BNewtonTest newton_test(const mppd_batch *B) {
+  const mppd_newton_params &p = B->prm.newton;
+  return BNewtonTest{p.nr_max_its, p.nr_max_tol, p.nr_rel_tol};
+}
+
+// The head of a Newton iteration (masks of the running RVEs, their tangent,
+// the start of their CGs) and its finishing pass (correction, residual, norm,
+// the loop's test), each captured once as a graph: the device decides per
+// RVE, so a batched Newton iteration has no host decision between kernels.
+int capture_newton_graphs(mppd_batch *B) {
+  int rc = sync(B);
+  if (rc) return rc;
+  const mppd_newton_params &p = B->prm.newton;
+  const BLaunch L = B->launch(M_MAIN);
+  for (int k = 0; k < 2; ++k) {
+    MPP_HIP(hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    if (k == 0) {
+      e = launch_b_nt_prepare(L, B->nst_d, B->a0 != nullptr, p.its_with_A0, B->mask_d[M_MAIN], B->mask_d[M_KE],
+                              B->mask_d[M_GATHER], B->mask_d[M_A0], B->pend_d, B->st);
+      if (e == hipSuccess) e = launch_b_mat(L, B->mask_d[M_KE], B->mask_d[M_GATHER], B->st);
+      if (e == hipSuccess) e = launch_b_cg_init(L, p.cg_abs_tol, p.cg_rel_tol, p.cg_max_its, B->st);
+    } else {
+      e = launch_b_nt_finish(L, newton_test(B), B->nst_d, B->pend_d, B->norms_d, B->st);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(B->st, &graph);
+    if (e != hipSuccess) return hip_fail(e, "Newton graph capture");
+    if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+    (k == 0 ? B->graph_head : B->graph_tail) = graph;
+    MPP_HIP(hipGraphInstantiate(k == 0 ? &B->gexec_head : &B->gexec_tail, graph, nullptr, nullptr, 0));
+  }
+  return MPPD_OK;
+}
+
+// the per-RVE Newton states and the CG state pairs in one read
+int read_newton(mppd_batch *B) {
+  MPP_HIP(hipMemcpyAsync(B->nst_h, B->nst_d, sizeof(BNewton) * B->n, hipMemcpyDeviceToHost, B->st));
+  MPP_HIP(hipMemcpyAsync(B->cg_h, B->cg, sizeof(CgState) * 2 * B->n, hipMemcpyDeviceToHost, B->st));
+  return sync(B);
+}
+
+// newton_raphson (src/solve.cpp:28-82) of every RVE in `on`, on uw with the
+// strains in eps_d: its / solver_its / converged per RVE.  FP64: the loop's
+// decisions run on the device (BNewton), and each Newton iteration is the
+// head graph, chunk graphs of the CG (as many as the last iteration's
+// slowest CG needed), the finishing graph and ONE read of the states; only
+// RVEs whose CG has not stopped by then get more chunks and another
+// finishing pass.  Each RVE runs the same kernels in the same order as
+// newton_batch_host, so its results are the same bit for bit.
+int newton_batch(mppd_batch *B, const std::vector<int> &on, std::vector<int> &its, std::vector<int> &sits,
+                 std::vector<int> &conv) {
+  if (B->prm.mixed_precision || B->host_newton) return newton_batch_host(B, on, its, sits, conv);
+  const mppd_newton_params &p = B->prm.newton;
+  const int n = B->n, G = B->gchunk;
+  int rc = set_mask(B, M_MAIN, on);
+  if (rc) return rc;
+  const BLaunch L = B->launch(M_MAIN);
+  MPP_HIP(launch_b_set_bc(L, B->eps_d, B->st));
+  MPP_HIP(launch_b_rhs(L, B->norms_d, B->st));
+  MPP_HIP(launch_b_nt_start(L, B->mask_d[M_MAIN], B->norms_d, newton_test(B), B->nst_d, B->st));
+  if ((rc = read_newton(B))) return rc;
+  std::vector<double> pn_prev(n), target(n);
+  std::vector<int> its_prev(n);
+  for (;;) {
+    bool any = false;
+    for (int r = 0; r < n; ++r) any |= B->nst_h[r].run != 0;
+    if (!any) break;
+    MPP_HIP(hipGraphLaunch(B->gexec_head, B->st));
+    ++B->newton_its_last;
+    int queued = 0, chunk = std::max(G, B->cg_est);
+    for (;;) {
+      const int reps = std::max(1, (std::min(chunk, p.cg_max_its - queued) + G - 1) / G);
+      for (int i = 0; i < reps; ++i) MPP_HIP(hipGraphLaunch(B->gexec, B->st));
+      queued += reps * G;
+      B->cg_queued += reps * G;
+      B->graph_launches += reps;
+      MPP_HIP(hipGraphLaunch(B->gexec_tail, B->st));
+      if ((rc = read_newton(B))) return rc;
+      // RVEs still in their CG (state cg[0]: queued is a multiple of the even G)
+      int next = 0;
+      bool pending = false;
+      for (int r = 0; r < n; ++r) {
+        const BNewton &s = B->nst_h[r];
+        if (!s.run || s.applied) continue;
+        pending = true;
+        const CgState &c = B->cg_h[2 * r];
+        int need = 2 * chunk;
+        const int dit = c.its - its_prev[r];
+        if (queued > reps * G && dit > 0 && c.pn > 0 && pn_prev[r] > 0 && c.pn < pn_prev[r]) {
+          const double rate = std::log(c.pn / pn_prev[r]) / dit;
+          const double est = std::log(std::max(c.min_err, c.pn0 * c.rel_err) / c.pn) / rate;
+          if (std::isfinite(est)) need = (int)std::ceil(est) + 2;
+        }
+        next = std::max(next, need);
+        pn_prev[r] = c.pn;
+        its_prev[r] = c.its;
+      }
+      if (!pending || queued >= p.cg_max_its) break;
+      chunk = std::max(G, std::min(256, next));
+    }
+    // the next iteration starts with as many as this one's slowest CG took
+    int most = 0;
+    for (int r = 0; r < n; ++r)
+      if (on[r]) most = std::max(most, B->cg_h[2 * r].its);
+    B->cg_est = most + G;
+  }
+  for (int r = 0; r < n; ++r) {
+    const BNewton &s = B->nst_h[r];
+    its[r] = on[r] ? s.its : 0;
+    sits[r] = on[r] ? s.sits : 0;
+    conv[r] = on[r] ? s.conv : 0;
   }
   return MPPD_OK;
 }
@@ -574,13 +724,18 @@ void destroy(mppd_batch *B) {
   if (B->graph) (void)hipGraphDestroy(B->graph);
   if (B->gexec32) (void)hipGraphExecDestroy(B->gexec32);
   if (B->graph32) (void)hipGraphDestroy(B->graph32);
+  for (hipGraphExec_t x : {B->gexec_head, B->gexec_tail})
+    if (x) (void)hipGraphExecDestroy(x);
+  for (hipGraph_t x : {B->graph_head, B->graph_tail})
+    if (x) (void)hipGraphDestroy(x);
   for (int k = 0; k < 3; ++k) {
     dfree(B->map_d[k]);
     dfree(B->blk0_d[k]);
   }
   for (void *p : {(void *)B->rv, (void *)B->vec, (void *)B->vals, (void *)B->a0, (void *)B->f32,
-                  (void *)B->outer_d, (void *)B->ke, (void *)B->be, (void *)B->vars, (void *)B->part, (void *)B->cg, (void *)B->flags_d, (void *)B->eps_d, (void *)B->norms_d,
-                  (void *)B->sig_d})
+                  (void *)B->outer_d, (void *)B->ke, (void *)B->be, (void *)B->vars, (void *)B->part, (void *)B->cg,
+                  (void *)B->flags_d, (void *)B->eps_d, (void *)B->norms_d, (void *)B->sig_d, (void *)B->nst_d,
+                  (void *)B->pend_d})
     dfree(p);
   for (auto &kv : B->cols) dfree(kv.second);
   for (int k = 0; k < M_NUM; ++k) {
@@ -588,7 +743,7 @@ void destroy(mppd_batch *B) {
     if (B->mask_h[k]) (void)hipHostFree(B->mask_h[k]);
   }
   for (void *p : {(void *)B->flags_h, (void *)B->eps_h, (void *)B->norms_h, (void *)B->sig_h, (void *)B->cg_h,
-                  (void *)B->outer_h})
+                  (void *)B->outer_h, (void *)B->nst_h})
     if (p) (void)hipHostFree(p);
   for (mppd_rve *r : B->rves) mppd_rve_destroy(r);
   if (B->st) (void)hipStreamDestroy(B->st);
@@ -777,6 +932,9 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
   if ((rc = pinned(&B->norms_h, n))) return rc;
   if ((rc = pinned(&B->sig_h, 6 * (size_t)n))) return rc;
   if ((rc = pinned(&B->cg_h, 2 * (size_t)n))) return rc;
+  if ((rc = dalloc(&B->nst_d, n))) return rc;
+  if ((rc = pinned(&B->nst_h, n))) return rc;
+  if ((rc = dalloc(&B->pend_d, n))) return rc;
   if ((rc = upload_table(B))) return rc;
   // element stiffness per phase of the all-elastic RVEs, once
   std::vector<int> ph(n);
@@ -790,11 +948,14 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
   if ((rc = tune_spmv(B))) return rc;
   if ((rc = set_mask(B, M_MAIN, std::vector<int>(n, 0)))) return rc;
   MPP_HIP(launch_b_cg_init(B->launch(M_MAIN), 0.0, 0.0, 0, B->st));  // every CG done: timing runs exit early
+  // an even chunk length: after whole chunks the CG state is state[0]
   if (B->prm.graph_chunk > 0)
-    rc = capture_graph(B, B->prm.graph_chunk);
+    rc = capture_graph(B, (B->prm.graph_chunk + 1) / 2 * 2);
   else
     rc = autotune(B);
   if (rc) return rc;
+  if ((rc = capture_newton_graphs(B))) return rc;
+  B->cg_est = 2 * B->gchunk;
   if (B->prm.mixed_precision && ((rc = tune_spmv(B, true)) || (rc = capture_graph32(B)))) return rc;
   if ((rc = sync(B))) return rc;
   // micropp constructor: ctan of every GP from the linear solves
@@ -845,6 +1006,8 @@ int mppd_batch_create(mppd_batch **out, int nrve, const mppd_batch_rve *rves, co
   B->prm = *p;
   const char *ff = getenv("MPPD_FORCE_FP64_FALLBACK");
   B->force_fallback = ff && atoi(ff) != 0;
+  const char *hn = getenv("MPPD_BATCH_HOST_NEWTON");
+  B->host_newton = hn && atoi(hn) != 0;
   const int rc = setup(B, rves);
   if (rc) {
     destroy(B);
@@ -871,6 +1034,8 @@ int mppd_batch_set_strains(mppd_batch *b, const double *eps) {
 int mppd_batch_homogenize(mppd_batch *B) {
   if (!B) return fail(MPPD_ERR_ARG, "null batch");
   const auto t0 = std::chrono::steady_clock::now();
+  const long syncs0 = B->syncs;
+  B->newton_its_last = 0;
   const int n = B->n;
   int rc = MPPD_OK;
   std::vector<int> fe(n, 0), its(n), sits(n), conv(n);
@@ -1009,6 +1174,7 @@ int mppd_batch_homogenize(mppd_batch *B) {
   }
   if ((rc = sync(B))) return rc;
   B->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  B->syncs_last = B->syncs - syncs0;
   return MPPD_OK;
 }
 
@@ -1046,12 +1212,55 @@ int mppd_batch_update_vars(mppd_batch *b) {
   return upload_table(b);
 }
 
-int mppd_batch_stats(const mppd_batch *b, double stats[4]) {
+int mppd_batch_stats(const mppd_batch *b, double stats[6]) {
   if (!b || !stats) return fail(MPPD_ERR_ARG, "null argument");
   stats[0] = b->cg_queued;
   stats[1] = b->gchunk;
   stats[2] = b->graph_launches;
   stats[3] = b->last_ms;
+  stats[4] = (double)b->syncs_last;
+  stats[5] = (double)b->newton_its_last;
+  return MPPD_OK;
+}
+
+int mppd_batch_spmv_profile(mppd_batch *b, int fp32, int iters, double *ms, double *bytes) {
+  if (!b || iters < 1 || !ms) return fail(MPPD_ERR_ARG, "bad batch SpMV profile arguments");
+  if (fp32 && !b->prm.mixed_precision) return fail(MPPD_ERR_ARG, "the batch has no FP32 matrices");
+  int rc = sync(b);
+  if (rc) return rc;
+  // every CG live (one iteration each), as during a solve; stopped again afterwards
+  std::vector<CgState> live(2 * (size_t)b->n);
+  for (auto &c : live) {
+    c = CgState{};
+    c.max_its = 1;
+  }
+  MPP_HIP(hipMemcpy(b->cg, live.data(), sizeof(CgState) * live.size(), hipMemcpyHostToDevice));
+  const BLaunch L = b->launch(M_MAIN);
+  const int shape = fp32 ? b->spmv_shape32 : b->spmv_shape;
+  auto spmv = [&]() { return fp32 ? launch_bm_spmv(L, shape, 0, b->st) : launch_b_spmv(L, shape, 0, b->st); };
+  hipEvent_t e0, e1;
+  MPP_HIP(hipEventCreate(&e0));
+  MPP_HIP(hipEventCreate(&e1));
+  hipError_t e = spmv();
+  if (e == hipSuccess) e = hipEventRecord(e0, b->st);
+  for (int i = 0; i < iters && e == hipSuccess; ++i) e = spmv();
+  if (e == hipSuccess) e = hipEventRecord(e1, b->st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return hip_fail(e, "batch SpMV profile");
+  for (auto &c : live) c.done = 1;
+  MPP_HIP(hipMemcpy(b->cg, live.data(), sizeof(CgState) * live.size(), hipMemcpyHostToDevice));
+  *ms = (double)t / iters;
+  if (bytes) {
+    // per row: 81 values (columns computed from the grid) + p in + Ap out, at the matrix precision
+    const double w = fp32 ? 4.0 : 8.0;
+    double rows = 0;
+    for (int r = 0; r < b->n; ++r) rows += b->h[r].g.nrow_pad;
+    *bytes = rows * (81 * w + 2 * w);
+  }
   return MPPD_OK;
 }
 

@@ -95,6 +95,14 @@ inline bool alloc_fault_injected() {
   return v && std::atoi(v) != 0;
 }
 
+// Mixed-precision Newton (mppd_newton_mixed, batches with mixed_precision):
+// the linear solve of the last Newton iteration nr_max_its permits stops at
+// cg_rel_tol times this factor.  When the Newton ends there unconverged (the
+// reference's exit at the iteration cap), that correction is the last term of
+// the returned state and no later iteration corrects its error, so that
+// solve is taken closer to the exact Newton step.
+constexpr double kMixedLastStepTighten = 1e-2;
+
 // material_base with the derived constants of the reference's constructors
 // (mppd_api.hip); MPPD_ERR_ARG for an unknown law.
 int make_material(const material_base &in, material_base &m);

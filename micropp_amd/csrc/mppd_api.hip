@@ -1158,9 +1158,10 @@ int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6]
     const char *ff = getenv("MPPD_FORCE_FP64_FALLBACK");
     bool broke = ff && atoi(ff) != 0;
     for (int outer = 0; !broke; ++outer) {
+      // the last Newton iteration nr_max_its permits solves to a tighter target (kMixedLastStepTighten)
+      const double rel = its == p->nr_max_its - 1 ? p->cg_rel_tol * kMixedLastStepTighten : p->cg_rel_tol;
       MPP_HIP(launch_m_resid_ell(g, K, A->cols, A->vals, s->b, s->du, ml->kinv64, ml->kinv, ml->R, ml->X, ml->P,
-                                 ml->part, outer == 0, p->cg_rel_tol, p->cg_abs_tol, inner_tol, max_inner,
-                                 ml->state, s->st));
+                                 ml->part, outer == 0, rel, p->cg_abs_tol, inner_tol, max_inner, ml->state, s->st));
       MPP_HIP(hipMemcpyAsync(ml->state_h, ml->state, mixed_state_bytes(), hipMemcpyDeviceToHost, s->st));
       MPP_HIP(hipStreamSynchronize(s->st));
       int all_conv = 0;
@@ -1170,8 +1171,9 @@ int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6]
         break;
       }
       if (all_conv) break;
-      if (outer >= max_outer) {
+      if (outer >= max_outer) {  // short of the tolerance: redone in FP64 below, like a breakdown
         ++short_solves;
+        broke = true;
         break;
       }
       int inner = 0;
@@ -1187,7 +1189,10 @@ int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6]
       }
       MPP_HIP(launch_m_accum_ell(g, K, ml->X, s->du, s->st));
     }
-    if (broke) {  // the FP64 Jacobi-PCG of mppd_newton on the same system, from 0
+    // a breakdown, or no convergence within max_outer outer steps: the FP64
+    // Jacobi-PCG of mppd_newton on the same system, from 0, so that every
+    // linear solve meets the reference CG's tolerance
+    if (broke) {
       int cg_its = 0;
       double err = 0;
       if ((rc = cg_run(s, A, s->b, p->cg_abs_tol, p->cg_rel_tol, p->cg_max_its, &cg_its, &err))) return rc;

@@ -88,7 +88,8 @@ def test_no_oracle_in_product():
 
 
 def test_c_and_fortran_callers_build(lib, tmp_path):
-    """The C header compiles as C99 and links; a Fortran iso_c_binding caller of micropp3_* links."""
+    """The C header compiles as C99 and links; the Fortran drop-in modules libmaterial / libmicropp
+    (micropp_amd/fortran/) compile and a Fortran caller that uses them links."""
     import shutil
 
     libdir = os.path.join(ROOT, "micropp_amd")
@@ -98,5 +99,6 @@ def test_c_and_fortran_callers_build(lib, tmp_path):
     fc = next((c for c in ("/opt/rocm/bin/amdflang", shutil.which("flang") or "") if c and os.path.exists(c)), None)
     if fc is None:
         pytest.skip("no Fortran compiler")
-    subprocess.run([fc, "-o", str(tmp_path / "f_driver"), os.path.join(drv, "f_driver.f90"), "-module-dir",
-                    str(tmp_path), f"-L{libdir}", "-lmicropp_amd"], check=True)
+    mods = [os.path.join(libdir, "fortran", f) for f in ("libmaterial.f95", "libmicropp.f95")]
+    subprocess.run([fc, "-o", str(tmp_path / "f_driver")] + mods + [os.path.join(drv, "f_driver.f90"), "-module-dir",
+                    str(tmp_path), f"-L{libdir}", "-lmicropp_amd"], check=True, cwd=str(tmp_path))

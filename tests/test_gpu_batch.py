@@ -241,3 +241,25 @@ def test_spmv_shape_does_not_change_results(mp, chunk, order):
     b, _ = run_batch(mp, descs, 2, **KW)
     for (sa, ca, ia), (sb, cb, ib) in zip(a, b):
         assert np.array_equal(sa, sb) and np.array_equal(ca, cb) and np.array_equal(ia, ib)
+
+
+@pytest.mark.parametrize("its_with_A0", [0, 2])
+def test_device_newton_equals_host_newton(mp, monkeypatch, its_with_A0):
+    """The FP64 batch's Newton loop decides on the device (per-RVE state, a captured head graph, CG chunk graphs
+    and a captured finishing graph per Newton iteration, one host read of the states after it).  Against the
+    host-driven loop of the same batch (MPPD_BATCH_HOST_NEWTON=1, forced from the host): sigma, ctan, cost and
+    flags bit for bit, with fewer host synchronisations (at most about one per batched Newton iteration plus a
+    fixed few per solve)."""
+    descs = mixed_batch(mp)
+    kw = dict(KW, use_A0=1, its_with_A0=its_with_A0) if its_with_A0 else dict(KW)
+    steps = 2
+    rec_d, st_d = run_batch(mp, descs, steps, **kw)
+    monkeypatch.setenv("MPPD_BATCH_HOST_NEWTON", "1")
+    rec_h, st_h = run_batch(mp, descs, steps, **kw)
+    print(f"host syncs of the last homogenize: device loop {st_d[4]:.0f}, host loop {st_h[4]:.0f}; batched Newton "
+          f"iterations {st_d[5]:.0f} / {st_h[5]:.0f}", flush=True)
+    for t in range(steps):
+        for x, y in zip(rec_d[t], rec_h[t]):
+            assert np.array_equal(x, y), f"step {t}"
+    assert st_d[4] < st_h[4]
+    assert st_d[5] > 0

@@ -6,10 +6,12 @@ FP64 (the rules of mppd_newton_mixed).  The bar is the FP64 batch, itself equal
 bit for bit to micropp<3> instances and within 1e-6 of the oracle:
 - sigma within 1e-6 relative and ctan within 1e-4 (ctan is a forward difference
   over a 1e-8 strain step, so it multiplies solve-level differences by ~1e8 /
-  |sigma|) for the elastic, J2 and elastic-regime RVEs;
-- 1e-4 / 1e-2 for the damaged RVE, whose FD Newton tangent is nonsymmetric and
-  ill-conditioned, so that its state carries the linear solves' error (the
-  bars of the single-RVE mixed Newton);
+  |sigma|) for every RVE, elastic, J2 and damaged, whose FP64 Newton converged
+  at this and every earlier step (a linear solve whose FP32 inner CG breaks
+  down or stalls short of the tolerance within the outer-step cap is redone by
+  the FP64 CG, so every solve meets the same tolerance); 1e-4 / 1e-2 where a
+  Newton stopped at its cap unconverged, whose path may cross a singular
+  tangent (tests/test_oracle.py::test_capped_damage_newton_crosses_a_singular_tangent);
 - the same nonlinear (history) and convergence flags.
 """
 import numpy as np
@@ -39,11 +41,13 @@ def test_mixed_batch_matches_fp64_batch(mp, its_with_A0):
     r64, s64 = run_batch(mp, descs, steps, **kw)
     r32, s32 = run_batch(mp, descs, steps, mixed_precision=1, **kw)
     print(f"CG iterations queued: fp64 {s64[0]:.0f}, mixed {s32[0]:.0f}", flush=True)
+    conv_so_far = [True] * len(descs)
     for t in range(steps):
         sig64, ctan64, info64 = r64[t]
         sig32, ctan32, info32 = r32[t]
         for i, d in enumerate(descs):
-            ts, tc = (1e-4, 1e-2) if d["kind"] == "damage" else (1e-6, 1e-4)
+            conv_so_far[i] = conv_so_far[i] and bool(info64[i][2])
+            ts, tc = (1e-6, 1e-4) if conv_so_far[i] else (1e-4, 1e-2)
             es, ec = rel(sig32[i], sig64[i]), rel(ctan32[i], ctan64[i])
             print(f"step {t} rve {i} ({d['kind']}): info mixed {tuple(info32[i])} fp64 {tuple(info64[i])}; "
                   f"sigma {es:.2e} ctan {ec:.2e}", flush=True)

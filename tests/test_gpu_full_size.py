@@ -12,7 +12,9 @@ Each is checked three ways:
   1e-6 relative, the same nonlinear / convergence flags, the cost within the
   +-1 CG iteration per linear solve that a different reduction order allows;
 - the rest by bit equality with micropp<3> instances run one by one;
-- the batch with mixed-precision solves against the FP64 batch.
+- the batch with mixed-precision solves against the FP64 batch (1e-6 on
+  sigma where the FP64 Newton converges; looser where it stops at its cap,
+  see check_workload).
 The adjoint: ctan against the forward six-solve ctan and the oracle's
 calc_ctan_lin_fe_models, d sigma / d (E, nu) against central differences.
 
@@ -93,11 +95,20 @@ def single_run(mp, d, ramp, params):
     return rec
 
 
-def check_workload(mp, oracle, descs, ramps, params, orc_ix, single_ix, ts_mixed, label):
-    """The three checks of the module docstring; returns the FP64 batch records."""
+def check_workload(mp, oracle, descs, ramps, params, orc_ix, single_ix, label):
+    """The three checks of the module docstring; returns the FP64 batch records.
+
+    Mixed vs FP64: an RVE whose FP64 Newton converged at every step so far has sigma within 1e-6 and ctan within
+    1e-4 (ctan is a forward difference over a 1e-8 strain step).  Where a Newton stopped at nr_max_its
+    unconverged, the damage law's Newton path can cross a numerically singular tangent, where linear solutions
+    meeting the CG tolerance are not unique and sigma depends on which one a solver returns (shown in FP64 on
+    the oracle by tests/test_oracle.py::test_capped_damage_newton_crosses_a_singular_tangent: a
+    tolerance-admissible change of one solve moves sigma by more than 1e-3); there the bars are 1e-4 / 1e-2.
+    The nonlinear flags must agree; a converged flag may differ only at such a point."""
     for i in orc_ix:
         descs[i] = dict(descs[i], coupling=mp.FE_ONE_WAY)
-    print(f"{label}: {len(descs)} RVEs, sizes {[d['size'] for d in descs]}; oracle RVEs {orc_ix}, "
+    n = len(descs)
+    print(f"{label}: {n} RVEs, sizes {[d['size'] for d in descs]}; oracle RVEs {orc_ix}, "
           f"one-by-one RVEs {single_ix}", flush=True)
     with cf.ThreadPoolExecutor(max_workers=len(orc_ix)) as ex:
         futs = {i: ex.submit(oracle_run, oracle, descs[i], ramps[i], params) for i in orc_ix}
@@ -109,6 +120,7 @@ def check_workload(mp, oracle, descs, ramps, params, orc_ix, single_ix, ts_mixed
         print(f"{label}: one-by-one instances done", flush=True)
         orc = {i: f.result() for i, f in futs.items()}
     nl_total = 0
+    conv_so_far = np.ones(n, bool)
     for t in range(len(ramps[0])):
         sig, ctan, info = rec[t]
         sig32, ctan32, info32 = rec32[t]
@@ -126,12 +138,22 @@ def check_workload(mp, oracle, descs, ramps, params, orc_ix, single_ix, ts_mixed
             assert np.array_equal(sig[i], s1), f"step {t} rve {i}"
             assert np.array_equal(ctan[i], c1), f"step {t} rve {i}"
             assert tuple(int(v) for v in info[i]) == tuple(int(v) for v in f1), f"step {t} rve {i}"
-        es = [rel(sig32[i], sig[i]) for i in range(len(descs))]
-        ec = [rel(ctan32[i], ctan[i]) for i in range(len(descs))]
-        print(f"{label} step {t}: mixed vs FP64 batch, sigma max {max(es):.2e}, ctan max {max(ec):.2e}; "
-              f"nonlinear RVEs {int(info[:, 0].sum())}, converged {int(info[:, 2].sum())}", flush=True)
-        assert max(es) < ts_mixed
-        assert np.array_equal(info32[:, 0], info[:, 0]) and np.array_equal(info32[:, 2], info[:, 2])
+        conv_so_far &= info[:, 2] == 1
+        es = np.array([rel(sig32[i], sig[i]) for i in range(n)])
+        ec = np.array([rel(ctan32[i], ctan[i]) for i in range(n)])
+        print(f"{label} step {t}: nonlinear RVEs {int(info[:, 0].sum())}, converged {int(info[:, 2].sum())}; "
+              f"mixed vs FP64 sigma max {es.max():.2e}, ctan max {ec.max():.2e}", flush=True)
+        for i in range(n):
+            print(f"  rve {i:2d} n={descs[i]['size'] - 1}: nl {info[i][0]} conv fp64/mixed {info[i][2]}/{info32[i][2]} "
+                  f"(all steps {int(conv_so_far[i])}); cost {info[i][1]}/{info32[i][1]}; mixed sigma {es[i]:.1e} "
+                  f"ctan {ec[i]:.1e}", flush=True)
+        for i in range(n):
+            assert info32[i][0] == info[i][0], f"step {t} rve {i}: nonlinear flag"
+            if conv_so_far[i]:
+                assert info32[i][2] == info[i][2], f"step {t} rve {i}: converged flag"
+                assert es[i] < 1e-6 and ec[i] < 1e-4, f"step {t} rve {i}"
+            else:
+                assert es[i] < 1e-4 and ec[i] < 1e-2, f"step {t} rve {i}"
     return rec, nl_total
 
 
@@ -144,7 +166,7 @@ def test_batched_damage_real_sizes(mp, oracle):
     assert set(sizes) == {21, 31, 41, 51}
     orc_ix = [sizes.index(21), sizes.index(31)]
     single_ix = [sizes.index(51), sizes.index(41), [i for i, s in enumerate(sizes) if s == 21][1]]
-    rec, nl_total = check_workload(mp, oracle, descs, ramps, params, orc_ix, single_ix, 1e-6, "damage")
+    rec, nl_total = check_workload(mp, oracle, descs, ramps, params, orc_ix, single_ix, "damage")
     assert int(rec[-1][2][:, 0].sum()) >= len(descs) // 2, "the ramp must damage most RVEs"
 
 
@@ -152,7 +174,7 @@ def test_batched_j2_real_size(mp, oracle):
     """16 RVEs of 30^3 with a z-fibre, J2 matrix, 3 cyclic steps (the matrix yields from the first step).
     Oracle on two RVEs; one-by-one on three FE_FULL RVEs."""
     descs, ramps, params = bench.j2_fibre_workload(mp, 16, STEPS)
-    rec, nl_total = check_workload(mp, oracle, descs, ramps, params, [0, 1], [2, 3, 4], 1e-6, "j2")
+    rec, nl_total = check_workload(mp, oracle, descs, ramps, params, [0, 1], [2, 3, 4], "j2")
     assert int(rec[0][2][:, 0].sum()) == len(descs), "every RVE yields in the first step"
 
 

@@ -5,20 +5,19 @@ with the return mapping or damage update), u and norms in FP64 and solves
 each tangent system by FP64 defect correction around FP32 Jacobi-PCG, to the
 reference CG's metric evaluated in FP64.
 
-Bars:
-- J2 plasticity: sigma within 1e-6 relative of the FP64 path and of the
-  oracle, ctan within 1e-4 (ctan is a forward difference over a 1e-8 strain
-  step, so it multiplies solve-level differences by ~1e8 / |sigma|; the FP64
-  path is held to 1e-6 by its own parity tests).
-- Damage: sigma within 1e-4, ctan within 1e-2.  With the reference's damage
-  law the FD tangent of a damaged RVE is nonsymmetric and ill-conditioned and
-  the reference Newton stops at nr_max_its without converging, so the
-  returned state carries the linear solves' error (about the condition
-  number times the CG tolerance: up to ~4e-5 on this load path, ~1e-13 at
-  the points that stay well conditioned), which two correct solvers with
-  different rounding do not share; the FP64 path agrees with the oracle more
-  closely only because both use the same arithmetic order.  The load path
-  keeps every reference CG below its iteration cap (checked on the oracle).
+Bars: sigma within 1e-6 relative of the FP64 path and of the oracle, ctan
+within 1e-4 (ctan is a forward difference over a 1e-8 strain step, so it
+multiplies solve-level differences by ~1e8 / |sigma|; the FP64 path is held
+to 1e-6 by its own parity tests), wherever the FP64 Newton converged at this
+and every earlier step.  Every linear solve meets the reference CG's
+tolerance: one whose FP32 inner CG breaks down or stalls short of it within
+max_outer outer steps is redone in FP64.  Where the FP64 Newton stops at
+nr_max_its unconverged (the damage case's second step), its path crosses a
+numerically singular tangent, where linear solutions meeting the tolerance
+are not unique and sigma depends on which one a solver returns: the FP64
+control tests/test_oracle.py::test_capped_damage_newton_crosses_a_singular_tangent
+restates that very point and moves sigma by more than 1e-3 with a
+tolerance-admissible change of one solve; there the bars are 1e-4 / 1e-2.
 """
 import numpy as np
 import pytest
@@ -66,16 +65,19 @@ def test_micropp_mixed_newton(mp, oracle, case):
     rng = np.random.default_rng(3)
     d = [rng.uniform(-1, 1, 6) * AMP[kind], rng.uniform(-1, 1, 6) * AMP[kind] * 0.7]
     steps, nl = (3, 0) if kind == "plastic" else (2, 0)
-    tol_s, tol_c = (1e-6, 1e-4) if kind == "plastic" else (1e-4, 1e-2)
+    conv_so_far = [True, True]
     for t in range(steps):
         for m in (mx, fp, orc):
             for g in range(2):
                 m.set_strain(g, d[g] * (t + 1) / 3)
             m.homogenize()
         for g in range(2):
+            conv_so_far[g] = conv_so_far[g] and fp.has_converged(g)
+            tol_s, tol_c = (1e-6, 1e-4) if conv_so_far[g] else (1e-4, 1e-2)
             s_mx, s_fp, s_or = mx.get_stress(g), fp.get_stress(g), orc.get_stress(g)
             c_mx, c_fp = mx.get_ctan(g), fp.get_ctan(g)
-            print(f"{case} step {t} gp {g}: nl {mx.is_non_linear(g)} cost mixed {mx.get_cost(g)} fp64 "
+            print(f"{case} step {t} gp {g}: nl {mx.is_non_linear(g)} converged fp64 {fp.has_converged(g)} mixed "
+                  f"{mx.has_converged(g)}; cost mixed {mx.get_cost(g)} fp64 "
                   f"{fp.get_cost(g)}; sigma mixed-fp64 {rel(s_mx, s_fp):.2e} mixed-oracle {rel(s_mx, s_or):.2e}; "
                   f"ctan mixed-fp64 {rel(c_mx, c_fp):.2e}", flush=True)
             assert rel(s_mx, s_fp) < tol_s and rel(s_mx, s_or) < tol_s
@@ -178,10 +180,10 @@ def test_newton_mixed_fp64_fallback_equals_fp64(mp, monkeypatch):
     assert np.array_equal(u32.numpy(), u64.numpy())
 
 
-def test_newton_mixed_outer_cap_counts_short_solves(mp):
-    """max_outer = 1 with a 1e-12 CG tolerance: each linear solve stops after one FP32 inner solve (about 1e-5
-    below its start), short of the tolerance, and is counted in stats[3]; the Newton state stays finite and
-    near the FP64 one."""
+def test_newton_mixed_outer_cap_redoes_short_solves_in_fp64(mp):
+    """max_outer = 1 with a 1e-12 CG tolerance: each linear solve is still short of the tolerance after one FP32
+    inner solve (about 1e-5 below its start), so it is redone by the FP64 CG from 0 -- counted in stats[3] and
+    as a fallback in stats[2] -- and the result is the FP64 path's, bit for bit."""
     mats = [mp.material(*s) for s in LAWS["plastic"]]
     r = mp.Rve(13, mp.MIC3D_RANDOM_SPHERES, (11.0, 6.0, 0.05, 0.2), mats)
     s = mp.Solver(r)
@@ -192,8 +194,8 @@ def test_newton_mixed_outer_cap_counts_short_solves(mp):
     out, stats = s.newton_mixed(eps, u32, max_outer=1, **kw)
     e = rel(s.ave_stress(u32), s.ave_stress(u64))
     print(f"out {out}, stats {stats}, sigma rel {e:.2e}", flush=True)
-    assert stats[3] == 1 and stats[0] == 1 and stats[2] == 0
-    assert np.isfinite(u32.numpy()).all() and e < 1e-3
+    assert stats[3] == 1 and stats[0] == 1 and stats[2] == 1
+    assert np.array_equal(u32.numpy(), u64.numpy()) and e == 0.0
 
 
 def test_newton_mixed_rejects_bad_arguments(mp):
