@@ -95,6 +95,36 @@ inline bool alloc_fault_injected() {
   return v && std::atoi(v) != 0;
 }
 
+// Two-level preconditioner of the multi-RHS inner CG (mixed.hip k_c_*): the
+// Jacobi weights plus a Galerkin coarse space of trilinear hats on a coarse
+// grid aligned with the fine one (h fine elements per coarse element and
+// side).  Only interior coarse nodes carry unknowns (the boundary is
+// Dirichlet); coarse DOF q * 3 + d of interior coarse node q = ((cz - 1) my +
+// (cy - 1)) mx + (cx - 1).
+struct CGrid {
+  int hx, hy, hz;  // fine elements per coarse element
+  int mx, my, mz;  // interior coarse nodes per side
+  int nq, nc;      // interior coarse nodes, coarse DOFs (3 nq)
+};
+// device buffers of the coarse level of one K-field system
+struct CoarseDev {
+  CGrid cg;
+  const float *ainv;  // nc x nc: the inverse of the Galerkin coarse matrix (symmetric)
+  float *t1;          // x-restricted residual: 3 K nz ny mx floats
+  float *rc;          // restricted residual, nc x K
+  float *ecp;         // split partials of the coarse solve, kCoarseSplit x nc x K
+  float *ec;          // coarse correction, nc x K
+  float *z;           // the preconditioned residual z, in R's layout
+  double *prz;        // partials of (r, z) at the inner CG's start: 3 K x (vector blocks)
+};
+constexpr int kCoarseSplit = 16;    // column splits of the coarse solve (one partial each)
+constexpr int kCoarseMaxLine = 256;  // nodes per side a coarse level allows
+// coarse.hip: the coarse level of a K-field system on the RVE grid g (per-phase
+// element stiffness ke_d, phase map et_d); *out = nullptr when the grid has
+// none (or MPPD_COARSE=0 in the environment)
+int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipStream_t st, CoarseDev **out);
+void coarse_free(CoarseDev *c);
+
 // Mixed-precision Newton (mppd_newton_mixed, batches with mixed_precision):
 // the linear solve of the last Newton iteration nr_max_its permits stops at
 // cg_rel_tol times this factor.  When the Newton ends there unconverged (the
@@ -185,8 +215,20 @@ size_t mixed_adj_partials(const Grid &g, int ns);
 hipError_t launch_m_adj(const Grid &g, int K, const double *bmat, const material_base *mats_d, const int *et,
                         const double *U, int s0, int ns, double *part, double *ctan, double *dmat, hipStream_t st);
 // one inner CG iteration; R X P AP in the 4-field-group layout
+// Two-level preconditioner: coarse (may be null: Jacobi alone).  start:
+// after launch_m_resid, the inner CG's start direction P = z = D^-1 R +
+// P_c A_c^-1 P_c^T R and the state's (r, z); launch_m_iter with coarse applies
+// it in every iteration.
+hipError_t launch_c_start(const Grid &g, int K, const CoarseDev &c, const float *kinv, const float *R, float *P,
+                          void *state, hipStream_t st);
+// the full symmetric FP32 copy of the lower triangle (column-major) of the
+// FP64 inverse a64 (n x n): ainv[i * n + j]
+hipError_t launch_c_sym32(int n, const double *a64, float *ainv, hipStream_t st);
 hipError_t launch_m_iter(const Grid &g, int K, const float *A, const float *kinv, float *R, float *X, float *P,
-                         float *AP, double *part, void *state, int parity, hipStream_t st);
+                         float *AP, double *part, void *state, int parity, hipStream_t st,
+                         const CoarseDev *coarse = nullptr);
+// blocks per component of the vector kernels (the partial tables' stride)
+size_t coarse_vec_blocks(const Grid &g);
 hipError_t launch_m_spmv_only(const Grid &g, int K, const float *A, const float *P, float *AP, double *part,
                               const void *state, hipStream_t st);
 hipError_t launch_m_accum(const Grid &g, int K, const float *X, double *U, hipStream_t st);

@@ -772,14 +772,296 @@ __device__ __forceinline__ MStep m_step(const MState *cur, int r, double zz, dou
   return o;
 }
 
+// ---------------------------------------------------------------------------
+// Two-level preconditioner: z = D^-1 r + P_c A_c^-1 P_c^T r, with P_c the
+// trilinear hats of the interior nodes of a coarse grid aligned with the fine
+// one (cg.hx fine elements per coarse element along x, ...), A_c = P_c^T A
+// P_c (Galerkin; built on the host, inverted once, kept as a dense FP32
+// inverse).  Per application: restriction along x (k_c_rx), then along y and
+// z (k_c_ryz), the dense coarse solve in kCoarseSplit column splits
+// (k_c_gemv) and their sum (k_c_gemv_sum), then z with the prolongation
+// (k_c_zdot), which k_m_pupd reads.  Restriction and prolongation use the
+// same hat weights (their products are rounded in different orders, a
+// difference at FP32 rounding level), so the preconditioner is symmetric to
+// that level.  All sums run in fixed orders.  Every thread's loads are
+// independent of each other, so they are in flight together: these kernels
+// are short and latency decides their time.
+// ---------------------------------------------------------------------------
+
+// hat of interior coarse node q (1 .. m) at fine index i along one side
+__device__ __forceinline__ float hat(int i, int q, int h) {
+  const int dd = abs(i - q * h);
+  return dd < h ? 1.0f - (float)dd / (float)h : 0.0f;
+}
+
+// weights and interior coarse indices (0-based) of the two coarse nodes at
+// and above fine index i along one side; a node outside 1 .. m gets weight 0
+// and index 0 (a valid address)
+__device__ __forceinline__ void hat_pair(int i, int h, int m, float (&w)[2], int (&q)[2]) {
+  const int q0 = i / h;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int c = q0 + a;
+    const bool in = c >= 1 && c <= m;
+    w[a] = in ? hat(i, c, h) : 0.0f;
+    q[a] = in ? c - 1 : 0;
+  }
+}
+
+// One fine index along one side: its two coarse nodes (0-based interior
+// indices, clamped) and their hat weights (hat_pair)
+struct AxisW {
+  int q[2];
+  float w[2];
+};
+
+// The three sides' tables of a block (x: nx entries, then y, then z), filled
+// by the whole block; every thread must reach the barrier.
+__device__ __forceinline__ void axis_tables(const Grid &g, const CGrid &cg, AxisW (*tab)[kCoarseMaxLine]) {
+  for (int t = threadIdx.x; t < g.nx + g.ny + g.nz; t += blockDim.x) {
+    const int a = t < g.nx ? 0 : t < g.nx + g.ny ? 1 : 2;
+    const int i = a == 0 ? t : a == 1 ? t - g.nx : t - g.nx - g.ny;
+    const int h = a == 0 ? cg.hx : a == 1 ? cg.hy : cg.hz, m = a == 0 ? cg.mx : a == 1 ? cg.my : cg.mz;
+    AxisW e;
+    hat_pair(i, h, m, e.w, e.q);
+    tab[a][i] = e;
+  }
+  __syncthreads();
+}
+
+// P_c e at a fine node whose sides' table entries are ax, ay, az, DOF
+// direction d, fields 4 grp .. 4 grp + 3: the 8 coarse nodes around it, all
+// loaded at once (zero weights outside)
+template <int K>
+__device__ __forceinline__ float4 prolong(const CGrid &cg, const float4 *ec4, const AxisW &ax, const AxisW &ay,
+                                          const AxisW &az, int d, int grp) {
+  constexpr int KG = K / 4;
+  float4 e[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int q = (az.q[t >> 2] * cg.my + ay.q[(t >> 1) & 1]) * cg.mx + ax.q[t & 1];
+    e[t] = ec4[(size_t)(q * 3 + d) * KG + grp];
+  }
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const float w = ax.w[t & 1] * (ay.w[(t >> 1) & 1] * az.w[t >> 2]);
+    acc.x += w * e[t].x;
+    acc.y += w * e[t].y;
+    acc.z += w * e[t].z;
+    acc.w += w * e[t].w;
+  }
+  return acc;
+}
+
+// Restriction along x: one thread per (component c = d KG + grp, interior
+// x-line (j, k), interior coarse x node cx): T1[((c nz + k) ny + j) mx + cx -
+// 1] = sum_i hat_cx(i) R(i, j, k) over the 2 hx - 1 nodes where the hat is
+// non-zero.  Lines on the y or z boundary are not needed (the y / z hats
+// vanish there).
+template <int K>
+__global__ void __launch_bounds__(256) k_c_rx(Grid g, CGrid cg, int NP, const float4 *R, float4 *T1,
+                                              const MState *cur) {
+  constexpr int KG = K / 4;
+  if (cur->all_done) return;
+  const int ly = g.ny - 2, lz = g.nz - 2;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 3 * KG * ly * lz * cg.mx) return;
+  const int cx = t % cg.mx + 1;
+  int rest = t / cg.mx;
+  const int j = rest % ly + 1;
+  rest /= ly;
+  const int k = rest % lz + 1, c = rest / lz;
+  const float4 *line = R + (size_t)c * NP + ((size_t)k * g.ny + j) * g.nx;
+  const int i0 = (cx - 1) * cg.hx + 1, n = 2 * cg.hx - 1;
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 8
+  for (int s = 0; s < n; ++s) {
+    const float w = hat(i0 + s, cx, cg.hx);
+    const float4 v = line[i0 + s];
+    acc.x += w * v.x;
+    acc.y += w * v.y;
+    acc.z += w * v.z;
+    acc.w += w * v.w;
+  }
+  T1[((size_t)(c * g.nz + k) * g.ny + j) * cg.mx + (cx - 1)] = acc;
+}
+
+// Restriction along y and z: one wave per (interior coarse node q, component
+// c); its lanes take the (2 hy - 1)(2 hz - 1) x-restricted lines under q's
+// hats in turn, then the wave adds them up.
+template <int K>
+__global__ void __launch_bounds__(256) k_c_ryz(Grid g, CGrid cg, const float4 *T1, float4 *rc4, const MState *cur) {
+  constexpr int KG = K / 4;
+  if (cur->all_done) return;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (wave >= cg.nq * 3 * KG) return;  // whole wave
+  const int c = wave % (3 * KG), q = wave / (3 * KG), d = c / KG, grp = c % KG;
+  const int cx = q % cg.mx + 1, cy = (q / cg.mx) % cg.my + 1, cz = q / (cg.mx * cg.my) + 1;
+  const int wy = 2 * cg.hy - 1, npair = wy * (2 * cg.hz - 1);
+  const int j0 = (cy - 1) * cg.hy + 1, k0 = (cz - 1) * cg.hz + 1;
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int s = lane; s < npair; s += 64) {
+    const int j = j0 + s % wy, k = k0 + s / wy;
+    const float w = hat(j, cy, cg.hy) * hat(k, cz, cg.hz);
+    const float4 v = T1[((size_t)(c * g.nz + k) * g.ny + j) * cg.mx + (cx - 1)];
+    acc.x += w * v.x;
+    acc.y += w * v.y;
+    acc.z += w * v.z;
+    acc.w += w * v.w;
+  }
+  const float4 sum = make_float4(wave_sum_dpp(acc.x), wave_sum_dpp(acc.y), wave_sum_dpp(acc.z), wave_sum_dpp(acc.w));
+  if (lane == 0) rc4[(size_t)(q * 3 + d) * KG + grp] = sum;
+}
+
+// The coarse solve e = A_c^-1 r_c (nc x nc times nc x K), split over the
+// columns: block (row tile of 64, split s) sums the columns of split s, each
+// wave a quarter of them (lane = row; the symmetric inverse is read by column,
+// one coalesced line per column), the waves combined in order, into ecp;
+// k_c_gemv_sum adds the splits.  (Summing them in the last block to finish,
+// counted with an atomic, measured 9x slower: the device-scope fences it needs
+// write the L2 back.)
+template <int K>
+__global__ void __launch_bounds__(256) k_c_gemv(CGrid cg, const float *__restrict__ ainv,
+                                                const float *__restrict__ rc, float *__restrict__ ecp,
+                                                const MState *cur) {
+  __shared__ float red[3][64][K];
+  if (cur->all_done) return;
+  const int nc = cg.nc, lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int row = blockIdx.x * 64 + lane, split = blockIdx.y;
+  const int chunk = (nc + kCoarseSplit - 1) / kCoarseSplit;
+  const int c0 = split * chunk, c1 = min(nc, c0 + chunk);
+  const int rowc = min(row, nc - 1);  // loads of rows past nc are clamped, their sums not stored
+  float acc[K];
+#pragma unroll
+  for (int f = 0; f < K; ++f) acc[f] = 0.0f;
+#pragma unroll 4
+  for (int col = c0 + w; col < c1; col += 4) {
+    const float a = ainv[(size_t)col * nc + rowc];
+    const float *x = rc + (size_t)col * K;
+#pragma unroll
+    for (int f = 0; f < K; ++f) acc[f] += a * x[f];
+  }
+  if (w > 0)
+#pragma unroll
+    for (int f = 0; f < K; ++f) red[w - 1][lane][f] = acc[f];
+  __syncthreads();
+  if (w == 0 && row < nc)
+#pragma unroll
+    for (int f = 0; f < K; ++f)
+      ecp[((size_t)split * nc + row) * K + f] = acc[f] + red[0][lane][f] + red[1][lane][f] + red[2][lane][f];
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) k_c_gemv_sum(CGrid cg, const float *ecp, float *ec, const MState *cur) {
+  if (cur->all_done) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const size_t n = (size_t)cg.nc * K;
+  if (t >= (int)n) return;
+  float v[kCoarseSplit];
+#pragma unroll
+  for (int p = 0; p < kCoarseSplit; ++p) v[p] = ecp[(size_t)p * n + t];
+  float s = 0.0f;
+#pragma unroll
+  for (int p = 0; p < kCoarseSplit; ++p) s += v[p];
+  ec[t] = s;
+}
+
+// z = D^-1 r + P_c e into Z (START: into P, the inner CG's first direction)
+// and the partials of (r, z) in k_m_upd's layout pr[(field * 3 + d) *
+// gridDim.x + block] (grid as k_m_upd).  Loads are clamped to valid nodes
+// (their contributions dropped), so every load of a thread is in flight at
+// once.
+template <int K, bool START>
+__global__ void __launch_bounds__(kRedThreads) k_c_zdot(Grid g, CGrid cg, int NP, const float *kinv, const float4 *R,
+                                                        const float4 *ec4, float4 *Z, double *pr, const MState *cur) {
+  __shared__ double lds[4 * 4];
+  constexpr int KG = K / 4;
+  const int c = blockIdx.y, grp = c % KG, d = c / KG, r0 = grp * 4;
+  if (!START) {
+    if (cur->all_done) return;
+    if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole block
+  }
+  __shared__ AxisW tab[3][kCoarseMaxLine];
+  axis_tables(g, cg, tab);
+  constexpr int J = kVecNodes / 256;
+  const int n0 = blockIdx.x * kVecNodes + threadIdx.x;
+  // the node's (i, j, k), stepped by 256 nodes per round (nodes past the
+  // grid, k >= nz, read the last plane's entries; their part is dropped)
+  int i = n0 % g.nx, j = (n0 / g.nx) % g.ny, k = n0 / (g.nx * g.ny);
+  const int di = 256 % g.nx, dl = 256 / g.nx;
+  double v[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int jn = 0; jn < J; ++jn) {
+    const int n = n0 + jn * 256;
+    const int nl = min(n, NP - 1);
+    const size_t o = (size_t)c * NP + nl;
+    const float kv = kinv[(size_t)d * NP + nl];
+    const float4 r = R[o];
+    const float4 e = prolong<K>(cg, ec4, tab[0][i], tab[1][j], tab[2][min(k, g.nz - 1)], d, grp);
+    const bool real = n < g.nn;  // padding nodes have r = 0 and no coarse part
+    i += di;
+    int carry = dl;
+    if (i >= g.nx) {
+      i -= g.nx;
+      ++carry;
+    }
+    j += carry;
+    while (j >= g.ny) {
+      j -= g.ny;
+      ++k;
+    }
+    const float4 z = make_float4(kv * r.x + (real ? e.x : 0.0f), kv * r.y + (real ? e.y : 0.0f),
+                                 kv * r.z + (real ? e.z : 0.0f), kv * r.w + (real ? e.w : 0.0f));
+    if (n < NP) {
+      Z[o] = z;
+      v[0] += (double)r.x * z.x;
+      v[1] += (double)r.y * z.y;
+      v[2] += (double)r.z * z.z;
+      v[3] += (double)r.w * z.w;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double w = wave_sum_dpp(v[q]);
+    if ((threadIdx.x & 63) == 0) lds[(threadIdx.x >> 6) * 4 + q] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int q = threadIdx.x;
+    pr[((size_t)(r0 + q) * 3 + d) * gridDim.x + blockIdx.x] = lds[q] + lds[4 + q] + lds[8 + q] + lds[12 + q];
+  }
+}
+
+// start: the state's (r, z) of every field from k_c_zdot<K, true>'s partials
+template <int K>
+__global__ void __launch_bounds__(kRedThreads) k_c_rzfix(int nbx, const double *prz, MState *s0) {
+  const size_t row = (size_t)3 * nbx;
+  const double rz = rows_sum<K>(prz, row, (int)row);
+  if (threadIdx.x < K) s0->rz[threadIdx.x] = rz;
+}
+
+// the full symmetric FP32 inverse from the lower triangle (column-major) of
+// the FP64 one: ainv[i * n + j] = a64[max(i, j) + min(i, j) * n]
+__global__ void __launch_bounds__(256) k_c_sym32(int n, const double *a64, float *ainv) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (size_t)n * n) return;
+  const int i = (int)(t / n), j = (int)(t % n);
+  const int hi = max(i, j), lo = min(i, j);
+  ainv[t] = (float)a64[(size_t)hi + (size_t)lo * n];
+}
+
 // k_m_pupd: X += alpha P; P = kinv R + beta P for the fields still running;
 // the next CG state.  Every block re-sums the (z, z) and (r, z) partials of
 // its four fields in the same fixed order (so all blocks agree bit for bit),
 // and block (0, 0) sums all fields and writes the state of the next iteration
 // (beta, ||z||, stop flags, iteration count).  Grid as k_m_upd.
-template <int K>
-__global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kinv, const float4 *R, float4 *P,
-                                                        float4 *X, const double *pz, const MState *cur,
+// COARSE (two-level preconditioner): z is read from Z (k_c_zdot) instead of
+// being formed from kinv and R.
+template <int K, bool COARSE>
+__global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kinv, const float4 *R, const float4 *Z,
+                                                        float4 *P, float4 *X, const double *pz, const MState *cur,
                                                         MState *nxt) {
   constexpr int KG = K / 4;
   __shared__ double beta_s[kMaxRhs];
@@ -806,8 +1088,13 @@ __global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kin
   for (int jn = 0; jn < J; ++jn) {
     const int n = min(n0 + jn * 256, NP - 1);
     const size_t o = (size_t)c * NP + n;
-    kvv[jn] = kinv[(size_t)d * NP + n];
-    rvv[jn] = R[o];
+    if constexpr (COARSE) {
+      kvv[jn] = 0.0f;
+      rvv[jn] = Z[o];  // z itself
+    } else {
+      kvv[jn] = kinv[(size_t)d * NP + n];
+      rvv[jn] = R[o];
+    }
     pvv[jn] = P[o];
     xvv[jn] = X[o];
   }
@@ -870,10 +1157,11 @@ __global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kin
     x.z += al.z * p.z;
     x.w += al.w * p.w;
     X[o] = x;
-    if (lv[0]) p.x = kv * r.x + be[0] * p.x;
-    if (lv[1]) p.y = kv * r.y + be[1] * p.y;
-    if (lv[2]) p.z = kv * r.z + be[2] * p.z;
-    if (lv[3]) p.w = kv * r.w + be[3] * p.w;
+    const float4 z = COARSE ? r : make_float4(kv * r.x, kv * r.y, kv * r.z, kv * r.w);
+    if (lv[0]) p.x = z.x + be[0] * p.x;
+    if (lv[1]) p.y = z.y + be[1] * p.y;
+    if (lv[2]) p.z = z.z + be[2] * p.z;
+    if (lv[3]) p.w = z.w + be[3] * p.w;
     P[o] = p;
   }
 }
@@ -1323,8 +1611,25 @@ hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke,
   return hipGetLastError();
 }
 
+// restriction and coarse solve of R into c.ec (state cur: skipped once every field has stopped)
+template <int KK>
+static void coarse_solve(const Grid &g, const MLaunch &s, const CoarseDev &c, const float *R, const MState *cur,
+                         hipStream_t st) {
+  constexpr int KG = KK / 4;
+  const CGrid &cg = c.cg;
+  const size_t rx_threads = (size_t)3 * KG * (g.ny - 2) * (g.nz - 2) * cg.mx;
+  hipLaunchKernelGGL(k_c_rx<KK>, dim3(nblk(rx_threads, 256)), dim3(256), 0, st, g, cg, s.NP,
+                     reinterpret_cast<const float4 *>(R), reinterpret_cast<float4 *>(c.t1), cur);
+  hipLaunchKernelGGL(k_c_ryz<KK>, dim3(nblk((size_t)cg.nq * 3 * KG, 4)), dim3(256), 0, st, g, cg,
+                     reinterpret_cast<const float4 *>(c.t1), reinterpret_cast<float4 *>(c.rc), cur);
+  hipLaunchKernelGGL(k_c_gemv<KK>, dim3(nblk(cg.nc, 64), kCoarseSplit), dim3(256), 0, st, cg, c.ainv, c.rc, c.ecp,
+                     cur);
+  hipLaunchKernelGGL(k_c_gemv_sum<KK>, dim3(nblk((size_t)cg.nc * KK, 256)), dim3(256), 0, st, cg, c.ecp, c.ec,
+                     cur);
+}
+
 hipError_t launch_m_iter(const Grid &g, int K, const float *A, const float *kinv, float *R, float *X, float *P,
-                         float *AP, double *part, void *state, int parity, hipStream_t st) {
+                         float *AP, double *part, void *state, int parity, hipStream_t st, const CoarseDev *coarse) {
   const MLaunch s = mshape(g);
   MState *ms = static_cast<MState *>(state);
   MState *cur = ms + (parity & 1);
@@ -1334,18 +1639,53 @@ hipError_t launch_m_iter(const Grid &g, int K, const float *A, const float *kinv
   double *pz = part + (size_t)K * s.nsl;         // 3K x nbx
   double *pr = pz + (size_t)3 * K * nbx;         // 3K x nbx
   const dim3 vgrid(nbx, 3 * K / 4);
+  const CGrid cg = coarse ? coarse->cg : CGrid{};
   MPP_K_DISPATCH(K, {
     hipLaunchKernelGGL(k_m_spmv<KK>, dim3(8 * s.pers), dim3(16 * KK), 0, st, g, s.NP, s.nsl, s.pers,
                        reinterpret_cast<const float4 *>(A), reinterpret_cast<const float4 *>(P),
                        reinterpret_cast<float4 *>(AP), part1, cur);
     hipLaunchKernelGGL(k_m_upd<KK>, vgrid, dim3(kRedThreads), 0, st, s.NP, s.nsl, part1, kinv,
                        reinterpret_cast<const float4 *>(AP), reinterpret_cast<float4 *>(R), pz, pr, cur);
-    hipLaunchKernelGGL(k_m_pupd<KK>, vgrid, dim3(kRedThreads), 0, st, s.NP, kinv,
-                       reinterpret_cast<const float4 *>(R), reinterpret_cast<float4 *>(P),
-                       reinterpret_cast<float4 *>(X), pz, cur, nxt);
+    if (coarse) {
+      // z and (r, z) with the coarse correction; its partials replace k_m_upd's Jacobi (r, z) ones
+      coarse_solve<KK>(g, s, *coarse, R, cur, st);
+      hipLaunchKernelGGL((k_c_zdot<KK, false>), vgrid, dim3(kRedThreads), 0, st, g, cg, s.NP, kinv,
+                         reinterpret_cast<const float4 *>(R), reinterpret_cast<const float4 *>(coarse->ec),
+                         reinterpret_cast<float4 *>(coarse->z), pr, cur);
+      hipLaunchKernelGGL((k_m_pupd<KK, true>), vgrid, dim3(kRedThreads), 0, st, s.NP, kinv,
+                         reinterpret_cast<const float4 *>(R), reinterpret_cast<const float4 *>(coarse->z),
+                         reinterpret_cast<float4 *>(P), reinterpret_cast<float4 *>(X), pz, cur, nxt);
+    } else {
+      hipLaunchKernelGGL((k_m_pupd<KK, false>), vgrid, dim3(kRedThreads), 0, st, s.NP, kinv,
+                         reinterpret_cast<const float4 *>(R), nullptr, reinterpret_cast<float4 *>(P),
+                         reinterpret_cast<float4 *>(X), pz, cur, nxt);
+    }
   });
   return hipGetLastError();
 }
+
+hipError_t launch_c_start(const Grid &g, int K, const CoarseDev &c, const float *kinv, const float *R, float *P,
+                          void *state, hipStream_t st) {
+  const MLaunch s = mshape(g);
+  MState *s0 = static_cast<MState *>(state);
+  const int nbx = (int)nblk(s.NP, kVecNodes);
+  const dim3 vgrid(nbx, 3 * K / 4);
+  MPP_K_DISPATCH(K, {
+    coarse_solve<KK>(g, s, c, R, s0, st);
+    hipLaunchKernelGGL((k_c_zdot<KK, true>), vgrid, dim3(kRedThreads), 0, st, g, c.cg, s.NP, kinv,
+                       reinterpret_cast<const float4 *>(R), reinterpret_cast<const float4 *>(c.ec),
+                       reinterpret_cast<float4 *>(P), c.prz, s0);
+    hipLaunchKernelGGL(k_c_rzfix<KK>, dim3(1), dim3(kRedThreads), 0, st, nbx, c.prz, s0);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_c_sym32(int n, const double *a64, float *ainv, hipStream_t st) {
+  hipLaunchKernelGGL(k_c_sym32, dim3(nblk((size_t)n * n, 256)), dim3(256), 0, st, n, a64, ainv);
+  return hipGetLastError();
+}
+
+size_t coarse_vec_blocks(const Grid &g) { return nblk(mshape(g).NP, kVecNodes); }
 
 hipError_t launch_m_spmv_only(const Grid &g, int K, const float *A, const float *P, float *AP, double *part,
                               const void *state, hipStream_t st) {

@@ -702,6 +702,7 @@ constexpr int kMaxRhsFields = 16;  // largest K of an msolver
 struct mppd_msolver {
   mppd_rve *r = nullptr;
   int K = 0;
+  CoarseDev *coarse = nullptr;  // two-level preconditioner of the inner CG (null: Jacobi alone)
   hipStream_t st = nullptr;
   double *ke = nullptr;         // phase table 3 x 576 (FP64)
   material_base *mats_d = nullptr;
@@ -759,7 +760,11 @@ int mppd_msolver_create(mppd_msolver **out, mppd_rve *r, int nrhs) {
     // (U serves as FP64 scratch for the Jacobi weights: 3 NP <= 3 K NP doubles)
     if (e == hipSuccess) e = launch_m_asm32(g, r->et, m->ke, 1, m->U, m->A, m->kinv, m->st);
     if (e == hipSuccess) e = hipStreamSynchronize(m->st);
-    if (e != hipSuccess) rc = hip_fail(e, "msolver set-up");
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "msolver set-up");
+      break;
+    }
+    rc = coarse_setup(g, r->et, m->ke, nrhs, m->st, &m->coarse);
   } while (0);
   if (rc) {
     mppd_msolver_destroy(m);
@@ -778,6 +783,7 @@ int mppd_msolver_destroy(mppd_msolver *m) {
     dfree(p);
   for (void *p : {m->state_h, (void *)m->eps_h, (void *)m->sig_h, (void *)m->act_h, (void *)m->adj_h})
     if (p) (void)hipHostFree(p);
+  coarse_free(m->coarse);
   if (m->st) (void)hipStreamDestroy(m->st);
   delete m;
   return MPPD_OK;
@@ -795,7 +801,8 @@ int mppd_msolver_destroy(mppd_msolver *m) {
 // field's ||z|| is finite at the end (no breakdown).
 static int mixed_inner(const Grid &g, int K, const float *A, const float *kinv, float *R, float *X, float *P,
                        float *AP, double *part, void *state, void *state_h, int max_inner, hipStream_t st,
-                       int *its_out, bool *finite = nullptr, bool stop_on_stall = false) {
+                       int *its_out, bool *finite = nullptr, bool stop_on_stall = false,
+                       const CoarseDev *coarse = nullptr) {
   int queued = 0, its = 0, done = 0, its_prev = 0;
   int chunk = 32;
   std::vector<double> pn_prev(K, 0.0);
@@ -806,7 +813,8 @@ static int mixed_inner(const Grid &g, int K, const float *A, const float *kinv, 
   while (!done) {
     const int nq = std::min(chunk, max_inner - queued);
     if (nq <= 0) break;
-    for (int i = 0; i < nq; ++i) MPP_HIP(launch_m_iter(g, K, A, kinv, R, X, P, AP, part, state, queued + i, st));
+    for (int i = 0; i < nq; ++i)
+      MPP_HIP(launch_m_iter(g, K, A, kinv, R, X, P, AP, part, state, queued + i, st, coarse));
     queued += nq;
     MPP_HIP(hipMemcpyAsync(state_h, state, mixed_state_bytes(), hipMemcpyDeviceToHost, st));
     MPP_HIP(hipStreamSynchronize(st));
@@ -869,12 +877,13 @@ static int msolver_run(mppd_msolver *m, int nadj, int n, const double *eps_h, co
   for (;;) {
     MPP_HIP(launch_m_resid(g, K, m->r->et, m->ke, m->U, F, m->kinv, m->R, m->X, m->P, m->part, outer == 0,
                            p->rel_tol, p->abs_tol, p->inner_tol, p->max_inner_its, m->state, m->st));
+    if (m->coarse) MPP_HIP(launch_c_start(g, K, *m->coarse, m->kinv, m->R, m->P, m->state, m->st));
     MPP_HIP(hipMemcpyAsync(m->state_h, m->state, mixed_state_bytes(), hipMemcpyDeviceToHost, m->st));
     MPP_HIP(hipStreamSynchronize(m->st));
     mixed_outer_view(m->state_h, K, &all_conv, pn.data(), pn0.data());
     if (all_conv || outer >= p->max_outer) break;
     int its = 0, rc = mixed_inner(g, K, m->A, m->kinv, m->R, m->X, m->P, m->AP, m->part, m->state, m->state_h,
-                                  p->max_inner_its, m->st, &its);
+                                  p->max_inner_its, m->st, &its, nullptr, false, m->coarse);
     if (rc) return rc;
     inner_total += its;
     MPP_HIP(launch_m_accum(g, K, m->X, m->U, m->st));
@@ -1006,6 +1015,7 @@ int mppd_msolver_newton(mppd_msolver *m, int n, const double *eps_h, const doubl
     MPP_HIP(hipMemcpyAsync(m->act_d, m->act_h, K * sizeof(int), hipMemcpyHostToDevice, m->st));
     MPP_HIP(launch_m_resid(g, K, m->r->et, m->ke, m->U, nullptr, m->kinv, m->R, m->X, m->P, m->part, first,
                            p->cg_rel_tol, p->cg_abs_tol, inner_tol, max_inner, m->state, m->st, m->act_d));
+    if (m->coarse) MPP_HIP(launch_c_start(g, K, *m->coarse, m->kinv, m->R, m->P, m->state, m->st));
     MPP_HIP(hipMemcpyAsync(m->state_h, m->state, mixed_state_bytes(), hipMemcpyDeviceToHost, m->st));
     MPP_HIP(hipStreamSynchronize(m->st));
     mixed_outer_norms(m->state_h, K, cflag.data(), rn.data(), rn0.data());
@@ -1043,7 +1053,7 @@ int mppd_msolver_newton(mppd_msolver *m, int n, const double *eps_h, const doubl
       if (all_conv || outer >= max_outer) break;
       int inner = 0;
       if ((rc = mixed_inner(g, K, m->A, m->kinv, m->R, m->X, m->P, m->AP, m->part, m->state, m->state_h, max_inner,
-                            m->st, &inner)))
+                            m->st, &inner, nullptr, false, m->coarse)))
         return rc;
       inner_total += inner;
       ++outer_total;
@@ -1230,6 +1240,7 @@ int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *it
   MPP_HIP(launch_m_setbc(g, K, m->eps_d, m->U, m->st));
   MPP_HIP(launch_m_resid(g, K, m->r->et, m->ke, m->U, nullptr, m->kinv, m->R, m->X, m->P, m->part, 1, 0.0, 0.0, 0.0,
                          iters + 1, m->state, m->st));
+  if (m->coarse) MPP_HIP(launch_c_start(g, K, *m->coarse, m->kinv, m->R, m->P, m->state, m->st));
   hipEvent_t ev[3];
   for (auto &x : ev) MPP_HIP(hipEventCreate(&x));
   hipError_t e = hipEventRecord(ev[0], m->st);
@@ -1237,7 +1248,7 @@ int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *it
     e = launch_m_spmv_only(g, K, m->A, m->P, m->AP, m->part, m->state, m->st);
   if (e == hipSuccess) e = hipEventRecord(ev[1], m->st);
   for (int i = 0; i < iters && e == hipSuccess; ++i)
-    e = launch_m_iter(g, K, m->A, m->kinv, m->R, m->X, m->P, m->AP, m->part, m->state, i, m->st);
+    e = launch_m_iter(g, K, m->A, m->kinv, m->R, m->X, m->P, m->AP, m->part, m->state, i, m->st, m->coarse);
   if (e == hipSuccess) e = hipEventRecord(ev[2], m->st);
   if (e == hipSuccess) e = hipStreamSynchronize(m->st);
   float ms1 = 0.f, ms2 = 0.f;

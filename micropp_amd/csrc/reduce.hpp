@@ -44,6 +44,22 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// the same DPP sequence for a float (one 32-bit move per step)
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v += dpp_move<0xb1>(v);
+  v += dpp_move<0x4e>(v);
+  v += dpp_move<0x124>(v);
+  v += dpp_move<0x128>(v);
+  v += dpp_move<0x142>(v);
+  v += dpp_move<0x143>(v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
