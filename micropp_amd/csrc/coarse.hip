@@ -1,21 +1,24 @@
 // coarse.hip -- set-up of the two-level preconditioner of the multi-RHS inner
-// CG (host code; the kernels that apply it are mixed.hip k_c_*).
+// CG (the kernels that apply it are mixed.hip k_c_*).
 //
 // The coarse space: trilinear hats of the interior nodes of a coarse grid
 // aligned with the RVE's grid (h fine elements per coarse element and side),
 // the Dirichlet boundary excluded.  A_c = P^T A P (Galerkin) is assembled in
-// FP64 element by element: a fine element at local position (px, py, pz)
-// inside its coarse element interpolates its 8 nodes from that coarse
-// element's 8 corners, so its contribution is C = P_e^T Ke P_e with P_e the
-// 24 x 24 map of that position, the same for every element of one phase and
-// position.  A_c is factored and inverted on the device (rocSOLVER potrf /
-// potri) and kept as a dense FP32 inverse, so one application of the coarse
-// solve is one matrix-vector product per field.
+// FP64 on the device, element by element: a fine element at local position
+// (px, py, pz) inside its coarse element interpolates its 8 nodes from that
+// coarse element's 8 corners, so its contribution is C = P_e^T Ke P_e with P_e
+// the 24 x 24 map of that position, the same for every element of one phase
+// and position (k_cs_table); the coarse element matrices sum those
+// (k_cs_elem) and the dense A_c gathers them (k_cs_dense).  A_c is factored
+// and inverted on the device (rocSOLVER potrf / potri) and kept as a dense
+// FP32 inverse, so one application of the coarse solve is one matrix-vector
+// product per field.
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cstdlib>
-#include <vector>
+#include <map>
+#include <mutex>
 
 #include "internal.hpp"
 
@@ -23,8 +26,8 @@ namespace mpp {
 
 namespace {
 
-constexpr int kCoarseMaxDofs = 2200;  // 3 x 9^3 = 2187: a 10^3-element coarse grid
-constexpr int kHexOff[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+constexpr int kCoarseMaxDofs = 2200;   // 3 x 9^3 = 2187: a 10^3-element coarse grid
+constexpr int kCoarseMaxPos = 4096;    // fine elements per coarse element (the C table's size)
 
 // coarse elements along a side of ne fine elements: the largest divisor
 // d <= dmax of ne with ne / d >= 2 (0: none with d >= 2)
@@ -33,6 +36,83 @@ int pick_side(int ne, int dmax) {
     if (ne % d == 0) return d;
   return 0;
 }
+
+// corner A of a hexahedron (the element's local node order: (0,0,0), (1,0,0),
+// (1,1,0), (0,1,0), then the same at z + 1) and its inverse
+__device__ __forceinline__ int hex_ox(int A) { return ((A + 1) >> 1) & 1; }
+__device__ __forceinline__ int hex_oy(int A) { return (A >> 1) & 1; }
+__device__ __forceinline__ int hex_oz(int A) { return A >> 2; }
+__device__ __forceinline__ int hex_corner(int ox, int oy, int oz) { return oz * 4 + (oy ? 3 - ox : ox); }
+
+// C[(pos * 3 + phase) * 576 + r * 24 + c] = (P_e^T Ke_phase P_e)[r][c] for the
+// fine element at position pos = (pz hy + py) hx + px: block (pos, phase),
+// one thread per entry
+__global__ void __launch_bounds__(576) k_cs_table(CGrid cg, const double *ke, double *C) {
+  __shared__ double N[8][8];  // N[a][A]: coarse corner A's hat at the fine element's node a
+  __shared__ double K[576];
+  const int pos = blockIdx.x, ph = blockIdx.y, t = threadIdx.x;
+  const int px = pos % cg.hx, py = (pos / cg.hx) % cg.hy, pz = pos / (cg.hx * cg.hy);
+  K[t] = ke[(size_t)ph * 576 + t];
+  if (t < 64) {
+    const int a = t >> 3, A = t & 7;
+    const double xi = (double)(px + hex_ox(a)) / cg.hx, yi = (double)(py + hex_oy(a)) / cg.hy,
+                 zi = (double)(pz + hex_oz(a)) / cg.hz;
+    N[a][A] = (hex_ox(A) ? xi : 1.0 - xi) * (hex_oy(A) ? yi : 1.0 - yi) * (hex_oz(A) ? zi : 1.0 - zi);
+  }
+  __syncthreads();
+  const int A = t / 72, d = (t / 24) % 3, B = (t % 24) / 3, e = t % 3;
+  double s = 0.0;
+  for (int a = 0; a < 8; ++a) {
+    double ta = 0.0;
+    for (int b = 0; b < 8; ++b) ta += K[(a * 3 + d) * 24 + b * 3 + e] * N[b][B];
+    s += N[a][A] * ta;
+  }
+  C[((size_t)pos * 3 + ph) * 576 + t] = s;
+}
+
+// KE[E * 576 + t]: the stiffness of coarse element E, the sum of its fine
+// elements' C entries in position order; block E, one thread per entry
+__global__ void __launch_bounds__(576) k_cs_elem(Grid g, CGrid cg, int dx, int dy, const int *et, const double *C,
+                                                 double *KE) {
+  const int E = blockIdx.x, t = threadIdx.x;
+  const int Ex = E % dx, Ey = (E / dx) % dy, Ez = E / (dx * dy);
+  double s = 0.0;
+  for (int pz = 0; pz < cg.hz; ++pz)
+    for (int py = 0; py < cg.hy; ++py)
+      for (int px = 0; px < cg.hx; ++px) {
+        const int e = ((Ez * cg.hz + pz) * g.ney + (Ey * cg.hy + py)) * g.nex + Ex * cg.hx + px;
+        const int pos = (pz * cg.hy + py) * cg.hx + px;
+        s += C[((size_t)pos * 3 + et[e]) * 576 + t];
+      }
+  KE[(size_t)E * 576 + t] = s;
+}
+
+// The dense A_c (column-major, nc x nc) over the interior coarse nodes: entry
+// (q_i 3 + d, q_j 3 + e) sums the coarse elements holding both nodes, in
+// element order (zero when the nodes share none)
+__global__ void __launch_bounds__(256) k_cs_dense(CGrid cg, int dx, int dy, const double *KE, double *Ac) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int nc = cg.nc;
+  if (t >= (size_t)nc * nc) return;
+  const int row = (int)(t % nc), col = (int)(t / nc);
+  const int qi = row / 3, d = row % 3, qj = col / 3, e = col % 3;
+  const int xi = qi % cg.mx + 1, yi = (qi / cg.mx) % cg.my + 1, zi = qi / (cg.mx * cg.my) + 1;
+  const int xj = qj % cg.mx + 1, yj = (qj / cg.mx) % cg.my + 1, zj = qj / (cg.mx * cg.my) + 1;
+  double s = 0.0;
+  if (abs(xi - xj) <= 1 && abs(yi - yj) <= 1 && abs(zi - zj) <= 1)
+    for (int ez = max(zi, zj) - 1; ez <= min(zi, zj); ++ez)
+      for (int ey = max(yi, yj) - 1; ey <= min(yi, yj); ++ey)
+        for (int ex = max(xi, xj) - 1; ex <= min(xi, xj); ++ex) {
+          const int A = hex_corner(xi - ex, yi - ey, zi - ez), B = hex_corner(xj - ex, yj - ey, zj - ez);
+          s += KE[((size_t)(ez * dy + ey) * dx + ex) * 576 + (A * 3 + d) * 24 + B * 3 + e];
+        }
+  Ac[t] = s;
+}
+
+// one rocBLAS handle per device for the process's lifetime (the factorisation
+// runs under the mutex, with the handle set to the caller's stream)
+std::mutex g_solver_mu;
+std::map<int, rocblas_handle> g_handles;
 
 }  // namespace
 
@@ -73,92 +153,13 @@ int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipS
   cg.nc = 3 * cg.nq;
   const int nc = cg.nc;
 
-  std::vector<int> et(g.nelem);
-  std::vector<double> ke(3 * 576);
-  MPP_HIP(hipMemcpyAsync(et.data(), et_d, sizeof(int) * g.nelem, hipMemcpyDeviceToHost, st));
-  MPP_HIP(hipMemcpyAsync(ke.data(), ke_d, sizeof(double) * 3 * 576, hipMemcpyDeviceToHost, st));
-  MPP_HIP(hipStreamSynchronize(st));
-
-  // C[(pos * 3 + phase)] = P_e^T Ke_phase P_e for every local position of a
-  // fine element inside its coarse element
-  const int npos = cg.hx * cg.hy * cg.hz;
-  std::vector<double> C((size_t)npos * 3 * 576, 0.0);
-  for (int pz = 0; pz < cg.hz; ++pz)
-    for (int py = 0; py < cg.hy; ++py)
-      for (int px = 0; px < cg.hx; ++px) {
-        const int pos = (pz * cg.hy + py) * cg.hx + px;
-        double N[8][8];  // N[a][A]: corner A's hat at fine node a
-        for (int a = 0; a < 8; ++a) {
-          const double xi[3] = {(double)(px + kHexOff[a][0]) / cg.hx, (double)(py + kHexOff[a][1]) / cg.hy,
-                                (double)(pz + kHexOff[a][2]) / cg.hz};
-          for (int A = 0; A < 8; ++A) {
-            double w = 1.0;
-            for (int t = 0; t < 3; ++t) w *= kHexOff[A][t] ? xi[t] : 1.0 - xi[t];
-            N[a][A] = w;
-          }
-        }
-        for (int ph = 0; ph < 3; ++ph) {
-          const double *K_ = &ke[(size_t)ph * 576];
-          double T[24][8];  // T[a d][B] for one column dof e: Ke P restricted to dof e
-          double *Cp = &C[((size_t)pos * 3 + ph) * 576];
-          for (int e = 0; e < 3; ++e) {
-            for (int r = 0; r < 24; ++r)
-              for (int B = 0; B < 8; ++B) {
-                double s = 0.0;
-                for (int b = 0; b < 8; ++b) s += K_[r * 24 + b * 3 + e] * N[b][B];
-                T[r][B] = s;
-              }
-            for (int A = 0; A < 8; ++A)
-              for (int d = 0; d < 3; ++d)
-                for (int B = 0; B < 8; ++B) {
-                  double s = 0.0;
-                  for (int a = 0; a < 8; ++a) s += N[a][A] * T[a * 3 + d][B];
-                  Cp[(A * 3 + d) * 24 + B * 3 + e] = s;
-                }
-          }
-        }
-      }
-  // coarse element stiffness, then the dense A_c over the interior coarse nodes (column-major)
-  std::vector<double> KE((size_t)dx * dy * dz * 576, 0.0);
-  for (int ez = 0; ez < g.nez; ++ez)
-    for (int ey = 0; ey < g.ney; ++ey)
-      for (int ex = 0; ex < g.nex; ++ex) {
-        const int e = (ez * g.ney + ey) * g.nex + ex;
-        const int E = ((ez / cg.hz) * dy + ey / cg.hy) * dx + ex / cg.hx;
-        const int pos = ((ez % cg.hz) * cg.hy + ey % cg.hy) * cg.hx + ex % cg.hx;
-        const double *Cp = &C[((size_t)pos * 3 + et[e]) * 576];
-        double *KEp = &KE[(size_t)E * 576];
-        for (int t = 0; t < 576; ++t) KEp[t] += Cp[t];
-      }
-  std::vector<double> Ac((size_t)nc * nc, 0.0);
-  for (int Ez = 0; Ez < dz; ++Ez)
-    for (int Ey = 0; Ey < dy; ++Ey)
-      for (int Ex = 0; Ex < dx; ++Ex) {
-        const double *KEp = &KE[((size_t)(Ez * dy + Ey) * dx + Ex) * 576];
-        int q[8];
-        for (int A = 0; A < 8; ++A) {
-          const int cx = Ex + kHexOff[A][0], cy = Ey + kHexOff[A][1], cz = Ez + kHexOff[A][2];
-          const bool in = cx >= 1 && cx <= cg.mx && cy >= 1 && cy <= cg.my && cz >= 1 && cz <= cg.mz;
-          q[A] = in ? ((cz - 1) * cg.my + (cy - 1)) * cg.mx + (cx - 1) : -1;
-        }
-        for (int A = 0; A < 8; ++A) {
-          if (q[A] < 0) continue;
-          for (int B = 0; B < 8; ++B) {
-            if (q[B] < 0) continue;
-            for (int d = 0; d < 3; ++d)
-              for (int e = 0; e < 3; ++e)
-                Ac[(size_t)(q[A] * 3 + d) + (size_t)(q[B] * 3 + e) * nc] += KEp[(A * 3 + d) * 24 + B * 3 + e];
-          }
-        }
-      }
-
-  // device: Cholesky factor and inverse in FP64, then the FP32 symmetric copy
+  const int npos = cg.hx * cg.hy * cg.hz, ncel = dx * dy * dz;
+  if (npos > kCoarseMaxPos) return MPPD_OK;  // coarse elements too large for the C table
   CoarseDev *c = new CoarseDev();
   c->cg = cg;
-  double *a64 = nullptr;
+  double *a64 = nullptr, *C = nullptr, *KE = nullptr;
   int *info_d = nullptr;
   int rc = MPPD_OK;
-  rocblas_handle h = nullptr;
   do {
     float *ainv = nullptr;
     if ((rc = dalloc(&ainv, (size_t)nc * nc, false))) break;
@@ -170,17 +171,34 @@ int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipS
     if ((rc = dalloc(&c->z, mixed_vector_len(g, K)))) break;
     if ((rc = dalloc(&c->prz, (size_t)3 * K * coarse_vec_blocks(g)))) break;
     if ((rc = dalloc(&a64, (size_t)nc * nc, false))) break;
+    if ((rc = dalloc(&C, (size_t)npos * 3 * 576, false))) break;
+    if ((rc = dalloc(&KE, (size_t)ncel * 576, false))) break;
     if ((rc = dalloc(&info_d, 1))) break;
-    hipError_t e = hipMemcpyAsync(a64, Ac.data(), sizeof(double) * Ac.size(), hipMemcpyHostToDevice, st);
+    hipLaunchKernelGGL(k_cs_table, dim3(npos, 3), dim3(576), 0, st, cg, ke_d, C);
+    hipLaunchKernelGGL(k_cs_elem, dim3(ncel), dim3(576), 0, st, g, cg, dx, dy, et_d, C, KE);
+    hipLaunchKernelGGL(k_cs_dense, dim3((unsigned)(((size_t)nc * nc + 255) / 256)), dim3(256), 0, st, cg, dx, dy, KE,
+                       a64);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
-      rc = hip_fail(e, "coarse matrix upload");
+      rc = hip_fail(e, "coarse matrix assembly");
       break;
     }
-    if (rocblas_create_handle(&h) != rocblas_status_success || rocblas_set_stream(h, st) != rocblas_status_success) {
+    int dev = 0, info = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) {
+      rc = hip_fail(e, "hipGetDevice");
+      break;
+    }
+    std::lock_guard<std::mutex> lock(g_solver_mu);
+    rocblas_handle &h = g_handles[dev];
+    if (!h && rocblas_create_handle(&h) != rocblas_status_success) {
+      h = nullptr;
       rc = fail(MPPD_ERR_HIP, "rocblas handle for the coarse solve");
       break;
     }
-    int info = 0;
+    if (rocblas_set_stream(h, st) != rocblas_status_success) {
+      rc = fail(MPPD_ERR_HIP, "rocblas stream for the coarse solve");
+      break;
+    }
     if (rocsolver_dpotrf(h, rocblas_fill_lower, nc, a64, nc, info_d) != rocblas_status_success) {
       rc = fail(MPPD_ERR_HIP, "rocsolver_dpotrf (coarse matrix)");
       break;
@@ -211,8 +229,10 @@ int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipS
     }
     *out = c;
   } while (0);
-  if (h) (void)rocblas_destroy_handle(h);
+  (void)hipStreamSynchronize(st);  // the temporaries below may still be in use after a failed step
   dfree(a64);
+  dfree(C);
+  dfree(KE);
   dfree(info_d);
   if (*out != c) coarse_free(c);
   return rc;

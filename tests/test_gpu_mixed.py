@@ -152,3 +152,29 @@ def test_mixed_solve_is_deterministic(mp, K):
         sig, st, _, ok = mp.MSolver(r, K).solve(eps) if rep == 1 else ms.solve(eps)
         assert ok and st == st0
         assert np.array_equal(sig, first), f"repeat {rep}: max diff {np.abs(sig - first).max():.3e}"
+
+
+@pytest.mark.parametrize("size", [(51, 11, 41), (13, 9, 17), (31, 31, 31)])
+def test_coarse_level_against_jacobi(mp, monkeypatch, size):
+    """The two-level preconditioner of the inner CG (coarse grids with h = 5, 2, 2 / 2, 2, 2 / 3, 3, 3 fine
+    elements per coarse element) changes the inner iterations only: sigma and ctan agree with the
+    Jacobi-only solve (MPPD_COARSE=0) and the FP64 path, and the inner iterations drop."""
+    rng = np.random.default_rng(sum(size))
+    strains = rng.uniform(-5e-3, 5e-3, (6, 6))
+    r = mp.Rve(size, mp.MIC3D_RANDOM_SPHERES, GEO, elastic(mp))
+    sig, ctan, stats, resid, ok = mp.homogenize_mixed(mp.MSolver(r, 12), strains)
+    monkeypatch.setenv("MPPD_COARSE", "0")
+    sig_j, ctan_j, stats_j, resid_j, ok_j = mp.homogenize_mixed(mp.MSolver(r, 12), strains)
+    monkeypatch.delenv("MPPD_COARSE")
+    print(f"{size}: inner iterations {stats[1]} (Jacobi {stats_j[1]}); sigma rel {rel(sig, sig_j):.1e} ctan rel "
+          f"{rel(ctan, ctan_j):.1e}", flush=True)
+    assert ok and ok_j and resid.max() < 1e-8 and resid_j.max() < 1e-8
+    assert rel(sig, sig_j) < 1e-7 and rel(ctan, ctan_j) < 1e-7
+    assert stats[1] < 0.7 * stats_j[1]
+    kw = dict(lin_stress=0, cg_rel_tol=1e-8, cg_max_its=3000)
+    m = mp.Micropp3(size, mp.MIC3D_RANDOM_SPHERES, GEO, elastic(mp), ngp=6, coupling=[mp.FE_FULL] * 6, **kw)
+    for g in range(6):
+        m.set_strain(g, strains[g])
+    m.homogenize()
+    assert rel(sig, np.array([m.get_stress(g) for g in range(6)])) < 1e-6
+    assert rel(ctan, m.ctan_lin()) < 1e-6
