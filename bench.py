@@ -253,10 +253,14 @@ def _batch_run(mp, descs, ramps, params, label):
         dt64, info64, stats64, _, sig64, _, (ms64, bytes64) = _batch_timed(
             mp, descs, ramps, dict(params, mixed_precision=0), label + " fp64")
         err = np.abs(sig_b - sig64).max(axis=1) / np.abs(sig64).max(axis=1)
+        both = (info[:, 2] != 0) & (info64[:, 2] != 0)  # the last step's Newton converged in both
+        mism = int(np.any(info[:, [0, 2]] != info64[:, [0, 2]], axis=1).sum())
         return dict(common, value=nrve * steps / dt_b, unit="RVE solves/s (sigma+ctan, FE_FULL)",
                     precision="FP32 inner CG, FP64 defect correction", fp64_batched_rve_solves_per_s=nrve * steps / dt64,
                     speedup_vs_fp64_batch=dt64 / dt_b, sigma_max_rel_err_vs_fp64=float(err.max()),
-                    same_flags_as_fp64=bool(np.array_equal(info[:, [0, 2]], info64[:, [0, 2]])),
+                    sigma_max_rel_err_vs_fp64_converged=float(err[both].max()) if both.any() else None,
+                    converged_rves_both=int(both.sum()),
+                    same_flags_as_fp64=mism == 0, rves_with_other_flags_than_fp64=mism,
                     fp64_cg_iterations_queued=int(stats64[0]), fp64_host_syncs_last_step=int(stats64[4]),
                     fp64_spmv={"ms": ms64, "bytes_per_launch": bytes64, "achieved_GBs": bytes64 / ms64 / 1e6,
                                "frac_of_hbm_peak": bytes64 / ms64 / 1e6 / HBM_PEAK_GBS},

@@ -391,8 +391,169 @@ __global__ void __launch_bounds__(256) kb_m_accum(const BRve *rv, const int *map
 }
 
 // ---------------------------------------------------------------------------
+// Two-level preconditioner of the inner FP32 CG (mixed precision): z = D^-1 r
+// + P_c A_c^-1 P_c^T r per RVE, with the trilinear hats of the RVE's coarse
+// grid (R.cgrid) and the inverse of the Galerkin coarse matrix of its initial
+// (elastic) tangent.  It runs at the inner CG's start, after kb_m_resid (act
+// != nullptr: the RVEs in act), and in every iteration between kb_m_step2 and
+// kb_m_step3 (act == nullptr: the RVEs whose CG state `parity` is running).
+// FP32 residual rows: row = node * 3 + d.  The stopping test keeps the
+// Jacobi ||D^-1 r|| of kb_m_step2; only (r, z) and z change.  Sums run in
+// fixed orders.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool kb_c_skip(const BRve &R, const int *act, int rid, int parity) {
+  return R.cgrid.nc == 0 || (act ? !act[rid] : R.cg[parity & 1].done != 0);
+}
+
+// restriction along x: one thread per (direction d, interior x-line (j, k),
+// interior coarse x node cx), over the 2 hx - 1 nodes of the hat
+__global__ void __launch_bounds__(256) kb_c_rx(const BRve *rv, const int *map, const int *blk0, const int *act,
+                                               int parity) {
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  if (kb_c_skip(R, act, rid, parity)) return;
+  const Grid &g = R.g;
+  const CGrid &cg = R.cgrid;
+  const int ly = g.ny - 2, lz = g.nz - 2;
+  const int t = (blockIdx.x - blk0[rid]) * 256 + threadIdx.x;
+  if (t >= 3 * ly * lz * cg.mx) return;
+  const int cx = t % cg.mx + 1;
+  int rest = t / cg.mx;
+  const int j = rest % ly + 1;
+  rest /= ly;
+  const int k = rest % lz + 1, d = rest / lz;
+  const float *line = R.r32 + ((size_t)k * g.ny + j) * g.nx * 3 + d;
+  const int i0 = (cx - 1) * cg.hx + 1, n = 2 * cg.hx - 1;
+  float acc = 0.0f;
+#pragma unroll 8
+  for (int s = 0; s < n; ++s) acc += hat(i0 + s, cx, cg.hx) * line[(size_t)(i0 + s) * 3];
+  R.ct1[((size_t)(d * g.nz + k) * g.ny + j) * cg.mx + (cx - 1)] = acc;
+}
+
+// restriction along y and z: one wave per coarse DOF q * 3 + d, its lanes over
+// the x-restricted lines under q's hats, then a wave sum
+__global__ void __launch_bounds__(256) kb_c_ryz(const BRve *rv, const int *map, const int *blk0, const int *act,
+                                                int parity) {
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  if (kb_c_skip(R, act, rid, parity)) return;
+  const Grid &g = R.g;
+  const CGrid &cg = R.cgrid;
+  const int wave = (blockIdx.x - blk0[rid]) * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (wave >= cg.nc) return;  // whole wave
+  const int q = wave / 3, d = wave % 3;
+  const int cx = q % cg.mx + 1, cy = (q / cg.mx) % cg.my + 1, cz = q / (cg.mx * cg.my) + 1;
+  const int wy = 2 * cg.hy - 1, npair = wy * (2 * cg.hz - 1);
+  const int j0 = (cy - 1) * cg.hy + 1, k0 = (cz - 1) * cg.hz + 1;
+  float acc = 0.0f;
+  for (int s = lane; s < npair; s += 64) {
+    const int j = j0 + s % wy, k = k0 + s / wy;
+    acc += hat(j, cy, cg.hy) * hat(k, cz, cg.hz) * R.ct1[((size_t)(d * g.nz + k) * g.ny + j) * cg.mx + (cx - 1)];
+  }
+  const float sum = wave_sum_dpp(acc);
+  if (lane == 0) R.crc[wave] = sum;
+}
+
+// the coarse solve e = A_c^-1 r_c: block (64 coarse rows, column split s of
+// R.csplit), each wave a quarter of the split's columns (the symmetric
+// inverse read by column), the waves added in order; into cec directly with
+// one split, else into the split's partial row of cecp
+__global__ void __launch_bounds__(256) kb_c_gemv(const BRve *rv, const int *map, const int *blk0, const int *act,
+                                                 int parity) {
+  __shared__ float red[3][64];
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  if (kb_c_skip(R, act, rid, parity)) return;
+  const int nc = R.cgrid.nc, lane = threadIdx.x & 63, ns = R.csplit;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lb = blockIdx.x - blk0[rid], tile = lb / ns, split = lb % ns;
+  const int chunk = (nc + ns - 1) / ns, c0 = split * chunk, c1 = min(nc, c0 + chunk);
+  const int row = tile * 64 + lane, rowc = min(row, nc - 1);
+  const float *ainv = R.cainv, *rc = R.crc;
+  float acc = 0.0f;
+#pragma unroll 4
+  for (int col = c0 + w; col < c1; col += 4) acc += ainv[(size_t)col * nc + rowc] * rc[col];
+  if (w > 0) red[w - 1][lane] = acc;
+  __syncthreads();
+  if (w == 0 && row < nc) {
+    const float v = acc + red[0][lane] + red[1][lane] + red[2][lane];
+    if (ns == 1)
+      R.cec[row] = v;
+    else
+      R.cecp[(size_t)split * nc + row] = v;
+  }
+}
+
+// the splits' sum in order (RVEs with csplit > 1)
+__global__ void __launch_bounds__(256) kb_c_gsum(const BRve *rv, const int *map, const int *blk0, const int *act,
+                                                 int parity) {
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  if (kb_c_skip(R, act, rid, parity)) return;
+  const int nc = R.cgrid.nc, row = (blockIdx.x - blk0[rid]) * 256 + threadIdx.x;
+  if (row >= nc) return;
+  float v = 0.0f;
+  for (int s = 0; s < R.csplit; ++s) v += R.cecp[(size_t)s * nc + row];
+  R.cec[row] = v;
+}
+
+// z += P_c e on the RVE's rows (the 8 coarse nodes around a node, zero
+// weights outside) and the block's (r, z) partial in place of the Jacobi one:
+// part[lb * 3 + 1] at the start (where p = z as well), else the (r, z) slot
+// of kb_m_step2's partials
+__global__ void __launch_bounds__(256) kb_c_zfix(const BRve *rv, const int *map, const int *blk0, const int *act,
+                                                 int parity) {
+  __shared__ double lds[4];
+  const int rid = map[blockIdx.x];
+  const BRve &R = rv[rid];
+  if (kb_c_skip(R, act, rid, parity)) return;
+  const Grid &g = R.g;
+  const CGrid &cg = R.cgrid;
+  const int lb = blockIdx.x - blk0[rid], row = lb * 256 + threadIdx.x;  // < nrow_pad
+  float z = R.z32[row];
+  const float r = R.r32[row];
+  if (row < g.nrow) {
+    const int node = row / 3, d = row % 3;
+    const int i = node % g.nx, j = (node / g.nx) % g.ny, k = node / (g.nx * g.ny);
+    float wx[2], wy[2], wz[2];
+    int qx[2], qy[2], qz[2];
+    hat_pair(i, cg.hx, cg.mx, wx, qx);
+    hat_pair(j, cg.hy, cg.my, wy, qy);
+    hat_pair(k, cg.hz, cg.mz, wz, qz);
+    float e[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      e[t] = R.cec[((qz[t >> 2] * cg.my + qy[(t >> 1) & 1]) * cg.mx + qx[t & 1]) * 3 + d];
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc += wx[t & 1] * (wy[(t >> 1) & 1] * wz[t >> 2]) * e[t];
+    z += acc;
+  }
+  R.z32[row] = z;
+  if (act) R.p32[row] = z;
+  double v[1] = {(double)r * z};
+  block_sum<1, 256>(v, lds);
+  if (threadIdx.x == 0) {
+    if (act)
+      R.part[(size_t)lb * 3 + 1] = v[0];
+    else
+      R.part[R.nb_row + (size_t)lb * 2 + 1] = v[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+static hipError_t launch_bc(const BLaunch &L, const int *act, int parity, hipStream_t st) {
+  hipLaunchKernelGGL(kb_c_rx, dim3(L.crx.nb), dim3(256), 0, st, L.rv, L.crx.map, L.crx.blk0, act, parity);
+  hipLaunchKernelGGL(kb_c_ryz, dim3(L.cryz.nb), dim3(256), 0, st, L.rv, L.cryz.map, L.cryz.blk0, act, parity);
+  hipLaunchKernelGGL(kb_c_gemv, dim3(L.cgemv.nb), dim3(256), 0, st, L.rv, L.cgemv.map, L.cgemv.blk0, act, parity);
+  if (L.cgsum.nb)
+    hipLaunchKernelGGL(kb_c_gsum, dim3(L.cgsum.nb), dim3(256), 0, st, L.rv, L.cgsum.map, L.cgsum.blk0, act, parity);
+  hipLaunchKernelGGL(kb_c_zfix, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, act, parity);
+  return hipGetLastError();
+}
+
 hipError_t launch_b_set_bc(const BLaunch &L, const double *eps, hipStream_t st) {
   hipLaunchKernelGGL(kb_set_bc, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0, eps);
   return hipGetLastError();
@@ -490,6 +651,10 @@ hipError_t launch_bm_prep(const BLaunch &L, hipStream_t st) {
 hipError_t launch_bm_resid(const BLaunch &L, int first, double rel, double abs_tol, double tau_min, int max_inner,
                            const int *tight, double tight_factor, BOuter *out, hipStream_t st) {
   hipLaunchKernelGGL(kb_m_resid, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.act, L.row.blk0, L.a0on);
+  if (L.coarse) {
+    const hipError_t e = launch_bc(L, L.act, 0, st);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(kb_m_outer, dim3(L.nrve), dim3(256), 0, st, L.rv, L.act, first, rel, abs_tol, tau_min,
                      max_inner, tight, tight_factor, out);
   return hipGetLastError();
@@ -512,6 +677,10 @@ hipError_t launch_bm_iter(const BLaunch &L, int parity, hipStream_t st) {
   const hipError_t e = launch_bm_spmv(L, L.spmv_shape32, parity, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kb_m_step2, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
+  if (L.coarse) {
+    const hipError_t e2 = launch_bc(L, nullptr, parity, st);
+    if (e2 != hipSuccess) return e2;
+  }
   hipLaunchKernelGGL(kb_m_step3, dim3(L.row.nb), dim3(256), 0, st, L.rv, L.row.map, L.row.blk0, parity);
   return hipGetLastError();
 }

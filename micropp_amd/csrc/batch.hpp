@@ -30,6 +30,15 @@ struct BRve {
   // solved (vals' layout), the inner CG's FP32 vectors and Jacobi weights;
   // x holds the FP64 correction du, r the FP64 residual, kinv the FP64 D^-1
   float *vals32, *x32, *r32, *z32, *p32, *Ap32, *kinv32;
+  // mixed precision, two-level preconditioner of the inner CG (cgrid.nc = 0:
+  // Jacobi alone): the inverse of the Galerkin coarse matrix of the RVE's
+  // initial (elastic) tangent, the x-restricted residual (3 nz ny mx), the
+  // restricted residual and the coarse correction (nc each)
+  CGrid cgrid;
+  const float *cainv;
+  float *ct1, *crc, *cec;
+  int csplit;    // column splits of the coarse solve
+  float *cecp;   // their partial products (csplit > 1), csplit x nc
   int phase_mode;              // all phases elastic: one Ke per phase
   int use_vars;                // the RVE has history (vars_n / vars_k in use)
   int nb_row, nb_elem;
@@ -72,6 +81,13 @@ struct BLaunch {
   const int *act;  // per-RVE activity mask of this launch
   const int *a0on;  // per RVE: the CG solves with a0 instead of vals
   BShape row, elem, ke;
+  // the two-level preconditioner's shapes: x-restriction (threads per
+  // (direction, interior x-line, coarse x node)), y/z-restriction (a wave per
+  // coarse DOF), coarse solve (64 coarse rows x one column split per block),
+  // the splits' sum (RVEs with more than one split); coarse = some RVE has a
+  // coarse level
+  BShape crx, cryz, cgemv, cgsum;
+  int coarse = 0;
   int spmv_shape = kSpmvDefault;    // see launch_b_spmv
   int spmv_shape32 = kSpmvDefault;  // the same choice for the inner FP32 SpMV (launch_bm_spmv)
 };
@@ -129,7 +145,8 @@ hipError_t launch_bm_prep(const BLaunch &L, hipStream_t st);
 // times tight_factor for the RVEs whose entry in tight (may be null) is set
 hipError_t launch_bm_resid(const BLaunch &L, int first, double rel, double abs_tol, double tau_min, int max_inner,
                            const int *tight, double tight_factor, BOuter *out, hipStream_t st);
-// one inner CG iteration of every RVE (done RVEs return at once)
+// one inner CG iteration of every RVE (done RVEs return at once); with a
+// coarse level, z = D^-1 r + P_c A_c^-1 P_c^T r
 hipError_t launch_bm_iter(const BLaunch &L, int parity, hipStream_t st);
 // the inner FP32 SpMV alone in one of kSpmvShapes shapes, as launch_b_spmv
 hipError_t launch_bm_spmv(const BLaunch &L, int shape, int parity, hipStream_t st);

@@ -23,6 +23,14 @@ using namespace mpp;
 namespace {
 
 constexpr double kDEpsCtanAve = 1e-8;  // D_EPS_CTAN_AVE (include/params.hpp:32)
+// coarse DOFs per RVE of a mixed-precision batch: every inner iteration
+// reads the nc^2 FP32 inverse beside the RVE's 324 B/row FP32 matrix, so nc
+// is held to max(400, 4 sqrt(rows)), an inverse of at most 64 B per row
+// (30^3: 6 coarse elements a side, 375 DOFs; 40^3: 8, 1029; 50^3: 10, 2187)
+// columns per split of the coarse solve (kb_c_gemv); more than one split is
+// summed by kb_c_gsum
+constexpr int kBatchCoarseSplitCols = 512;
+int batch_coarse_max_dofs(const Grid &g) { return std::max(400, (int)(4.0 * std::sqrt((double)g.nrow))); }
 
 struct BState {
   double strain[6] = {0}, strain_old[6] = {0}, stress[6] = {0}, ctan[36] = {0};
@@ -47,8 +55,13 @@ struct mppd_batch {
   std::vector<BState> s;
   BRve *rv = nullptr;
   // launch shapes
-  int *map_d[3] = {nullptr, nullptr, nullptr}, *blk0_d[3] = {nullptr, nullptr, nullptr};
-  int nb[3] = {0, 0, 0};
+  // launch shapes: rows, elements, element stiffness, then the coarse
+  // level's x-restriction, y/z-restriction and coarse solve
+  static constexpr int kShapes = 7;
+  int *map_d[kShapes] = {}, *blk0_d[kShapes] = {};
+  int nb[kShapes] = {};
+  int coarse_any = 0;    // mixed precision: some RVE has a coarse level
+  float *c32 = nullptr;  // coarse arena: inverses, restriction and correction vectors
   // arenas
   double *vec = nullptr, *vals = nullptr, *ke = nullptr, *be = nullptr, *vars = nullptr, *part = nullptr;
   double *a0 = nullptr;  // initial matrices (use_A0), one per distinct RVE
@@ -102,6 +115,11 @@ struct mppd_batch {
     L.row = {nb[0], map_d[0], blk0_d[0]};
     L.elem = {nb[1], map_d[1], blk0_d[1]};
     L.ke = {nb[2], map_d[2], blk0_d[2]};
+    L.crx = {nb[3], map_d[3], blk0_d[3]};
+    L.cryz = {nb[4], map_d[4], blk0_d[4]};
+    L.cgemv = {nb[5], map_d[5], blk0_d[5]};
+    L.cgsum = {nb[6], map_d[6], blk0_d[6]};
+    L.coarse = coarse_any;
     L.spmv_shape = spmv_shape;
     L.spmv_shape32 = spmv_shape32;
     return L;
@@ -728,11 +746,11 @@ void destroy(mppd_batch *B) {
     if (x) (void)hipGraphExecDestroy(x);
   for (hipGraph_t x : {B->graph_head, B->graph_tail})
     if (x) (void)hipGraphDestroy(x);
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < mppd_batch::kShapes; ++k) {
     dfree(B->map_d[k]);
     dfree(B->blk0_d[k]);
   }
-  for (void *p : {(void *)B->rv, (void *)B->vec, (void *)B->vals, (void *)B->a0, (void *)B->f32,
+  for (void *p : {(void *)B->rv, (void *)B->vec, (void *)B->vals, (void *)B->a0, (void *)B->f32, (void *)B->c32,
                   (void *)B->outer_d, (void *)B->ke, (void *)B->be, (void *)B->vars, (void *)B->part, (void *)B->cg,
                   (void *)B->flags_d, (void *)B->eps_d, (void *)B->norms_d, (void *)B->sig_d, (void *)B->nst_d,
                   (void *)B->pend_d})
@@ -811,6 +829,61 @@ int setup_a0(mppd_batch *B, const mppd_batch_rve *in) {
   return upload_table(B);
 }
 
+// The coarse levels of a mixed-precision batch: per RVE with a coarse grid,
+// the FP32 inverse of the Galerkin coarse matrix of its initial tangent
+// (per-phase element stiffness at u = 0 without history: the elastic one),
+// shared by equal RVEs, and its work vectors.  An RVE whose coarse matrix is
+// not positive definite keeps Jacobi alone (cgrid.nc = 0).
+int setup_coarse(mppd_batch *B, const mppd_batch_rve *in, size_t c32) {
+  const int n = B->n;
+  int rc = dalloc(&B->c32, c32, false);
+  if (rc) return rc;
+  double *ke = nullptr;
+  if ((rc = dalloc(&ke, 3 * 576, false))) return rc;
+  std::vector<int> owner(n, -1);
+  size_t off = 0;
+  for (int r = 0; r < n && !rc; ++r) {
+    BRve &R = B->h[r];
+    const Grid &g = R.g;
+    const CGrid cg = R.cgrid;
+    if (!cg.nc) continue;
+    for (int q = 0; q < r && owner[r] < 0; ++q)
+      if (owner[q] == q && same_rve(in[q], in[r])) owner[r] = q;
+    if (owner[r] >= 0) {
+      R.cainv = B->h[owner[r]].cainv;
+    } else {
+      float *ainv = B->c32 + off;
+      off += (size_t)cg.nc * cg.nc;
+      const mppd_rve *v = B->rves[r];
+      bool ok = false;
+      hipError_t e = launch_elem_ke(g, v->bmat, v->mats_d, v->et, nullptr, nullptr, 1, ke, B->st);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "coarse level: element stiffness");
+        break;
+      }
+      if ((rc = coarse_inverse(g, cg, v->et, ke, ainv, B->st, &ok))) break;
+      if (!ok) {
+        R.cgrid = CGrid{};
+        continue;
+      }
+      owner[r] = r;
+      R.cainv = ainv;
+    }
+    R.ct1 = B->c32 + off;
+    off += (size_t)3 * g.nz * g.ny * cg.mx;
+    R.crc = B->c32 + off;
+    R.cec = B->c32 + off + cg.nc;
+    off += 2 * (size_t)cg.nc;
+    if (R.csplit > 1) {
+      R.cecp = B->c32 + off;
+      off += (size_t)R.csplit * cg.nc;
+    }
+  }
+  (void)hipStreamSynchronize(B->st);
+  dfree(ke);
+  return rc;
+}
+
 int setup(mppd_batch *B, const mppd_batch_rve *in) {
   const int n = B->n;
   int rc = MPPD_OK;
@@ -826,10 +899,13 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
     B->s[r].coupling = d.coupling;
     B->s[r].nonlinear_mat = !B->rves[r]->all_elastic;
   }
-  // sizes, block tables and arena offsets
-  std::vector<int> map[3], blk0[3];
-  size_t rows = 0, slots = 0, kes = 0, bes = 0, nvars = 0, parts = 0;
-  for (int k = 0; k < 3; ++k) blk0[k].assign(n, 0);
+  // sizes, block tables and arena offsets; with mixed precision, each RVE's
+  // coarse grid (a small one: its inverse is read by every inner iteration)
+  constexpr int kS = mppd_batch::kShapes;
+  std::vector<int> map[kS], blk0[kS];
+  size_t rows = 0, slots = 0, kes = 0, bes = 0, nvars = 0, parts = 0, c32 = 0;
+  for (int k = 0; k < kS; ++k) blk0[k].assign(n, 0);
+  const bool coarse = B->prm.mixed_precision && coarse_enabled();
   for (int r = 0; r < n; ++r) {
     const mppd_rve *v = B->rves[r];
     const Grid &g = v->g;
@@ -839,8 +915,19 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
     R.nb_row = g.nrow_pad / 256;
     R.nb_elem = (g.nelem + 255) / 256;
     const int nb_ke = ((R.phase_mode ? 3 : g.nelem) + kKeWaves - 1) / kKeWaves;
-    const int counts[3] = {R.nb_row, R.nb_elem, nb_ke};
-    for (int k = 0; k < 3; ++k) {
+    R.cgrid = CGrid{};
+    if (coarse && !coarse_grid(g, batch_coarse_max_dofs(g), &R.cgrid)) R.cgrid = CGrid{};
+    const CGrid &cg = R.cgrid;
+    const int nrx = cg.nc ? (3 * (g.ny - 2) * (g.nz - 2) * cg.mx + 255) / 256 : 0;
+    R.csplit = (cg.nc + kBatchCoarseSplitCols - 1) / kBatchCoarseSplitCols;
+    const int ngsum = R.csplit > 1 ? (cg.nc + 255) / 256 : 0;
+    const int counts[kS] = {R.nb_row, R.nb_elem, nb_ke, nrx, (cg.nc + 3) / 4, (cg.nc + 63) / 64 * R.csplit, ngsum};
+    if (cg.nc) {
+      B->coarse_any = 1;
+      c32 += (size_t)cg.nc * cg.nc + (size_t)3 * g.nz * g.ny * cg.mx + 2 * (size_t)cg.nc;
+      if (R.csplit > 1) c32 += (size_t)R.csplit * cg.nc;
+    }
+    for (int k = 0; k < kS; ++k) {
       blk0[k][r] = (int)map[k].size();
       map[k].insert(map[k].end(), counts[k], r);
     }
@@ -851,7 +938,7 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
     if (!v->all_elastic) nvars += (size_t)g.nelem * 8 * NVAR;
     parts += std::max<size_t>(3 * (size_t)R.nb_row, 6 * (size_t)R.nb_elem);
   }
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < kS; ++k) {
     B->nb[k] = (int)map[k].size();
     if ((rc = dalloc(&B->map_d[k], map[k].size(), false))) return rc;
     if ((rc = dalloc(&B->blk0_d[k], n, false))) return rc;
@@ -919,6 +1006,7 @@ int setup(mppd_batch *B, const mppd_batch_rve *in) {
     if ((rc = dalloc(&B->outer_d, n))) return rc;
     if ((rc = pinned(&B->outer_h, n))) return rc;
   }
+  if (B->coarse_any && (rc = setup_coarse(B, in, c32))) return rc;
   for (int k = 0; k < M_NUM; ++k) {
     if ((rc = dalloc(&B->mask_d[k], n))) return rc;
     if ((rc = pinned(&B->mask_h[k], n))) return rc;

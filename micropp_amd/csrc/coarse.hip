@@ -26,8 +26,6 @@ namespace mpp {
 
 namespace {
 
-constexpr int kCoarseMaxDofs = 2200;   // 3 x 9^3 = 2187: a 10^3-element coarse grid
-constexpr int kCoarseMaxPos = 4096;    // fine elements per coarse element (the C table's size)
 
 // coarse elements along a side of ne fine elements: the largest divisor
 // d <= dmax of ne with ne / d >= 2 (0: none with d >= 2)
@@ -123,25 +121,26 @@ void coarse_free(CoarseDev *c) {
   delete c;
 }
 
-// Builds the coarse level of a K-field system on the RVE grid g whose phase
-// per element is et_d and whose per-phase element stiffness is ke_d (3 x 576,
-// FP64, on the device).  *out = nullptr (Jacobi alone) when the grid has no
-// usable coarse grid (a side with fewer than 4 elements or prime, a side
-// beyond kCoarseMaxLine nodes) or when MPPD_COARSE=0.
-int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipStream_t st, CoarseDev **out) {
-  *out = nullptr;
+bool coarse_enabled() {
   const char *env = std::getenv("MPPD_COARSE");
-  if (env && std::atoi(env) == 0) return MPPD_OK;
-  if (g.nx > kCoarseMaxLine || g.ny > kCoarseMaxLine || g.nz > kCoarseMaxLine) return MPPD_OK;  // k_c_zdot's tables
+  return !(env && std::atoi(env) == 0);
+}
+
+// The coarsest-first search: the largest coarse grid aligned with g (a
+// divisor of each side's element count, at least 2 coarse elements a side)
+// with at most max_dofs coarse DOFs and at most kCoarseMaxPos fine elements
+// per coarse element.
+bool coarse_grid(const Grid &g, int max_dofs, CGrid *out) {
+  if (g.nx > kCoarseMaxLine || g.ny > kCoarseMaxLine || g.nz > kCoarseMaxLine) return false;
   int dx = 0, dy = 0, dz = 0;
   for (int dmax = 32; dmax >= 2; --dmax) {
     dx = pick_side(g.nex, dmax);
     dy = pick_side(g.ney, dmax);
     dz = pick_side(g.nez, dmax);
-    if (dx && dy && dz && 3 * (dx - 1) * (dy - 1) * (dz - 1) <= kCoarseMaxDofs) break;
+    if (dx && dy && dz && 3 * (dx - 1) * (dy - 1) * (dz - 1) <= max_dofs) break;
     dx = dy = dz = 0;
   }
-  if (!dx) return MPPD_OK;
+  if (!dx) return false;
   CGrid cg;
   cg.hx = g.nex / dx;
   cg.hy = g.ney / dy;
@@ -151,25 +150,20 @@ int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipS
   cg.mz = dz - 1;
   cg.nq = cg.mx * cg.my * cg.mz;
   cg.nc = 3 * cg.nq;
-  const int nc = cg.nc;
+  if (cg.hx * cg.hy * cg.hz > kCoarseMaxPos) return false;
+  *out = cg;
+  return true;
+}
 
-  const int npos = cg.hx * cg.hy * cg.hz, ncel = dx * dy * dz;
-  if (npos > kCoarseMaxPos) return MPPD_OK;  // coarse elements too large for the C table
-  CoarseDev *c = new CoarseDev();
-  c->cg = cg;
+int coarse_inverse(const Grid &g, const CGrid &cg, const int *et_d, const double *ke_d, float *ainv, hipStream_t st,
+                   bool *ok) {
+  *ok = false;
+  const int nc = cg.nc, npos = cg.hx * cg.hy * cg.hz;
+  const int dx = cg.mx + 1, dy = cg.my + 1, ncel = dx * dy * (cg.mz + 1);
   double *a64 = nullptr, *C = nullptr, *KE = nullptr;
   int *info_d = nullptr;
   int rc = MPPD_OK;
   do {
-    float *ainv = nullptr;
-    if ((rc = dalloc(&ainv, (size_t)nc * nc, false))) break;
-    c->ainv = ainv;
-    if ((rc = dalloc(&c->t1, (size_t)3 * K * g.nz * g.ny * cg.mx))) break;
-    if ((rc = dalloc(&c->rc, (size_t)nc * K))) break;
-    if ((rc = dalloc(&c->ecp, (size_t)kCoarseSplit * nc * K))) break;
-    if ((rc = dalloc(&c->ec, (size_t)nc * K))) break;
-    if ((rc = dalloc(&c->z, mixed_vector_len(g, K)))) break;
-    if ((rc = dalloc(&c->prz, (size_t)3 * K * coarse_vec_blocks(g)))) break;
     if ((rc = dalloc(&a64, (size_t)nc * nc, false))) break;
     if ((rc = dalloc(&C, (size_t)npos * 3 * 576, false))) break;
     if ((rc = dalloc(&KE, (size_t)ncel * 576, false))) break;
@@ -209,7 +203,7 @@ int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipS
       rc = hip_fail(e, "coarse factor status");
       break;
     }
-    if (info != 0) break;  // not positive definite: no coarse level (c stays unused)
+    if (info != 0) break;  // not positive definite: no coarse level
     if (rocsolver_dpotri(h, rocblas_fill_lower, nc, a64, nc, info_d) != rocblas_status_success) {
       rc = fail(MPPD_ERR_HIP, "rocsolver_dpotri (coarse matrix)");
       break;
@@ -227,15 +221,49 @@ int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipS
       rc = hip_fail(e, "coarse inverse copy");
       break;
     }
-    *out = c;
+    *ok = true;
   } while (0);
   (void)hipStreamSynchronize(st);  // the temporaries below may still be in use after a failed step
   dfree(a64);
   dfree(C);
   dfree(KE);
   dfree(info_d);
-  if (*out != c) coarse_free(c);
   return rc;
+}
+
+// Builds the coarse level of a K-field system on the RVE grid g whose phase
+// per element is et_d and whose per-phase element stiffness is ke_d (3 x 576,
+// FP64, on the device).  *out = nullptr (Jacobi alone) when the grid has no
+// usable coarse grid (a side with fewer than 4 elements or prime, a side
+// beyond kCoarseMaxLine nodes), when A_c is not positive definite, or when
+// MPPD_COARSE=0.
+int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipStream_t st, CoarseDev **out) {
+  *out = nullptr;
+  CGrid cg;
+  if (!coarse_enabled() || !coarse_grid(g, kCoarseMaxDofs, &cg)) return MPPD_OK;
+  const int nc = cg.nc;
+  CoarseDev *c = new CoarseDev();
+  c->cg = cg;
+  int rc = MPPD_OK;
+  bool ok = false;
+  do {
+    float *ainv = nullptr;
+    if ((rc = dalloc(&ainv, (size_t)nc * nc, false))) break;
+    c->ainv = ainv;
+    if ((rc = dalloc(&c->t1, (size_t)3 * K * g.nz * g.ny * cg.mx))) break;
+    if ((rc = dalloc(&c->rc, (size_t)nc * K))) break;
+    if ((rc = dalloc(&c->ecp, (size_t)kCoarseSplit * nc * K))) break;
+    if ((rc = dalloc(&c->ec, (size_t)nc * K))) break;
+    if ((rc = dalloc(&c->z, mixed_vector_len(g, K)))) break;
+    if ((rc = dalloc(&c->prz, (size_t)3 * K * coarse_vec_blocks(g)))) break;
+    rc = coarse_inverse(g, cg, et_d, ke_d, ainv, st, &ok);
+  } while (0);
+  if (rc || !ok) {
+    coarse_free(c);
+    return rc;
+  }
+  *out = c;
+  return MPPD_OK;
 }
 
 }  // namespace mpp

@@ -119,6 +119,18 @@ struct CoarseDev {
 };
 constexpr int kCoarseSplit = 16;    // column splits of the coarse solve (one partial each)
 constexpr int kCoarseMaxLine = 256;  // nodes per side a coarse level allows
+constexpr int kCoarseMaxDofs = 2200;  // coarse DOFs of a multi-RHS solver (3 x 9^3 = 2187: 10^3 coarse elements)
+constexpr int kCoarseMaxPos = 4096;   // fine elements per coarse element (the set-up's table size)
+// MPPD_COARSE=0 in the environment turns the coarse levels off
+bool coarse_enabled();
+// the largest coarse grid aligned with g with at most max_dofs coarse DOFs
+// (false: none)
+bool coarse_grid(const Grid &g, int max_dofs, CGrid *out);
+// ainv (nc x nc floats, device) = the FP32 inverse of the Galerkin coarse
+// matrix of the RVE (phase map et_d, per-phase element stiffness ke_d) on cg;
+// *ok = false when that matrix is not positive definite
+int coarse_inverse(const Grid &g, const CGrid &cg, const int *et_d, const double *ke_d, float *ainv, hipStream_t st,
+                   bool *ok);
 // coarse.hip: the coarse level of a K-field system on the RVE grid g (per-phase
 // element stiffness ke_d, phase map et_d); *out = nullptr when the grid has
 // none (or MPPD_COARSE=0 in the environment)

@@ -788,26 +788,6 @@ __device__ __forceinline__ MStep m_step(const MState *cur, int r, double zz, dou
 // are short and latency decides their time.
 // ---------------------------------------------------------------------------
 
-// hat of interior coarse node q (1 .. m) at fine index i along one side
-__device__ __forceinline__ float hat(int i, int q, int h) {
-  const int dd = abs(i - q * h);
-  return dd < h ? 1.0f - (float)dd / (float)h : 0.0f;
-}
-
-// weights and interior coarse indices (0-based) of the two coarse nodes at
-// and above fine index i along one side; a node outside 1 .. m gets weight 0
-// and index 0 (a valid address)
-__device__ __forceinline__ void hat_pair(int i, int h, int m, float (&w)[2], int (&q)[2]) {
-  const int q0 = i / h;
-#pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    const int c = q0 + a;
-    const bool in = c >= 1 && c <= m;
-    w[a] = in ? hat(i, c, h) : 0.0f;
-    q[a] = in ? c - 1 : 0;
-  }
-}
-
 // One fine index along one side: its two coarse nodes (0-based interior
 // indices, clamped) and their hat weights (hat_pair)
 struct AxisW {

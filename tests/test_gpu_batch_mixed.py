@@ -112,3 +112,32 @@ def test_micropp_mixed_instance_runs_as_mixed_batch(mp, monkeypatch, slots, lin_
             assert ra[2][0] == rb[2][0] and ra[2][2] == rb[2][2]
             nl += ra[2][0]
     assert nl > 0, "the case must allocate history"
+
+
+def test_mixed_batch_coarse_level_against_jacobi(mp, monkeypatch):
+    """The two-level preconditioner of the batch's inner FP32 CG (a coarse grid per RVE from its initial
+    tangent; 4 elements per coarse element at 20^3, 2 at 12^3) against Jacobi alone (MPPD_COARSE=0 at
+    mppd_batch_create), both against the FP64 batch: J2 and damage RVEs of two sizes over two load steps.  The
+    coarse level cuts the inner iterations, and sigma / ctan keep the mixed-precision bars."""
+    descs = [dict(size=21, micro_type=mp.MIC_CILI_FIB_Z, geo=(0.25, 0, 0, 0), kind="j2", coupling=mp.FE_FULL),
+             dict(size=13, micro_type=mp.MIC_LAYER_Y, geo=(0.3, 0, 0, 0), kind="damage", coupling=mp.FE_ONE_WAY),
+             dict(size=21, micro_type=mp.MIC_SPHERE, geo=(0.3, 0, 0, 0), kind="plastic", coupling=mp.FE_FULL)]
+    kw = dict(KW, cg_rel_tol=1e-8)
+    steps = 2
+    r64, _ = run_batch(mp, descs, steps, **kw)
+    r2l, s2l = run_batch(mp, descs, steps, mixed_precision=1, **kw)
+    monkeypatch.setenv("MPPD_COARSE", "0")
+    rj, sj = run_batch(mp, descs, steps, mixed_precision=1, **kw)
+    monkeypatch.delenv("MPPD_COARSE")
+    print(f"inner iterations queued: two-level {s2l[0]:.0f}, Jacobi {sj[0]:.0f}", flush=True)
+    assert s2l[0] < 0.8 * sj[0]
+    conv_so_far = [True] * len(descs)
+    for t in range(steps):
+        for i, d in enumerate(descs):
+            conv_so_far[i] = conv_so_far[i] and bool(r64[t][2][i][2])
+            ts, tc = (1e-6, 1e-4) if conv_so_far[i] else (1e-4, 1e-2)
+            for lab, r in (("two-level", r2l), ("jacobi", rj)):
+                es, ec = rel(r[t][0][i], r64[t][0][i]), rel(r[t][1][i], r64[t][1][i])
+                print(f"step {t} rve {i} ({d['kind']}) {lab}: sigma {es:.2e} ctan {ec:.2e}", flush=True)
+                assert es < ts and ec < tc
+                assert r[t][2][i][0] == r64[t][2][i][0] and r[t][2][i][2] == r64[t][2][i][2]
