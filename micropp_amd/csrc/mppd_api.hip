@@ -78,10 +78,16 @@ struct MixedLin {
   float *A = nullptr, *kinv = nullptr, *R = nullptr, *X = nullptr, *P = nullptr, *AP = nullptr;
   double *kinv64 = nullptr, *part = nullptr;
   void *state = nullptr, *state_h = nullptr;
+  // two-level preconditioner of the inner CG from the RVE's initial (elastic)
+  // tangent (null: Jacobi alone); the tangent of a Newton iteration changes,
+  // the coarse matrix does not, so it stays a symmetric positive definite
+  // preconditioner
+  CoarseDev *coarse = nullptr;
 };
 
 static void mixed_lin_free(MixedLin *m) {
   if (!m) return;
+  coarse_free(m->coarse);
   for (void *p : {(void *)m->A, (void *)m->kinv, (void *)m->R, (void *)m->X, (void *)m->P, (void *)m->AP,
                   (void *)m->kinv64, (void *)m->part, m->state})
     dfree(p);
@@ -1128,8 +1134,22 @@ int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6]
       char *st = nullptr;
       if ((rc = dalloc(&st, mixed_state_bytes()))) break;
       ml->state = st;
-      const hipError_t e = hipHostMalloc(&ml->state_h, mixed_state_bytes(), hipHostMallocDefault);
-      if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc");
+      hipError_t e = hipHostMalloc(&ml->state_h, mixed_state_bytes(), hipHostMallocDefault);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "hipHostMalloc");
+        break;
+      }
+      if (!coarse_enabled()) break;
+      // the coarse level from the per-phase element stiffness at u = 0 without history
+      double *ke = nullptr;
+      if ((rc = dalloc(&ke, 3 * 576, false))) break;
+      e = launch_elem_ke(g, s->r->bmat, s->r->mats_d, s->r->et, nullptr, nullptr, 1, ke, s->st);
+      if (e != hipSuccess)
+        rc = hip_fail(e, "coarse level: element stiffness");
+      else
+        rc = coarse_setup(g, s->r->et, ke, K, s->st, &ml->coarse);
+      (void)hipStreamSynchronize(s->st);
+      dfree(ke);
     } while (0);
     if (!rc && alloc_fault_injected()) rc = fail(MPPD_ERR_NOMEM, "injected allocation failure (MPPD_FAIL_ALLOC)");
     if (rc) {
@@ -1172,6 +1192,7 @@ int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6]
       const double rel = its == p->nr_max_its - 1 ? p->cg_rel_tol * kMixedLastStepTighten : p->cg_rel_tol;
       MPP_HIP(launch_m_resid_ell(g, K, A->cols, A->vals, s->b, s->du, ml->kinv64, ml->kinv, ml->R, ml->X, ml->P,
                                  ml->part, outer == 0, rel, p->cg_abs_tol, inner_tol, max_inner, ml->state, s->st));
+      if (ml->coarse) MPP_HIP(launch_c_start(g, K, *ml->coarse, ml->kinv, ml->R, ml->P, ml->state, s->st));
       MPP_HIP(hipMemcpyAsync(ml->state_h, ml->state, mixed_state_bytes(), hipMemcpyDeviceToHost, s->st));
       MPP_HIP(hipStreamSynchronize(s->st));
       int all_conv = 0;
@@ -1189,7 +1210,7 @@ int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6]
       int inner = 0;
       bool finite = true;
       if ((rc = mixed_inner(g, K, ml->A, ml->kinv, ml->R, ml->X, ml->P, ml->AP, ml->part, ml->state, ml->state_h,
-                            max_inner, s->st, &inner, &finite, true)))
+                            max_inner, s->st, &inner, &finite, true, ml->coarse)))
         return rc;
       solver_its += inner;
       ++outers;
