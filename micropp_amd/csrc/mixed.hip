@@ -695,18 +695,29 @@ __global__ void __launch_bounds__(256) k_m_spmv(Grid g, int NP, int nsl, int per
 // [(field * 3 + d) * gridDim.x + block] (the 3 gridDim.x partials of a field
 // contiguous; pr = pz + 3 K gridDim.x).  X is advanced in k_m_pupd, which
 // reads P anyway.
-template <int K>
+// COARSE (two-level preconditioner): the block holds whole x-lines, lpb of
+// them (lpb nx <= kVecNodes nodes), so that it can also restrict its new r
+// along x from LDS: T1 as k_c_rx writes it, the same sums in the same order.
+// Its partials then number ceil(ny nz / lpb) per (field, d) (gridDim.x).
+template <int K, bool COARSE>
 __global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const double *part1, const float *kinv,
                                                        const float4 *AP, float4 *R, double *pz, double *pr,
-                                                       MState *cur) {
+                                                       MState *cur, Grid g, CGrid cg, int lpb, float4 *T1) {
   __shared__ double lds[4 * 8];
   __shared__ double al_s[4];
+  __shared__ float4 rl[COARSE ? kVecNodes : 1];
   constexpr int KG = K / 4;
   const int c = blockIdx.y, grp = c % KG, d = c / KG, r0 = grp * 4;
   if (cur->all_done) return;
   if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole block
   constexpr int J = kVecNodes / 256;
-  const int n0 = blockIdx.x * kVecNodes + threadIdx.x;
+  int nbeg = blockIdx.x * kVecNodes, nend = min(nbeg + kVecNodes, NP), l0 = 0, l1 = 0;
+  if constexpr (COARSE) {
+    l0 = blockIdx.x * lpb;
+    l1 = min(l0 + lpb, g.ny * g.nz);
+    nbeg = l0 * g.nx;
+    nend = l1 * g.nx;
+  }
   const double pap = rows_sum<4, 8>(part1 + (size_t)r0 * nsl, nsl, nsl);  // one partial per slice
   if (threadIdx.x < 4) {
     const int r = r0 + threadIdx.x;
@@ -719,8 +730,8 @@ __global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const do
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int jn = 0; jn < J; ++jn) {
-    const int n = n0 + jn * 256;
-    if (n >= NP) break;
+    const int n = nbeg + threadIdx.x + jn * 256;
+    if (n >= nend) break;
     const size_t o = (size_t)c * NP + n;
     const float kv = kinv[(size_t)d * NP + n];
     const float4 ap = AP[o];
@@ -730,6 +741,7 @@ __global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const do
     r.z -= al.z * ap.z;
     r.w -= al.w * ap.w;
     R[o] = r;
+    if constexpr (COARSE) rl[n - nbeg] = r;
     const float rv[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -751,8 +763,28 @@ __global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const do
     const size_t o = ((size_t)(r0 + (q & 3)) * 3 + d) * gridDim.x + blockIdx.x;
     if (q < 4)
       pz[o] = t;
-    else
+    else if (!COARSE)  // the coarse variant's (r, z) partials come from k_c_zdot
       pr[o] = t;
+  }
+  if constexpr (COARSE) {
+    // restriction along x of the block's lines (interior lines only)
+    for (int idx = threadIdx.x; idx < (l1 - l0) * cg.mx; idx += kRedThreads) {
+      const int line = idx / cg.mx, cx = idx % cg.mx + 1, L = l0 + line;
+      const int j = L % g.ny, k = L / g.ny;
+      if (j == 0 || k == 0 || j == g.ny - 1 || k == g.nz - 1) continue;
+      const float4 *rline = rl + line * g.nx;
+      const int i0 = (cx - 1) * cg.hx + 1, n = 2 * cg.hx - 1;
+      float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int s2 = 0; s2 < n; ++s2) {
+        const float w = hat(i0 + s2, cx, cg.hx);
+        const float4 rv = rline[i0 + s2];
+        acc.x += w * rv.x;
+        acc.y += w * rv.y;
+        acc.z += w * rv.z;
+        acc.w += w * rv.w;
+      }
+      T1[((size_t)(c * g.nz + k) * g.ny + j) * cg.mx + (cx - 1)] = acc;
+    }
   }
 }
 
@@ -916,7 +948,7 @@ __global__ void __launch_bounds__(256) k_c_gemv(CGrid cg, const float *__restric
   float acc[K];
 #pragma unroll
   for (int f = 0; f < K; ++f) acc[f] = 0.0f;
-#pragma unroll 4
+#pragma unroll 16
   for (int col = c0 + w; col < c1; col += 4) {
     const float a = ainv[(size_t)col * nc + rowc];
     const float *x = rc + (size_t)col * K;
@@ -950,13 +982,16 @@ __global__ void __launch_bounds__(256) k_c_gemv_sum(CGrid cg, const float *ecp, 
 
 // z = D^-1 r + P_c e into Z (START: into P, the inner CG's first direction)
 // and the partials of (r, z) in k_m_upd's layout pr[(field * 3 + d) *
-// gridDim.x + block] (grid as k_m_upd).  Loads are clamped to valid nodes
-// (their contributions dropped), so every load of a thread is in flight at
-// once.
+// gridDim.x + block] (grid as k_m_upd: kVecNodes nodes per block, here on
+// kZdotThreads threads, so that each thread's rounds of dependent gathers are
+// few).  Loads are clamped to valid nodes (their contributions dropped).
+constexpr int kZdotThreads = 512;
 template <int K, bool START>
-__global__ void __launch_bounds__(kRedThreads) k_c_zdot(Grid g, CGrid cg, int NP, const float *kinv, const float4 *R,
-                                                        const float4 *ec4, float4 *Z, double *pr, const MState *cur) {
-  __shared__ double lds[4 * 4];
+__global__ void __launch_bounds__(kZdotThreads) k_c_zdot(Grid g, CGrid cg, int NP, const float *kinv,
+                                                         const float4 *R, const float4 *ec4, float4 *Z, double *pr,
+                                                         const MState *cur) {
+  constexpr int kW = kZdotThreads / 64;
+  __shared__ double lds[kW * 4];
   constexpr int KG = K / 4;
   const int c = blockIdx.y, grp = c % KG, d = c / KG, r0 = grp * 4;
   if (!START) {
@@ -965,16 +1000,16 @@ __global__ void __launch_bounds__(kRedThreads) k_c_zdot(Grid g, CGrid cg, int NP
   }
   __shared__ AxisW tab[3][kCoarseMaxLine];
   axis_tables(g, cg, tab);
-  constexpr int J = kVecNodes / 256;
+  constexpr int J = kVecNodes / kZdotThreads;
   const int n0 = blockIdx.x * kVecNodes + threadIdx.x;
-  // the node's (i, j, k), stepped by 256 nodes per round (nodes past the
-  // grid, k >= nz, read the last plane's entries; their part is dropped)
+  // the node's (i, j, k), stepped by kZdotThreads nodes per round (nodes past
+  // the grid, k >= nz, read the last plane's entries; their part is dropped)
   int i = n0 % g.nx, j = (n0 / g.nx) % g.ny, k = n0 / (g.nx * g.ny);
-  const int di = 256 % g.nx, dl = 256 / g.nx;
+  const int di = kZdotThreads % g.nx, dl = kZdotThreads / g.nx;
   double v[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int jn = 0; jn < J; ++jn) {
-    const int n = n0 + jn * 256;
+    const int n = n0 + jn * kZdotThreads;
     const int nl = min(n, NP - 1);
     const size_t o = (size_t)c * NP + nl;
     const float kv = kinv[(size_t)d * NP + nl];
@@ -1010,7 +1045,10 @@ __global__ void __launch_bounds__(kRedThreads) k_c_zdot(Grid g, CGrid cg, int NP
   __syncthreads();
   if (threadIdx.x < 4) {
     const int q = threadIdx.x;
-    pr[((size_t)(r0 + q) * 3 + d) * gridDim.x + blockIdx.x] = lds[q] + lds[4 + q] + lds[8 + q] + lds[12 + q];
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kW; ++w) t += lds[w * 4 + q];
+    pr[((size_t)(r0 + q) * 3 + d) * gridDim.x + blockIdx.x] = t;
   }
 }
 
@@ -1041,7 +1079,8 @@ __global__ void __launch_bounds__(256) k_c_sym32(int n, const double *a64, float
 // being formed from kinv and R.
 template <int K, bool COARSE>
 __global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kinv, const float4 *R, const float4 *Z,
-                                                        float4 *P, float4 *X, const double *pz, const MState *cur,
+                                                        float4 *P, float4 *X, const double *pz, int nbz,
+                                                        const double *pr, const MState *cur,
                                                         MState *nxt) {
   constexpr int KG = K / 4;
   __shared__ double beta_s[kMaxRhs];
@@ -1078,11 +1117,10 @@ __global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kin
     pvv[jn] = P[o];
     xvv[jn] = X[o];
   }
-  const size_t row = (size_t)3 * gridDim.x;  // partials per field
-  const double *pr = pz + (size_t)K * row;
+  const size_t row = (size_t)3 * gridDim.x, rowz = (size_t)3 * nbz;  // (r, z) / (z, z) partials per field
   const int its = cur->its + 1;
   if (head) {
-    const double zz = rows_sum<K>(pz, row, (int)row);
+    const double zz = rows_sum<K>(pz, rowz, (int)rowz);
     const double rz = rows_sum<K>(pr, row, (int)row);
     const int r = threadIdx.x;
     if (r < K) {
@@ -1109,7 +1147,7 @@ __global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kin
       nxt->all_done = all || its >= cur->max_its;
     }
   } else {
-    const double zz = rows_sum<4>(pz + r0 * row, row, (int)row);
+    const double zz = rows_sum<4>(pz + r0 * rowz, rowz, (int)rowz);
     const double rz = rows_sum<4>(pr + r0 * row, row, (int)row);
     if (threadIdx.x < 4) {
       const MStep o = m_step(cur, r0 + threadIdx.x, zz, rz);
@@ -1594,12 +1632,13 @@ hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke,
 // restriction and coarse solve of R into c.ec (state cur: skipped once every field has stopped)
 template <int KK>
 static void coarse_solve(const Grid &g, const MLaunch &s, const CoarseDev &c, const float *R, const MState *cur,
-                         hipStream_t st) {
+                         hipStream_t st, bool rx = true) {
   constexpr int KG = KK / 4;
   const CGrid &cg = c.cg;
   const size_t rx_threads = (size_t)3 * KG * (g.ny - 2) * (g.nz - 2) * cg.mx;
-  hipLaunchKernelGGL(k_c_rx<KK>, dim3(nblk(rx_threads, 256)), dim3(256), 0, st, g, cg, s.NP,
-                     reinterpret_cast<const float4 *>(R), reinterpret_cast<float4 *>(c.t1), cur);
+  if (rx)  // else T1 is already made (k_m_upd<K, true>)
+    hipLaunchKernelGGL(k_c_rx<KK>, dim3(nblk(rx_threads, 256)), dim3(256), 0, st, g, cg, s.NP,
+                       reinterpret_cast<const float4 *>(R), reinterpret_cast<float4 *>(c.t1), cur);
   hipLaunchKernelGGL(k_c_ryz<KK>, dim3(nblk((size_t)cg.nq * 3 * KG, 4)), dim3(256), 0, st, g, cg,
                      reinterpret_cast<const float4 *>(c.t1), reinterpret_cast<float4 *>(c.rc), cur);
   hipLaunchKernelGGL(k_c_gemv<KK>, dim3(nblk(cg.nc, 64), kCoarseSplit), dim3(256), 0, st, cg, c.ainv, c.rc, c.ecp,
@@ -1615,30 +1654,39 @@ hipError_t launch_m_iter(const Grid &g, int K, const float *A, const float *kinv
   MState *cur = ms + (parity & 1);
   MState *nxt = ms + ((parity + 1) & 1);
   const int nbx = (int)nblk(s.NP, kVecNodes);
-  double *part1 = part;                          // K x nsl
-  double *pz = part + (size_t)K * s.nsl;         // 3K x nbx
-  double *pr = pz + (size_t)3 * K * nbx;         // 3K x nbx
+  // with a coarse level k_m_upd runs on blocks of whole x-lines (lpb a block, nbu blocks)
+  const int lpb = coarse ? kVecNodes / g.nx : 0;
+  const int nbu = coarse ? (g.ny * g.nz + lpb - 1) / lpb : nbx;
+  double *part1 = part;                                   // K x nsl
+  double *pz = part + (size_t)K * s.nsl;                  // 3K x nbu: (z, z)
+  double *pr = pz + (size_t)3 * K * std::max(nbx, nbu);   // 3K x nbx: (r, z)
   const dim3 vgrid(nbx, 3 * K / 4);
   const CGrid cg = coarse ? coarse->cg : CGrid{};
   MPP_K_DISPATCH(K, {
     hipLaunchKernelGGL(k_m_spmv<KK>, dim3(8 * s.pers), dim3(16 * KK), 0, st, g, s.NP, s.nsl, s.pers,
                        reinterpret_cast<const float4 *>(A), reinterpret_cast<const float4 *>(P),
                        reinterpret_cast<float4 *>(AP), part1, cur);
-    hipLaunchKernelGGL(k_m_upd<KK>, vgrid, dim3(kRedThreads), 0, st, s.NP, s.nsl, part1, kinv,
-                       reinterpret_cast<const float4 *>(AP), reinterpret_cast<float4 *>(R), pz, pr, cur);
     if (coarse) {
-      // z and (r, z) with the coarse correction; its partials replace k_m_upd's Jacobi (r, z) ones
-      coarse_solve<KK>(g, s, *coarse, R, cur, st);
-      hipLaunchKernelGGL((k_c_zdot<KK, false>), vgrid, dim3(kRedThreads), 0, st, g, cg, s.NP, kinv,
+      // R -= alpha AP with the x-restriction of the new R; then the coarse
+      // solve, z and (r, z) with the coarse correction (its partials in
+      // place of the Jacobi (r, z) ones), and the p-update from z
+      hipLaunchKernelGGL((k_m_upd<KK, true>), dim3(nbu, 3 * K / 4), dim3(kRedThreads), 0, st, s.NP, s.nsl, part1,
+                         kinv, reinterpret_cast<const float4 *>(AP), reinterpret_cast<float4 *>(R), pz, pr, cur, g,
+                         cg, lpb, reinterpret_cast<float4 *>(coarse->t1));
+      coarse_solve<KK>(g, s, *coarse, R, cur, st, false);
+      hipLaunchKernelGGL((k_c_zdot<KK, false>), vgrid, dim3(kZdotThreads), 0, st, g, cg, s.NP, kinv,
                          reinterpret_cast<const float4 *>(R), reinterpret_cast<const float4 *>(coarse->ec),
                          reinterpret_cast<float4 *>(coarse->z), pr, cur);
       hipLaunchKernelGGL((k_m_pupd<KK, true>), vgrid, dim3(kRedThreads), 0, st, s.NP, kinv,
                          reinterpret_cast<const float4 *>(R), reinterpret_cast<const float4 *>(coarse->z),
-                         reinterpret_cast<float4 *>(P), reinterpret_cast<float4 *>(X), pz, cur, nxt);
+                         reinterpret_cast<float4 *>(P), reinterpret_cast<float4 *>(X), pz, nbu, pr, cur, nxt);
     } else {
+      hipLaunchKernelGGL((k_m_upd<KK, false>), vgrid, dim3(kRedThreads), 0, st, s.NP, s.nsl, part1, kinv,
+                         reinterpret_cast<const float4 *>(AP), reinterpret_cast<float4 *>(R), pz, pr, cur, g, cg,
+                         0, nullptr);
       hipLaunchKernelGGL((k_m_pupd<KK, false>), vgrid, dim3(kRedThreads), 0, st, s.NP, kinv,
                          reinterpret_cast<const float4 *>(R), nullptr, reinterpret_cast<float4 *>(P),
-                         reinterpret_cast<float4 *>(X), pz, cur, nxt);
+                         reinterpret_cast<float4 *>(X), pz, nbx, pr, cur, nxt);
     }
   });
   return hipGetLastError();
@@ -1652,7 +1700,7 @@ hipError_t launch_c_start(const Grid &g, int K, const CoarseDev &c, const float 
   const dim3 vgrid(nbx, 3 * K / 4);
   MPP_K_DISPATCH(K, {
     coarse_solve<KK>(g, s, c, R, s0, st);
-    hipLaunchKernelGGL((k_c_zdot<KK, true>), vgrid, dim3(kRedThreads), 0, st, g, c.cg, s.NP, kinv,
+    hipLaunchKernelGGL((k_c_zdot<KK, true>), vgrid, dim3(kZdotThreads), 0, st, g, c.cg, s.NP, kinv,
                        reinterpret_cast<const float4 *>(R), reinterpret_cast<const float4 *>(c.ec),
                        reinterpret_cast<float4 *>(P), c.prz, s0);
     hipLaunchKernelGGL(k_c_rzfix<KK>, dim3(1), dim3(kRedThreads), 0, st, nbx, c.prz, s0);
