@@ -497,6 +497,54 @@ __device__ __forceinline__ T row_dot_grid(const Grid &g, const T *vals, const T 
   return acc;
 }
 
+// The gathers of row_dot_grid for a block of 256 consecutive rows, staged in
+// LDS: the block's rows belong to nodes n_lo .. n_hi (at most 86), whose
+// stencil neighbours lie on 9 runs of consecutive nodes (one per (dz, dy)),
+// each n_hi - n_lo + 3 <= kSegNodes long.  The block copies those runs
+// (clamped node indices, as row_dot_grid clamps each neighbour) into seg once,
+// coalesced, and every row reads its 81 values there instead of gathering
+// them from L1/L2: the same values in the same slot order, so the result is
+// bit-identical to row_dot_grid.  Every thread of the block must call it.
+constexpr int kSegNodes = 88;
+template <typename T>
+__device__ __forceinline__ void stage_grid_runs(const Grid &g, int row0, const T *x, T (*seg)[kSegNodes * 3]) {
+  const int n_lo = row0 / 3, n_hi = (row0 + 255) / 3, len = (n_hi - n_lo + 3) * 3;
+  const int nxny = g.nx * g.ny, hi = g.nn - 1;
+  for (int idx = threadIdx.x; idx < 9 * len; idx += 256) {
+    const int sgi = idx / len, e3 = idx - sgi * len;
+    const int base = n_lo - 1 + (sgi / 3 - 1) * nxny + (sgi % 3 - 1) * g.nx;
+    const int m = min(max(base + e3 / 3, 0), hi);
+    seg[sgi][e3] = x[3 * m + e3 % 3];
+  }
+  __syncthreads();
+}
+
+template <int CH = kChunk, typename T>
+__device__ __forceinline__ T row_dot_grid_seg(const T *vals, const T (*seg)[kSegNodes * 3], int row0, int row) {
+#pragma clang fp contract(off)
+  static_assert(kSlots % CH == 0 && CH % 3 == 0, "chunks tile the row in whole neighbours");
+  const size_t base = (size_t)(row >> 6) * kSlots * 64 + (row & 63);
+  const T *v = vals + base;
+  const int e0 = row / 3 - row0 / 3;  // the row's node within the runs, less 1
+  T acc = 0;
+#pragma unroll
+  for (int ch = 0; ch < kSlots / CH; ++ch) {
+    T vi[CH], xv[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) vi[k] = __builtin_nontemporal_load(&v[(ch * CH + k) * 64]);
+#pragma unroll
+    for (int k = 0; k < CH; k += 3) {
+      const int nb = (ch * CH + k) / 3;
+      const T *sv = &seg[nb / 3][(e0 + nb % 3) * 3];
+#pragma unroll
+      for (int fj = 0; fj < 3; ++fj) xv[k + fj] = sv[fj];
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) acc += vi[k] * xv[k];
+  }
+  return acc;
+}
+
 // ---------------------------------------------------------------------------
 // ell_solve_cgpd (src/ell.cpp:66-122), the reference recurrence in three
 // kernels per iteration:
@@ -574,10 +622,13 @@ __device__ __forceinline__ void cg_step1_body(const Grid &g, int lb, const int *
   if (cur->done) return;
   const int row = lb * 256 + threadIdx.x;
   T y;
-  if constexpr (STENCIL)
-    y = row_dot_grid<CH>(g, vals, p, row);
-  else
+  if constexpr (STENCIL) {
+    __shared__ T seg[9][kSegNodes * 3];
+    stage_grid_runs(g, lb * 256, p, seg);
+    y = row_dot_grid_seg<CH>(vals, seg, lb * 256, row);
+  } else {
     y = row_dot<CH>(cols, vals, p, row);
+  }
   Ap[row] = y;
   double v[1] = {(double)p[row] * (double)y};
   block_sum<1, 256>(v, lds);
