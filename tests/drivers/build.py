@@ -3,9 +3,11 @@
 c_driver.c (gcc) and f_driver.f90 (ROCm flang, when present, with the
 drop-in modules micropp_amd/fortran/libmaterial.f95 and libmicropp.f95) link
 micropp_amd/libmicropp_amd.so with an rpath relative to the binary, so the
-built tree can move.  Run after the library is built:
+built tree can move.  sanitize_driver.cpp links a sanitizer build of the
+library (clang++, the same -fsanitize).  Run after the library is built:
 
-    python tests/drivers/build.py
+    python tests/drivers/build.py            # C and Fortran drivers
+    python tests/drivers/build.py address    # sanitize_driver_asan (or: thread)
 """
 from __future__ import annotations
 
@@ -28,6 +30,22 @@ def flang():
     return None
 
 
+CLANGXX = "/opt/rocm/llvm/bin/clang++"
+
+
+def build_sanitize(sanitize: str) -> str:
+    """sanitize_driver.cpp against libmicropp_amd_<asan|tsan>.so (python -m micropp_amd.build --sanitize ...),
+    with the same sanitizer, whose runtime clang links into the executable."""
+    tag = {"address": "asan", "thread": "tsan"}[sanitize]
+    os.makedirs(BIN, exist_ok=True)
+    exe = os.path.join(BIN, f"sanitize_driver_{tag}")
+    cmd = [CLANGXX, "-std=c++17", "-O1", "-g", f"-fsanitize={sanitize}", "-fno-omit-frame-pointer", "-Wno-non-c-typedef-for-linkage",
+           f"-I{ROOT}/include", "-o", exe, os.path.join(HERE, "sanitize_driver.cpp"), f"-L{LIBDIR}",
+           f"-lmicropp_amd_{tag}", RPATH]
+    subprocess.run(cmd, check=True)
+    return exe
+
+
 def build() -> dict:
     os.makedirs(BIN, exist_ok=True)
     out = {}
@@ -47,4 +65,6 @@ def build() -> dict:
 
 
 if __name__ == "__main__":
-    print(build())
+    import sys
+
+    print(build_sanitize(sys.argv[1]) if len(sys.argv) > 1 else build())
