@@ -209,7 +209,7 @@ def batched_lines():
     out = {}
     out["batch_j2"] = batch_j2(64, 2, mixed=True)
     out["batch_damage"] = batch_damage(16, 3, mixed=True)
-    out["adjoint"] = adjoint_workload(4, 0)
+    out["adjoint"] = adjoint_workload(16, 0)
     return out
 
 
@@ -370,26 +370,39 @@ def batch_damage(nrve, steps, mixed=False):
     return out
 
 
-def adjoint_workload(nrve, _steps):
+def adjoint_workload(nrve, _steps, fd_rves=4):
     """Elastic RVEs of 40^3 hexes with 1-10 seeded spheres (radii U[0.05, 0.2]): d sigma / d eps and
-    d sigma / d (E, nu) per phase from the adjoint (one multi-RHS solve: 6 adjoint + 2 forward fields) vs the
-    forward approach (6 perturbed solves for ctan + central differences for the 4 material parameters)."""
+    d sigma / d (E, nu) per phase from the adjoint (one multi-RHS solve per RVE: 6 adjoint + 2 forward fields),
+    the whole batch at once (mppd_msolver_adjoint_many, 4 host threads, each solver on its own stream) and one RVE
+    after another, vs the forward approach (6 perturbed solves for ctan + central differences for the 4 material
+    parameters) on the first fd_rves RVEs."""
     import micropp_amd as mp
 
     nrve = nrve or 4
     spec = [(0, 1e6, 0.3), (0, 1e7, 0.25), (0, 1e7, 0.25)]
+    rv = sphere_rves(nrve)
+    rves = [mp.Rve(41, mp.MIC3D_RANDOM_SPHERES, geo, [mp.material(*s) for s in spec]) for geo, _ in rv]
+    ms = [mp.MSolver(r, 8) for r in rves]
+    strains = np.array([s for _, s in rv])
+    mp.load().mppd_synchronize()
+    t0 = time.perf_counter()
+    ctan_b, sig_b, dmat_b, _, _, ok_b = mp.adjoint_many(ms, strains, workers=4)
+    t_many = time.perf_counter() - t0
+    print("adjoint batch done", file=sys.stderr, flush=True)
     t_adj = t_fwd = 0.0
     err_ctan = err_fd = 0.0
-    for i, (geo, strains) in enumerate(sphere_rves(nrve)):
-        r = mp.Rve(41, mp.MIC3D_RANDOM_SPHERES, geo, [mp.material(*s) for s in spec])
-        ms = mp.MSolver(r, 8)
+    same = ok_b
+    for i, ((geo, st), m) in enumerate(zip(rv, ms)):
         t0 = time.perf_counter()
-        ctan, sig, dmat, _, _, ok = ms.adjoint(strains)
+        ctan, sig, dmat, _, _, ok = m.adjoint(st)
         t_adj += time.perf_counter() - t0
+        same = same and ok and np.array_equal(ctan, ctan_b[i]) and np.array_equal(dmat, dmat_b[i])
+        if i >= fd_rves:
+            continue
         # forward: ctan from 6 unit-strain solves + sigma, then central differences (step 1e-6 relative) for
         # (E, nu) of phases 0 and 1, each a fresh RVE and solve
         t0 = time.perf_counter()
-        sg, ctan_f, _, _, _ = mp.homogenize_mixed(ms, strains)
+        sg, ctan_f, _, _, _ = mp.homogenize_mixed(m, st)
         fd = np.zeros((2, 6, 2, 2))
         for p in range(2):
             for w in (1, 2):
@@ -399,17 +412,20 @@ def adjoint_workload(nrve, _steps):
                     sp = [list(s) for s in spec]
                     sp[p][w] += sgn * h
                     rr = mp.Rve(41, mp.MIC3D_RANDOM_SPHERES, geo, [mp.material(*s) for s in sp])
-                    res.append(mp.MSolver(rr, 8).solve(strains, rel_tol=1e-12, max_outer=30)[0])
+                    res.append(mp.MSolver(rr, 8).solve(st, rel_tol=1e-12, max_outer=30)[0])
                 fd[:, :, p, w - 1] = (res[0] - res[1]) / (2 * h)
         t_fwd += time.perf_counter() - t0
         err_ctan = max(err_ctan, rel(ctan, ctan_f))
         err_fd = max(err_fd, rel(dmat[:, :, :2, :], fd))
         print(f"adjoint RVE {i} done", file=sys.stderr, flush=True)
-    return {"metric": "adjoint derivatives (40^3 elastic RVEs), RVEs/s", "value": nrve / t_adj,
+    nfd = min(fd_rves, nrve)
+    return {"metric": "adjoint derivatives (40^3 elastic RVEs), RVEs/s", "value": nrve / t_many,
             "unit": "RVEs/s (ctan + d sigma/d(E,nu) for 2 strains)", "rves": nrve,
-            "adjoint_ms_per_rve": 1e3 * t_adj / nrve, "forward_fd_ms_per_rve": 1e3 * t_fwd / nrve,
-            "speedup_vs_forward": t_fwd / t_adj, "ctan_rel_err_vs_forward": err_ctan,
-            "dmat_rel_err_vs_central_fd": err_fd}
+            "path": "mppd_msolver_adjoint_many, 4 host threads, one stream per solver",
+            "batch_ms_per_rve": 1e3 * t_many / nrve, "one_by_one_ms_per_rve": 1e3 * t_adj / nrve,
+            "batch_vs_one_by_one": t_adj / t_many, "batch_equals_one_by_one": bool(same),
+            "forward_fd_ms_per_rve": 1e3 * t_fwd / nfd, "speedup_vs_forward": (t_fwd / nfd) / (t_many / nrve),
+            "forward_compared_rves": nfd, "ctan_rel_err_vs_forward": err_ctan, "dmat_rel_err_vs_central_fd": err_fd}
 
 
 def newton_mixed_workload(nrve, steps):

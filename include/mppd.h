@@ -206,6 +206,17 @@ int mppd_msolver_newton(mppd_msolver *m, int n, const double *eps_h, const doubl
  * final residual ratios.  n = 0 gives ctan alone. */
 int mppd_msolver_adjoint(mppd_msolver *m, int n, const double *eps_h, const mppd_mixed_params *p, double *sig_h,
                          double *ctan_h, double *dmat_h, int stats[2], double *rel_h);
+/* The adjoint derivatives of a batch of RVEs (no reference counterpart):
+ * mppd_msolver_adjoint of each of the nm distinct solvers ms[0 .. nm-1], all
+ * with the same n and parameters, run concurrently: up to `workers` host
+ * threads (0: 4) take solvers in turn, each solver on its own stream, all on
+ * the calling thread's device.  Per-solver arrays are concatenated in solver
+ * order: eps_h and sig_h n * 6 each, ctan_h 36, dmat_h n * 36, stats 2, rel_h
+ * 6 + n (stats and rel_h may be NULL).  Each solver's results equal
+ * mppd_msolver_adjoint's bit for bit.  On failure every solver still runs and
+ * the error of the lowest-numbered failing one is returned. */
+int mppd_msolver_adjoint_many(int nm, mppd_msolver *const *ms, int n, const double *eps_h, const mppd_mixed_params *p,
+                              int workers, double *sig_h, double *ctan_h, double *dmat_h, int *stats, double *rel_h);
 /* timing aid: average ms of the FP32 multi-RHS SpMV kernel and of a whole
  * inner iteration, over iters launches, events on the solver's stream */
 int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *iter_ms);

@@ -146,6 +146,7 @@ SIGNATURES = {
     "mppd_msolver_solve": (_I, [_VP, _I, _PD, C.c_void_p, _PD, _PI, _PD]),
     "mppd_msolver_get_u": (_I, [_VP, _I, _PD]),
     "mppd_msolver_adjoint": (_I, [_VP, _I, _PD, C.c_void_p, _PD, _PD, _PD, _PI, _PD]),
+    "mppd_msolver_adjoint_many": (_I, [_I, C.c_void_p, _I, _PD, C.c_void_p, _I, _PD, _PD, _PD, _PI, _PD]),
     "mppd_msolver_profile": (_I, [_VP, _I, _PD, _PD]),
     "mppd_msolver_newton": (_I, [_VP, _I, _PD, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _PD, _PI, _PI]),
     "mppd_newton_mixed": (_I, [_VP, _VP, _PD, _VP, _VP, C.POINTER(NewtonParams), _VP, _PI, _PI]),
@@ -624,6 +625,29 @@ class MSolver:
         a, t = C.c_double(), C.c_double()
         _chk(load().mppd_msolver_profile(self.h, iters, C.byref(a), C.byref(t)), "mppd_msolver_profile")
         return a.value, t.value
+
+
+def adjoint_many(msolvers, strains=None, workers=0, rel_tol=1e-8, abs_tol=1e-50, inner_tol=1e-5, max_inner_its=2000,
+                 max_outer=10):
+    """MSolver.adjoint of many RVEs at once (mppd_msolver_adjoint_many: up to `workers` host threads, 0: 4, each
+    solver on its own stream).  strains: None or (nm, n, 6).  Returns (ctan nm x 6 x 6, sigma nm x n x 6,
+    dsigma/d(E, nu) nm x n x 6 x 3 x 2, stats nm x 2, residual ratios nm x (6 + n), converged)."""
+    nm = len(msolvers)
+    e = np.zeros((nm, 0, 6)) if strains is None else np.ascontiguousarray(strains, dtype=np.float64).reshape(nm, -1, 6)
+    n = e.shape[1]
+    ctan = np.zeros((nm, 36))
+    sig = np.zeros((nm, max(n, 1), 6))
+    dmat = np.zeros((nm, max(n, 1), 36))
+    stats = np.zeros((nm, 2), dtype=np.int32)
+    rel = np.zeros((nm, 6 + n))
+    p = MixedParams(rel_tol, abs_tol, inner_tol, max_inner_its, max_outer)
+    hs = (C.c_void_p * nm)(*[m.h for m in msolvers])
+    e_arg = np.ascontiguousarray(e if n else np.zeros((nm, 1, 6)))
+    rc = load().mppd_msolver_adjoint_many(nm, C.cast(hs, C.c_void_p), n, _dp(e_arg), C.cast(C.byref(p), C.c_void_p),
+                                          int(workers), _dp(sig), _dp(ctan), _dp(dmat), _ip(stats), _dp(rel))
+    if rc not in (0, MSolver.NOT_CONVERGED):
+        _chk(rc, "mppd_msolver_adjoint_many")
+    return (ctan.reshape(nm, 6, 6), sig[:, :n], dmat[:, :n].reshape(nm, n, 6, 3, 2), stats, rel, rc == 0)
 
 
 def homogenize_mixed(msolver: MSolver, strains, d_eps=1e-8, **kw):

@@ -5,12 +5,15 @@
 // assembly.hip and solver.hip.  The host reads back only the scalars that
 // steer control flow (residual norms, CG state, nonlinearity flags) and the
 // 6-component averages.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -1247,6 +1250,65 @@ int mppd_newton_mixed(mppd_solver *s, const mppd_solver *A0, const double eps[6]
     stats[2] = fallbacks;
     stats[3] = short_solves;
   }
+  return MPPD_OK;
+}
+
+int mppd_msolver_adjoint_many(int nm, mppd_msolver *const *ms, int n, const double *eps_h, const mppd_mixed_params *p,
+                              int workers, double *sig_h, double *ctan_h, double *dmat_h, int *stats, double *rel_h) {
+  if (nm < 1 || !ms || !p || !ctan_h || n < 0 || workers < 0 || (n > 0 && (!eps_h || !sig_h || !dmat_h)))
+    return fail(MPPD_ERR_ARG, "bad msolver_adjoint_many arguments");
+  for (int i = 0; i < nm; ++i) {
+    if (!ms[i]) return fail(MPPD_ERR_ARG, "msolver_adjoint_many: null solver " + std::to_string(i));
+    if (6 + n > ms[i]->K) return fail(MPPD_ERR_ARG, "msolver_adjoint_many: solver " + std::to_string(i) + " needs 6 + n <= K");
+  }
+  {  // each solver runs on its own stream from one thread at a time: a solver given twice would race with itself
+    std::vector<mppd_msolver *> v(ms, ms + nm);
+    std::sort(v.begin(), v.end());
+    if (std::adjacent_find(v.begin(), v.end()) != v.end())
+      return fail(MPPD_ERR_ARG, "msolver_adjoint_many: a solver appears twice");
+  }
+  int dev = 0;
+  MPP_HIP(hipGetDevice(&dev));
+  const int W = std::min(nm, workers ? workers : 4);
+  std::atomic<int> next{0};
+  std::mutex mu;
+  int first_i = nm, first_rc = MPPD_OK;
+  std::string first_msg;
+  auto work = [&]() {
+    if (hipSetDevice(dev) != hipSuccess) {
+      std::lock_guard<std::mutex> lk(mu);
+      if (first_rc == MPPD_OK) {
+        first_rc = MPPD_ERR_HIP;
+        first_msg = "hipSetDevice in a worker thread";
+        first_i = -1;
+      }
+      return;
+    }
+    for (int i; (i = next.fetch_add(1)) < nm;) {
+      int st[2] = {0, 0};
+      const size_t fo = (size_t)i * n;
+      const int rc = mppd_msolver_adjoint(ms[i], n, n ? eps_h + fo * 6 : nullptr, p, n ? sig_h + fo * 6 : nullptr,
+                                          ctan_h + (size_t)i * 36, n ? dmat_h + fo * 36 : nullptr, st,
+                                          rel_h ? rel_h + (size_t)i * (6 + n) : nullptr);
+      if (stats) {
+        stats[2 * i] = st[0];
+        stats[2 * i + 1] = st[1];
+      }
+      if (rc) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (first_rc == MPPD_OK || i < first_i) {
+          first_i = i;
+          first_rc = rc;
+          first_msg = mppd_last_error();  // this thread's message
+        }
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int w = 1; w < W; ++w) th.emplace_back(work);
+  work();
+  for (auto &t : th) t.join();
+  if (first_rc) return fail(first_rc, "msolver_adjoint_many: solver " + std::to_string(first_i) + ": " + first_msg);
   return MPPD_OK;
 }
 

@@ -114,6 +114,31 @@ def test_adjoint_autograd_gradcheck(mp):
     assert torch.autograd.gradcheck(AdjointSigma.apply, (eps, theta), eps=1e-6, atol=1e-4, rtol=1e-5)
 
 
+def test_adjoint_many_equals_one_by_one(mp):
+    """mppd_msolver_adjoint_many (the batched adjoint: solvers of 6 RVEs of three sizes, on 3 host threads,
+    each solver on its own stream) gives each RVE's ctan, sigma, d sigma / d (E, nu) and residual ratios bit for
+    bit as mppd_msolver_adjoint one RVE after another; a solver given twice is refused."""
+    rng = np.random.default_rng(21)
+    sizes = [9, 13, 17, 9, 13, 17]
+    rves = [mp.Rve(n, mp.MIC3D_RANDOM_SPHERES, (float(100 + i), float(1 + i % 5), 0.08, 0.2),
+                   [mp.material(*s) for s in SPEC]) for i, n in enumerate(sizes)]
+    ms = [mp.MSolver(r, 8) for r in rves]
+    strains = rng.uniform(-5e-3, 5e-3, (len(ms), 2, 6))
+    one = [m.adjoint(strains[i]) for i, m in enumerate(ms)]
+    ctan, sig, dmat, stats, relr, ok = mp.adjoint_many(ms, strains, workers=3)
+    assert ok
+    for i in range(len(ms)):
+        assert np.array_equal(ctan[i], one[i][0]) and np.array_equal(sig[i], one[i][1])
+        assert np.array_equal(dmat[i], one[i][2]) and tuple(stats[i]) == one[i][3]
+        assert np.array_equal(relr[i], one[i][4])
+    # ctan alone (n = 0), all solvers on one thread
+    c0 = mp.adjoint_many(ms, None, workers=1)[0]
+    for i in range(len(ms)):
+        assert np.array_equal(c0[i], ms[i].adjoint()[0])
+    with pytest.raises(mp.MppdError):
+        mp.adjoint_many([ms[0], ms[0]], strains[:2])
+
+
 def test_adjoint_argument_checks(mp):
     r = mp.Rve(5, mp.MIC_SPHERE, (0.2, 0, 0, 0), [mp.material(*s) for s in SPEC])
     with pytest.raises(mp.MppdError):
