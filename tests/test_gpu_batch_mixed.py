@@ -116,12 +116,14 @@ def test_micropp_mixed_instance_runs_as_mixed_batch(mp, monkeypatch, slots, lin_
 
 def test_mixed_batch_coarse_level_against_jacobi(mp, monkeypatch):
     """The two-level preconditioner of the batch's inner FP32 CG (a coarse grid per RVE from its initial
-    tangent; 4 elements per coarse element at 20^3, 2 at 12^3) against Jacobi alone (MPPD_COARSE=0 at
+    tangent; 4 elements per coarse element at 20^3, 2 at 12^3, 4 / 2 / 4 at 20 x 12 x 16) against Jacobi alone (MPPD_COARSE=0 at
     mppd_batch_create), both against the FP64 batch: J2 and damage RVEs of two sizes over two load steps.  The
     coarse level cuts the inner iterations, and sigma / ctan keep the mixed-precision bars."""
     descs = [dict(size=21, micro_type=mp.MIC_CILI_FIB_Z, geo=(0.25, 0, 0, 0), kind="j2", coupling=mp.FE_FULL),
              dict(size=13, micro_type=mp.MIC_LAYER_Y, geo=(0.3, 0, 0, 0), kind="damage", coupling=mp.FE_ONE_WAY),
-             dict(size=21, micro_type=mp.MIC_SPHERE, geo=(0.3, 0, 0, 0), kind="plastic", coupling=mp.FE_FULL)]
+             dict(size=21, micro_type=mp.MIC_SPHERE, geo=(0.3, 0, 0, 0), kind="plastic", coupling=mp.FE_FULL),
+             dict(size=(21, 13, 17), micro_type=mp.MIC_SPHERE, geo=(0.25, 0, 0, 0), kind="plastic",
+                  coupling=mp.FE_ONE_WAY)]
     kw = dict(KW, cg_rel_tol=1e-8)
     steps = 2
     r64, _ = run_batch(mp, descs, steps, **kw)
