@@ -97,6 +97,11 @@ struct micropp<tdim>::impl {
   // with K fields, for the constructor's ctan and the FE GPs of homogenize
   mppd_msolver *ms = nullptr;
   int ms_K = 0;
+  // with ms: the constructor's six ctan solves are deferred to the first pass
+  // of homogenize, which solves them beside its GPs (one pass of up to K
+  // fields instead of two); anything that reads a ctan first (get_ctan,
+  // get_ctan_lin, print_info, a linear GP) solves them on their own
+  bool ctan_pending = false;
 
   ~impl() {
     mppd_msolver_destroy(ms);
@@ -135,6 +140,28 @@ struct micropp<tdim>::impl {
 
   void copy(worker &w, double *dst, const double *src) {
     check(mppd_solver_copy(w.sol, dst, src, sizeof(double) * nndim), "device copy");
+  }
+
+  // the linear ctan into every GP but the mixing-rule ones (micropp
+  // constructor, src/micropp.cpp:113-121)
+  void fill_gp_ctan() {
+    for (auto &q : gp)
+      if (q.coupling != MIX_RULE_CHAMIS) memcpy(q.ctan, ctan_lin_fe, sizeof(q.ctan));
+  }
+
+  // the deferred ctan solves, on their own; a device failure is reported on
+  // stderr (the accessors that call this cross the C interface) and the
+  // solves are tried again at the next read
+  void ensure_ctan() {
+    if (!ctan_pending) return;
+    try {
+      ctan_lin_fe_models();
+    } catch (const std::exception &ex) {
+      cerr << ex.what() << endl;
+      return;
+    }
+    ctan_pending = false;
+    fill_gp_ctan();
   }
 
   // calc_ctan_lin_fe_models (src/micropp.cpp:255-284)
@@ -229,28 +256,45 @@ struct micropp<tdim>::impl {
   // allocates history, so calc_vars_new and the FE_FULL perturbations have
   // nothing to do.  A GP whose Newton does not converge, with subiterations
   // on, is solved again on the one-GP path, which subiterates.
+  // With the constructor's ctan still pending, the first pass carries its
+  // six unit-strain solves (fields 0-5, from u = 0, as
+  // calc_ctan_lin_fe_models) ahead of its GPs.  Each field's solve does not
+  // depend on the others in the pass.
   void fe_multi() {
     const int nfe = (int)fe_ids.size(), K = ms_K;
     std::vector<double> eps(6 * (size_t)K), sig(6 * (size_t)K);
     std::vector<const double *> uin(K);
     std::vector<double *> uout(K);
     std::vector<int> out(3 * (size_t)K);
-    for (int c0 = 0; c0 < nfe; c0 += K) {
-      const int m = std::min(K, nfe - c0);
+    for (int c0 = 0; c0 < nfe;) {
+      const int lead = ctan_pending ? 6 : 0;
+      const int m = std::min(K - lead, nfe - c0);
       try {
+        for (int i = 0; i < lead; ++i) {
+          for (int v = 0; v < 6; ++v) eps[6 * i + v] = v == i ? D_EPS_CTAN_AVE : 0.0;
+          uin[i] = nullptr;
+          uout[i] = nullptr;
+        }
         for (int r = 0; r < m; ++r) {
           const gp_state &q = gp[fe_ids[c0 + r]];
-          memcpy(&eps[6 * r], q.strain, sizeof(q.strain));
-          uin[r] = q.u_n;
-          uout[r] = q.u_k;
+          memcpy(&eps[6 * (lead + r)], q.strain, sizeof(q.strain));
+          uin[lead + r] = q.u_n;
+          uout[lead + r] = q.u_k;
         }
-        check(mppd_msolver_newton(ms, m, eps.data(), uin.data(), uout.data(), &np, nullptr, sig.data(), out.data(),
-                                  nullptr),
-              "newton_raphson (multi-RHS)");
+        check(mppd_msolver_newton(ms, lead + m, eps.data(), uin.data(), uout.data(), &np, nullptr, sig.data(),
+                                  out.data(), nullptr),
+              lead ? "calc_ctan_lin_fe_models and newton_raphson (multi-RHS)" : "newton_raphson (multi-RHS)");
+        if (lead) {
+          for (int i = 0; i < 6; ++i)
+            for (int v = 0; v < 6; ++v) ctan_lin_fe[v * 6 + i] = sig[i * 6 + v] / D_EPS_CTAN_AVE;
+          ctan_pending = false;
+          fill_gp_ctan();
+        }
         for (int r = 0; r < m; ++r) {
           gp_state &q = gp[fe_ids[c0 + r]];
-          q.cost = out[3 * r + 1];
-          q.converged = out[3 * r + 2] != 0;
+          const int f = lead + r;
+          q.cost = out[3 * f + 1];
+          q.converged = out[3 * f + 2] != 0;
           q.subiterated = false;
           if (!q.converged && subiterations) {
             fe(wk[0], q, q.coupling == FE_FULL);
@@ -259,12 +303,13 @@ struct micropp<tdim>::impl {
           if (lin_stress)
             linear(q);
           else
-            memcpy(q.stress, &sig[6 * r], sizeof(q.stress));
+            memcpy(q.stress, &sig[6 * f], sizeof(q.stress));
         }
       } catch (const std::exception &ex) {
         for (int r = 0; r < m; ++r) gp[fe_ids[c0 + r]].converged = false;
         cerr << ex.what() << " (gps " << fe_ids[c0] << " .. " << fe_ids[c0 + m - 1] << ")" << endl;
       }
+      c0 += m;
     }
   }
 
@@ -555,17 +600,21 @@ micropp<tdim>::micropp(const micropp_params_t &params, const micropp_options_t &
     const char *route = getenv("MPPD_MSOLVER_ROUTE");
     if (s.mixed && elastic && (!route || strtol(route, nullptr, 10) != 0) &&
         (n_fe > 0 && (s.calc_ctan_lin_flag || !s.fe_ids.empty()))) {
-      const int want = std::max<int>(s.calc_ctan_lin_flag ? 6 : 1, std::min<int>(16, (int)s.fe_ids.size()));
+      // fields for the six ctan solves beside the FE GPs of the first homogenize pass
+      const int lead = s.calc_ctan_lin_flag ? 6 : 0;
+      const int want = std::max<int>(std::max(lead, 1), std::min<int>(16, (int)s.fe_ids.size() + lead));
       s.ms_K = (want + 3) / 4 * 4;
       check(mppd_msolver_create(&s.ms, s.rve, s.ms_K), "multi-RHS solver set-up");
     }
-    if (s.calc_ctan_lin_flag && n_fe > 0) s.ctan_lin_fe_models();
-    for (auto &q : s.gp) {
-      if (q.coupling == MIX_RULE_CHAMIS)
-        s.chamis(q.ctan);
+    if (s.calc_ctan_lin_flag && n_fe > 0) {
+      if (s.ms)
+        s.ctan_pending = true;  // solved with the first homogenize pass, or when first read
       else
-        memcpy(q.ctan, s.ctan_lin_fe, sizeof(q.ctan));
+        s.ctan_lin_fe_models();
     }
+    for (auto &q : s.gp)
+      if (q.coupling == MIX_RULE_CHAMIS) s.chamis(q.ctan);
+    if (!s.ctan_pending) s.fill_gp_ctan();
     if (s.write_log_flag) {
       std::stringstream fn;
       fn << "micropp-profiling-" << s.mpi_rank << ".log";
@@ -600,6 +649,7 @@ void micropp<tdim>::get_stress(const int gp_id, double *stress) const {
 template <int tdim>
 void micropp<tdim>::get_ctan(const int gp_id, double *ctan) const {
   assert(gp_id >= 0 && gp_id < p_->ngp);
+  p_->ensure_ctan();
   memcpy(ctan, p_->gp[gp_id].ctan, 36 * sizeof(double));
 }
 
@@ -631,6 +681,7 @@ void micropp<tdim>::homogenize() {
     }
     if (parallel) s.fe_parallel();
   }
+  s.ensure_ctan();  // no FE pass carried the pending ctan solves (linear GPs read it below)
   for (int g = 0; g < s.ngp; ++g) {
     gp_state &q = s.gp[g];
     try {
@@ -658,6 +709,7 @@ void micropp<tdim>::homogenize() {
 
 template <int tdim>
 void micropp<tdim>::homogenize_linear() {
+  p_->ensure_ctan();
   for (auto &q : p_->gp) impl::linear(q);
 }
 
@@ -864,6 +916,7 @@ void micropp<tdim>::output2(const int gp_id, const int elem_global, const int ti
 // print_info (src/micropp.cpp:546-597)
 template <int tdim>
 void micropp<tdim>::print_info() const {
+  p_->ensure_ctan();
   const impl &s = *p_;
   int cnt[4] = {0, 0, 0, 0};
   for (const auto &q : s.gp)
@@ -906,6 +959,7 @@ bool micropp<tdim>::get_state(int gp_id, int which, double *out) const {
 
 template <int tdim>
 void micropp<tdim>::get_ctan_lin(double *ctan) const {
+  p_->ensure_ctan();
   memcpy(ctan, p_->ctan_lin_fe, sizeof(p_->ctan_lin_fe));
 }
 
