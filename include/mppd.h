@@ -220,6 +220,16 @@ int mppd_msolver_adjoint_many(int nm, mppd_msolver *const *ms, int n, const doub
 /* timing aid: average ms of the FP32 multi-RHS SpMV kernel and of a whole
  * inner iteration, over iters launches, events on the solver's stream */
 int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *iter_ms);
+/* The coarse level of the solver's two-level inner preconditioner (no
+ * reference counterpart: the reference's CG is Jacobi-preconditioned, src/
+ * ell.cpp:80-117).  dims = {nc, hx, hy, hz, mx, my, mz}: coarse DOFs, fine
+ * elements per coarse element along x, y, z, interior coarse nodes along x,
+ * y, z (all 0 without a coarse level).  ainv_h (may be NULL): nc x nc floats,
+ * the inverse of the Galerkin matrix P^T A P, P the trilinear hats of the
+ * interior coarse nodes; coarse DOF q * 3 + d of node q = ((cz - 1) my +
+ * (cy - 1)) mx + (cx - 1), hat of (cx, cy, cz) at fine node (i, j, k) = the
+ * product over the sides of max(0, 1 - |i - cx hx| / hx). */
+int mppd_msolver_coarse(const mppd_msolver *m, int dims[7], float *ainv_h);
 /* newton_raphson (src/solve.cpp:28-82) with mixed-precision linear solves,
  * for an RVE of any material law (a plastic or damaged RVE's Newton tangent
  * included).  The Newton control flow, the residual (assembly_rhs with the

@@ -148,6 +148,7 @@ SIGNATURES = {
     "mppd_msolver_adjoint": (_I, [_VP, _I, _PD, C.c_void_p, _PD, _PD, _PD, _PI, _PD]),
     "mppd_msolver_adjoint_many": (_I, [_I, C.c_void_p, _I, _PD, C.c_void_p, _I, _PD, _PD, _PD, _PI, _PD]),
     "mppd_msolver_profile": (_I, [_VP, _I, _PD, _PD]),
+    "mppd_msolver_coarse": (_I, [_VP, _PI, C.POINTER(C.c_float)]),
     "mppd_msolver_newton": (_I, [_VP, _I, _PD, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _PD, _PI, _PI]),
     "mppd_newton_mixed": (_I, [_VP, _VP, _PD, _VP, _VP, C.POINTER(NewtonParams), _VP, _PI, _PI]),
     "mppd_batch_params_default": (None, [_VP]),
@@ -625,6 +626,19 @@ class MSolver:
         a, t = C.c_double(), C.c_double()
         _chk(load().mppd_msolver_profile(self.h, iters, C.byref(a), C.byref(t)), "mppd_msolver_profile")
         return a.value, t.value
+
+    def coarse(self, inverse=True):
+        """The coarse level of the two-level inner preconditioner: (dims {nc, hx, hy, hz, mx, my, mz}, the FP32
+        inverse of the Galerkin coarse matrix, nc x nc, or None).  nc = 0 without a coarse level."""
+        dims = np.zeros(7, dtype=np.int32)
+        _chk(load().mppd_msolver_coarse(self.h, _ip(dims), None), "mppd_msolver_coarse")
+        nc = int(dims[0])
+        if not (inverse and nc):
+            return dims, None
+        ainv = np.zeros((nc, nc), dtype=np.float32)
+        _chk(load().mppd_msolver_coarse(self.h, _ip(dims), ainv.ctypes.data_as(C.POINTER(C.c_float))),
+             "mppd_msolver_coarse")
+        return dims, ainv
 
 
 def adjoint_many(msolvers, strains=None, workers=0, rel_tol=1e-8, abs_tol=1e-50, inner_tol=1e-5, max_inner_its=2000,

@@ -79,7 +79,7 @@ def build(verbose: bool = False, sanitize: str | None = None) -> str:
     out = lib_path(sanitize)
     tmp = out + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp] + objs + [
-        "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"]
+        "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -9,24 +9,19 @@
 // coarse element's 8 corners, so its contribution is C = P_e^T Ke P_e with P_e
 // the 24 x 24 map of that position, the same for every element of one phase
 // and position (k_cs_table); the coarse element matrices sum those
-// (k_cs_elem) and the dense A_c gathers them (k_cs_dense).  A_c is factored
-// and inverted on the device (rocSOLVER potrf, then two rocBLAS triangular
-// solves with the identity) and kept as a dense
-// FP32 inverse, so one application of the coarse solve is one matrix-vector
-// product per field.
-#include <rocsolver/rocsolver.h>
-
+// (k_cs_elem) and the band tiles of A_c gather them (k_cs_band).  A_c is
+// factored and inverted on the device by a banded tile Cholesky that forms
+// W = L^-1 in the same launches (k_cs_chol), then A_c^-1 = W^T W (k_cs_wtw),
+// kept as a dense FP32 inverse, so one application of the coarse solve is one
+// matrix-vector product per field.
 #include <algorithm>
 #include <cstdlib>
-#include <map>
-#include <mutex>
 
 #include "internal.hpp"
 
 namespace mpp {
 
 namespace {
-
 
 // coarse elements along a side of ne fine elements: the largest divisor
 // d <= dmax of ne with ne / d >= 2 (0: none with d >= 2)
@@ -86,15 +81,59 @@ __global__ void __launch_bounds__(576) k_cs_elem(Grid g, CGrid cg, int dx, int d
   KE[(size_t)E * 576 + t] = s;
 }
 
-// The dense A_c (column-major, nc x nc) over the interior coarse nodes: entry
-// (q_i 3 + d, q_j 3 + e) sums the coarse elements holding both nodes, in
-// element order (zero when the nodes share none)
-__global__ void __launch_bounds__(256) k_cs_dense(CGrid cg, int dx, int dy, const double *KE, double *Ac) {
+// ---------------------------------------------------------------------------
+// Factorisation and inverse.  A_c is banded: a coarse node couples only with
+// its 26 neighbours, so with the node order above entry (r, c) is zero for
+// |r - c| > bw = 3 (mx my + mx + 1) + 2, and so is the Cholesky factor L's.
+// Everything works on T x T tiles (kChT = 32, row-major inside a tile, the
+// order padded to nt T with the identity): tile (i, j) of A or L is zero for
+// i - j > bt = floor((bw + T - 1) / T).  Band tiles are stored by block
+// column: slot j (bt + 1) + (i - j).  W = L^-1 is lower triangular and dense:
+// tile (i, j), j <= i, in slot i (i + 1) / 2 + j.  Then A_c^-1 = W^T W.
+//
+// One launch per tile step k (k_cs_chol), k = -1 .. nt - 1:
+//  - k = -1: factor tile (0, 0);
+//  - the right-looking step k, one block per tile (i, j), k < j <= i <= k +
+//    bt: L_ik = A_ik L_kk^-T and L_jk likewise (triangular solves, a lane per
+//    row), A_ij -= L_ik L_jk^T; block (i, k + 1) stores L_ik; block (k + 1,
+//    k + 1) then factors its updated tile: L_{k+1,k+1};
+//  - row k of W, one block per tile (k, j), j <= k: W_kj = L_kk^-1 (delta_kj
+//    I - sum_{m = max(j, k - bt)}^{k - 1} L_km W_mj) (a lane per column).
+// Every tile a launch reads was written by an earlier launch; L_ik goes to its
+// own array because the step's other blocks still read A_ik.  Then one block
+// per lower tile of A_c^-1 (k_cs_wtw) sums W_mi^T W_mj over m >= i and
+// writes the FP32 inverse, both triangles.  No pivoting: A_c is symmetric
+// positive definite (a non-positive pivot sets *info and the coarse level is
+// dropped).  nt + 1 launches of at most bt (bt + 1) / 2 + nt blocks.
+// ---------------------------------------------------------------------------
+constexpr int kChT = 32;
+constexpr int kChTT = kChT * kChT;
+constexpr int kChPad = kChT + 1;  // LDS row stride (doubles): rows of a tile fall in distinct banks
+
+struct Band {
+  int n;   // order
+  int nt;  // tiles per side: ceil(n / T)
+  int bt;  // band in tiles
+};
+
+__host__ __device__ inline size_t band_slot(const Band &b, int i, int j) { return (size_t)j * (b.bt + 1) + (i - j); }
+__host__ __device__ inline size_t tri_slot(int i, int j) { return (size_t)i * (i + 1) / 2 + j; }
+
+// The band tiles of A_c: thread per entry of tiles (i, j), 0 <= i - j <= bt;
+// entry (r, c) of A_c sums the coarse elements holding both nodes, in element
+// order (zero when they share none); rows or columns past n hold the identity
+__global__ void __launch_bounds__(256) k_cs_band(CGrid cg, int dx, int dy, Band b, const double *KE, double *Ab) {
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-  const int nc = cg.nc;
-  if (t >= (size_t)nc * nc) return;
-  const int row = (int)(t % nc), col = (int)(t / nc);
-  const int qi = row / 3, d = row % 3, qj = col / 3, e = col % 3;
+  if (t >= (size_t)b.nt * (b.bt + 1) * kChTT) return;
+  const int slot = (int)(t / kChTT), e = (int)(t % kChTT);
+  const int j = slot / (b.bt + 1), i = j + slot % (b.bt + 1);
+  if (i >= b.nt) return;  // past the last tile row
+  const int row = i * kChT + e / kChT, col = j * kChT + e % kChT;
+  if (row >= b.n || col >= b.n) {
+    Ab[t] = row == col ? 1.0 : 0.0;
+    return;
+  }
+  const int qi = row / 3, d = row % 3, qj = col / 3, f = col % 3;
   const int xi = qi % cg.mx + 1, yi = (qi / cg.mx) % cg.my + 1, zi = qi / (cg.mx * cg.my) + 1;
   const int xj = qj % cg.mx + 1, yj = (qj / cg.mx) % cg.my + 1, zj = qj / (cg.mx * cg.my) + 1;
   double s = 0.0;
@@ -103,22 +142,196 @@ __global__ void __launch_bounds__(256) k_cs_dense(CGrid cg, int dx, int dy, cons
       for (int ey = max(yi, yj) - 1; ey <= min(yi, yj); ++ey)
         for (int ex = max(xi, xj) - 1; ex <= min(xi, xj); ++ex) {
           const int A = hex_corner(xi - ex, yi - ey, zi - ez), B = hex_corner(xj - ex, yj - ey, zj - ez);
-          s += KE[((size_t)(ez * dy + ey) * dx + ex) * 576 + (A * 3 + d) * 24 + B * 3 + e];
+          s += KE[((size_t)(ez * dy + ey) * dx + ex) * 576 + (A * 3 + d) * 24 + B * 3 + f];
         }
-  Ac[t] = s;
+  Ab[t] = s;
 }
 
-// X = I (n x n)
-__global__ void __launch_bounds__(256) k_cs_eye(int n, double *X) {
-  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (size_t)n * n) return;
-  X[t] = (t % n == t / n) ? 1.0 : 0.0;
+// a tile (row-major, kChTT doubles) into LDS rows of stride kChPad, by the
+// whole block
+__device__ __forceinline__ void tile_in(const double *src, double (*dst)[kChPad]) {
+  for (int e = threadIdx.x; e < kChTT; e += blockDim.x) dst[e / kChT][e % kChT] = src[e];
+}
+__device__ __forceinline__ void tile_out(const double (*src)[kChPad], double *dst) {
+  for (int e = threadIdx.x; e < kChTT; e += blockDim.x) dst[e] = src[e / kChT][e % kChT];
 }
 
-// one rocBLAS handle per device for the process's lifetime (the factorisation
-// runs under the mutex, with the handle set to the caller's stream)
-std::mutex g_solver_mu;
-std::map<int, rocblas_handle> g_handles;
+__device__ __forceinline__ double lane_bcast(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Cholesky of the tile in t (lower triangle read), by wave 0: lane r holds
+// row r (lanes 32..63 a copy of row r - 32, not stored); the factor goes to
+// dst (row-major, zero above the diagonal).  A pivot that is not positive
+// sets *info.
+__device__ void tile_potrf(const double (*t)[kChPad], double *dst, int *info) {
+  const int lane = threadIdx.x, r = lane & (kChT - 1);
+  double a[kChT];
+#pragma unroll
+  for (int c = 0; c < kChT; ++c) a[c] = t[r][c];
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < kChT; ++j) {
+    const double piv = lane_bcast(a[j], j);
+    bad = bad || !(piv > 0.0);
+    const double dg = sqrt(piv), rd = 1.0 / dg;
+    const double l = r > j ? a[j] * rd : (r == j ? dg : 0.0);
+    a[j] = l;
+#pragma unroll
+    for (int m = j + 1; m < kChT; ++m) a[m] -= l * lane_bcast(l, m);
+  }
+  if (lane < kChT)
+#pragma unroll
+    for (int c = 0; c < kChT; ++c) dst[r * kChT + c] = c <= r ? a[c] : 0.0;
+  if (bad && lane == 0) *info = 1;
+}
+
+__global__ void __launch_bounds__(256) k_cs_chol(Band b, int k, double *Ab, double *Lb, double *Wp, int *info) {
+  __shared__ double sL[kChT][kChPad];     // L_kk
+  __shared__ double sX[2][kChT][kChPad];  // A_ik, A_jk -> L_ik, L_jk (or L_km, W_mj)
+  __shared__ double sF[kChT][kChPad];     // the updated tile (i, j) / the right-hand side of W_kj
+  __shared__ double sD[kChT];             // 1 / diag(L_kk)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (k < 0) {  // factor tile (0, 0)
+    tile_in(Ab, sF);
+    __syncthreads();
+    if (wave == 0) tile_potrf(sF, Lb, info);
+    return;
+  }
+  const int m = min(b.bt, b.nt - 1 - k), nch = m * (m + 1) / 2;
+  tile_in(Lb + band_slot(b, k, k) * kChTT, sL);
+  const int r0 = tid >> 5, cc = tid & (kChT - 1);  // this thread's outputs: (r0 + 8 q, cc), q = 0..3
+  if ((int)blockIdx.x < nch) {
+    // ---- the right-looking step: tile (i, j) ----
+    int di = 0, rest = blockIdx.x;
+    while (rest > di) {
+      rest -= di + 1;
+      ++di;
+    }
+    const int i = k + 1 + di, j = k + 1 + rest;
+    tile_in(Ab + band_slot(b, i, k) * kChTT, sX[0]);
+    if (i != j) tile_in(Ab + band_slot(b, j, k) * kChTT, sX[1]);
+    if (tid < kChT) sD[tid] = 1.0 / Lb[band_slot(b, k, k) * kChTT + tid * (kChT + 1)];
+    __syncthreads();
+    // L_ik = A_ik L_kk^-T (lanes 0..31) and L_jk (lanes 32..63): row r solves
+    // L_kk x = a_r by forward substitution
+    if (wave == 0 && (lane < kChT || i != j)) {
+      const int tt = lane >> 5, r = lane & (kChT - 1);
+      double x[kChT];
+#pragma unroll
+      for (int c = 0; c < kChT; ++c) x[c] = sX[tt][r][c];
+#pragma unroll
+      for (int c = 0; c < kChT; ++c) {
+        double s = x[c];
+#pragma unroll
+        for (int q = 0; q < c; ++q) s -= sL[c][q] * x[q];
+        x[c] = s * sD[c];
+      }
+#pragma unroll
+      for (int c = 0; c < kChT; ++c) sX[tt][r][c] = x[c];
+    }
+    __syncthreads();
+    // A_ij -= L_ik L_jk^T
+    const int tj = i != j ? 1 : 0;
+    double *aij = Ab + band_slot(b, i, j) * kChTT;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + 8 * q;
+      double s = aij[r * kChT + cc];
+#pragma unroll 8
+      for (int p = 0; p < kChT; ++p) s -= sX[0][r][p] * sX[tj][cc][p];
+      aij[r * kChT + cc] = s;
+      sF[r][cc] = s;
+    }
+    if (j == k + 1) tile_out(sX[0], Lb + band_slot(b, i, k) * kChTT);
+    if (i == k + 1 && j == k + 1) {
+      __syncthreads();
+      if (wave == 0) tile_potrf(sF, Lb + band_slot(b, k + 1, k + 1) * kChTT, info);
+    }
+    return;
+  }
+  // ---- row k of W: tile (k, j) ----
+  const int j = blockIdx.x - nch;
+  if (tid < kChT) sD[tid] = 1.0 / Lb[band_slot(b, k, k) * kChTT + tid * (kChT + 1)];
+  double acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = (j == k && r0 + 8 * q == cc) ? 1.0 : 0.0;
+  for (int mm = max(j, k - b.bt); mm < k; ++mm) {
+    __syncthreads();  // the previous tiles are read
+    tile_in(Lb + band_slot(b, k, mm) * kChTT, sX[0]);
+    tile_in(Wp + tri_slot(mm, j) * kChTT, sX[1]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double s = acc[q];
+#pragma unroll 8
+      for (int p = 0; p < kChT; ++p) s -= sX[0][r0 + 8 * q][p] * sX[1][p][cc];
+      acc[q] = s;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) sF[r0 + 8 * q][cc] = acc[q];
+  __syncthreads();
+  // W_kj = L_kk^-1 R: lane c solves column c by forward substitution
+  if (wave == 0 && lane < kChT) {
+    const int c = lane;
+    double w[kChT];
+#pragma unroll
+    for (int r = 0; r < kChT; ++r) {
+      double s = sF[r][c];
+#pragma unroll
+      for (int q = 0; q < r; ++q) s -= sL[r][q] * w[q];
+      w[r] = s * sD[r];
+    }
+    double *dst = Wp + tri_slot(k, j) * kChTT;
+#pragma unroll
+    for (int r = 0; r < kChT; ++r) dst[r * kChT + c] = w[r];
+  }
+}
+
+// A_c^-1 = W^T W: block per lower tile (i, j) (tri_slot order), tile (i, j)
+// = sum over m = i .. nt - 1 of W_mi^T W_mj; the FP32 result goes to ainv
+// (n x n, symmetric) at (i, j) and, off the diagonal, transposed through LDS
+// at (j, i)
+__global__ void __launch_bounds__(256) k_cs_wtw(Band b, const double *Wp, float *ainv) {
+  __shared__ double sA[kChT][kChPad], sB[kChT][kChPad];
+  int i = 0, rest = blockIdx.x;
+  while (rest > i) {
+    rest -= i + 1;
+    ++i;
+  }
+  const int j = rest, tid = threadIdx.x, r0 = tid >> 5, cc = tid & (kChT - 1);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int m = i; m < b.nt; ++m) {
+    __syncthreads();
+    tile_in(Wp + tri_slot(m, i) * kChTT, sA);
+    tile_in(Wp + tri_slot(m, j) * kChTT, sB);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double s = acc[q];
+#pragma unroll 8
+      for (int p = 0; p < kChT; ++p) s += sA[p][r0 + 8 * q] * sB[p][cc];
+      acc[q] = s;
+    }
+  }
+  const int n = b.n;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int row = i * kChT + r0 + 8 * q, col = j * kChT + cc;
+    if (row < n && col < n) ainv[(size_t)row * n + col] = (float)acc[q];
+    sA[r0 + 8 * q][cc] = acc[q];
+  }
+  if (i == j) return;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int row = j * kChT + r0 + 8 * q, col = i * kChT + cc;  // entry (cc, r0 + 8 q) of tile (i, j)
+    if (row < n && col < n) ainv[(size_t)row * n + col] = (float)sA[cc][r0 + 8 * q];
+  }
+}
 
 }  // namespace
 
@@ -168,75 +381,46 @@ int coarse_inverse(const Grid &g, const CGrid &cg, const int *et_d, const double
   *ok = false;
   const int nc = cg.nc, npos = cg.hx * cg.hy * cg.hz;
   const int dx = cg.mx + 1, dy = cg.my + 1, ncel = dx * dy * (cg.mz + 1);
-  double *a64 = nullptr, *C = nullptr, *KE = nullptr, *X = nullptr;
+  Band b;
+  b.n = nc;
+  b.nt = (nc + kChT - 1) / kChT;
+  b.bt = (3 * (cg.mx * cg.my + cg.mx + 1) + 2 + kChT - 1) / kChT;
+  const size_t nband = (size_t)b.nt * (b.bt + 1) * kChTT, nw = tri_slot(b.nt, 0) * kChTT;
+  const size_t nC = (size_t)npos * 3 * 576, nKE = (size_t)ncel * 576;
+  double *buf = nullptr;
   int *info_d = nullptr;
   int rc = MPPD_OK;
   do {
-    if ((rc = dalloc(&a64, (size_t)nc * nc, false))) break;
-    if ((rc = dalloc(&X, (size_t)nc * nc, false))) break;
-    if ((rc = dalloc(&C, (size_t)npos * 3 * 576, false))) break;
-    if ((rc = dalloc(&KE, (size_t)ncel * 576, false))) break;
-    if ((rc = dalloc(&info_d, 1))) break;
-    hipLaunchKernelGGL(k_cs_table, dim3(npos, 3), dim3(576), 0, st, cg, ke_d, C);
-    hipLaunchKernelGGL(k_cs_elem, dim3(ncel), dim3(576), 0, st, g, cg, dx, dy, et_d, C, KE);
-    hipLaunchKernelGGL(k_cs_dense, dim3((unsigned)(((size_t)nc * nc + 255) / 256)), dim3(256), 0, st, cg, dx, dy, KE,
-                       a64);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-      rc = hip_fail(e, "coarse matrix assembly");
-      break;
-    }
-    int dev = 0, info = 0;
-    if ((e = hipGetDevice(&dev)) != hipSuccess) {
-      rc = hip_fail(e, "hipGetDevice");
-      break;
-    }
-    std::lock_guard<std::mutex> lock(g_solver_mu);
-    rocblas_handle &h = g_handles[dev];
-    if (!h && rocblas_create_handle(&h) != rocblas_status_success) {
-      h = nullptr;
-      rc = fail(MPPD_ERR_HIP, "rocblas handle for the coarse solve");
-      break;
-    }
-    if (rocblas_set_stream(h, st) != rocblas_status_success) {
-      rc = fail(MPPD_ERR_HIP, "rocblas stream for the coarse solve");
-      break;
-    }
-    if (rocsolver_dpotrf(h, rocblas_fill_lower, nc, a64, nc, info_d) != rocblas_status_success) {
-      rc = fail(MPPD_ERR_HIP, "rocsolver_dpotrf (coarse matrix)");
-      break;
-    }
-    e = hipMemcpyAsync(&info, info_d, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    // one allocation for the temporaries: Ab | Lb | Wp | C | KE
+    if ((rc = dalloc(&buf, 2 * nband + nw + nC + nKE, false))) break;
+    if ((rc = dalloc(&info_d, 1, false))) break;
+    double *Ab = buf, *Lb = Ab + nband, *Wp = Lb + nband, *C = Wp + nw, *KE = C + nC;
+    hipError_t e = hipMemsetAsync(info_d, 0, sizeof(int), st);
     if (e != hipSuccess) {
       rc = hip_fail(e, "coarse factor status");
       break;
     }
-    if (info != 0) break;  // not positive definite: no coarse level
-    // A_c^-1 = L^-T L^-1 by two triangular solves with the identity (level-3
-    // BLAS; 0.4 ms less than rocSOLVER's potri at 2 187 coarse DOFs)
-    hipLaunchKernelGGL(k_cs_eye, dim3((unsigned)(((size_t)nc * nc + 255) / 256)), dim3(256), 0, st, nc, X);
-    const double one = 1.0;
-    if (rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
-                      nc, nc, &one, a64, nc, X, nc) != rocblas_status_success ||
-        rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
-                      rocblas_diagonal_non_unit, nc, nc, &one, a64, nc, X, nc) != rocblas_status_success) {
-      rc = fail(MPPD_ERR_HIP, "rocblas_dtrsm (coarse inverse)");
-      break;
+    hipLaunchKernelGGL(k_cs_table, dim3(npos, 3), dim3(576), 0, st, cg, ke_d, C);
+    hipLaunchKernelGGL(k_cs_elem, dim3(ncel), dim3(576), 0, st, g, cg, dx, dy, et_d, C, KE);
+    hipLaunchKernelGGL(k_cs_band, dim3((unsigned)((nband + 255) / 256)), dim3(256), 0, st, cg, dx, dy, b, KE, Ab);
+    for (int k = -1; k < b.nt; ++k) {
+      const int m = k < 0 ? 0 : std::min(b.bt, b.nt - 1 - k);
+      const int nblk = k < 0 ? 1 : m * (m + 1) / 2 + k + 1;
+      hipLaunchKernelGGL(k_cs_chol, dim3(nblk), dim3(256), 0, st, b, k, Ab, Lb, Wp, info_d);
     }
-    e = launch_c_sym32(nc, X, ainv, st);
+    hipLaunchKernelGGL(k_cs_wtw, dim3((unsigned)tri_slot(b.nt, 0)), dim3(256), 0, st, b, Wp, ainv);
+    int info = 0;
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&info, info_d, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
-      rc = hip_fail(e, "coarse inverse copy");
+      rc = hip_fail(e, "coarse matrix factorisation");
       break;
     }
-    *ok = true;
+    *ok = info == 0;  // else not positive definite: no coarse level
   } while (0);
   (void)hipStreamSynchronize(st);  // the temporaries below may still be in use after a failed step
-  dfree(a64);
-  dfree(X);
-  dfree(C);
-  dfree(KE);
+  dfree(buf);
   dfree(info_d);
   return rc;
 }

@@ -235,7 +235,6 @@ hipError_t launch_c_start(const Grid &g, int K, const CoarseDev &c, const float 
                           void *state, hipStream_t st);
 // the full symmetric FP32 copy of the lower triangle (column-major) of the
 // FP64 inverse a64 (n x n): ainv[i * n + j]
-hipError_t launch_c_sym32(int n, const double *a64, float *ainv, hipStream_t st);
 hipError_t launch_m_iter(const Grid &g, int K, const float *A, const float *kinv, float *R, float *X, float *P,
                          float *AP, double *part, void *state, int parity, hipStream_t st,
                          const CoarseDev *coarse = nullptr);

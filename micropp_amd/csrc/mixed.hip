@@ -1060,16 +1060,6 @@ __global__ void __launch_bounds__(kRedThreads) k_c_rzfix(int nbx, const double *
   if (threadIdx.x < K) s0->rz[threadIdx.x] = rz;
 }
 
-// the full symmetric FP32 inverse from the lower triangle (column-major) of
-// the FP64 one: ainv[i * n + j] = a64[max(i, j) + min(i, j) * n]
-__global__ void __launch_bounds__(256) k_c_sym32(int n, const double *a64, float *ainv) {
-  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (size_t)n * n) return;
-  const int i = (int)(t / n), j = (int)(t % n);
-  const int hi = max(i, j), lo = min(i, j);
-  ainv[t] = (float)a64[(size_t)hi + (size_t)lo * n];
-}
-
 // k_m_pupd: X += alpha P; P = kinv R + beta P for the fields still running;
 // the next CG state.  Every block re-sums the (z, z) and (r, z) partials of
 // its four fields in the same fixed order (so all blocks agree bit for bit),
@@ -1705,11 +1695,6 @@ hipError_t launch_c_start(const Grid &g, int K, const CoarseDev &c, const float 
                        reinterpret_cast<float4 *>(P), c.prz, s0);
     hipLaunchKernelGGL(k_c_rzfix<KK>, dim3(1), dim3(kRedThreads), 0, st, nbx, c.prz, s0);
   });
-  return hipGetLastError();
-}
-
-hipError_t launch_c_sym32(int n, const double *a64, float *ainv, hipStream_t st) {
-  hipLaunchKernelGGL(k_c_sym32, dim3(nblk((size_t)n * n, 256)), dim3(256), 0, st, n, a64, ainv);
   return hipGetLastError();
 }
 

@@ -1344,4 +1344,17 @@ int mppd_msolver_profile(mppd_msolver *m, int iters, double *spmv_ms, double *it
   return MPPD_OK;
 }
 
+int mppd_msolver_coarse(const mppd_msolver *m, int dims[7], float *ainv_h) {
+  if (!m || !dims) return fail(MPPD_ERR_ARG, "bad msolver_coarse arguments");
+  const CoarseDev *c = m->coarse;
+  const CGrid cg = c ? c->cg : CGrid{};
+  const int v[7] = {cg.nc, cg.hx, cg.hy, cg.hz, cg.mx, cg.my, cg.mz};
+  for (int i = 0; i < 7; ++i) dims[i] = v[i];
+  if (c && ainv_h) {
+    MPP_HIP(hipMemcpyAsync(ainv_h, c->ainv, (size_t)cg.nc * cg.nc * sizeof(float), hipMemcpyDeviceToHost, m->st));
+    MPP_HIP(hipStreamSynchronize(m->st));
+  }
+  return MPPD_OK;
+}
+
 }  // extern "C"
