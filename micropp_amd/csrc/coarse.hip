@@ -16,6 +16,7 @@
 // matrix-vector product per field.
 #include <algorithm>
 #include <cstdlib>
+#include <utility>
 
 #include "internal.hpp"
 
@@ -147,13 +148,46 @@ __global__ void __launch_bounds__(256) k_cs_band(CGrid cg, int dx, int dy, Band 
   Ab[t] = s;
 }
 
-// a tile (row-major, kChTT doubles) into LDS rows of stride kChPad, by the
-// whole block
-__device__ __forceinline__ void tile_in(const double *src, double (*dst)[kChPad]) {
-  for (int e = threadIdx.x; e < kChTT; e += blockDim.x) dst[e / kChT][e % kChT] = src[e];
+// A tile in LDS: rows of stride kChPad doubles (kChTile per tile).
+constexpr int kChTile = kChT * kChPad;
+// tiles of a W row's products loaded together (into registers, then LDS)
+constexpr int kChChunk = 4;
+
+// a tile (row-major in global memory) into LDS, by the whole block
+__device__ __forceinline__ void tile_in(const double *src, double *dst) {
+  for (int e = threadIdx.x; e < kChTT; e += blockDim.x) dst[(e >> 5) * kChPad + (e & 31)] = src[e];
 }
-__device__ __forceinline__ void tile_out(const double (*src)[kChPad], double *dst) {
-  for (int e = threadIdx.x; e < kChTT; e += blockDim.x) dst[e] = src[e / kChT][e % kChT];
+__device__ __forceinline__ void tile_out(const double *src, double *dst) {
+  for (int e = threadIdx.x; e < kChTT; e += blockDim.x) dst[e] = src[(e >> 5) * kChPad + (e & 31)];
+}
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Quadrant (qi, qj) (16 x 16) of a 32 x 32 product C += A B by one wave, with
+// FP64 MFMA (16 x 16 x 4 per instruction, p in steps of 4).  a(x, p) =
+// A[16 qi + x][p], b(p, x) = B[p][16 qj + x].  The FP64 instruction's
+// layouts: the A operand of lane l is row l % 16 at p0 + l / 16, the B
+// operand column l % 16 at p0 + l / 16, and accumulator entry v (0 .. 3) of
+// lane l is row 4 v + l / 16, column l % 16 of the quadrant (the FP32
+// instruction of this shape differs: row 4 (l / 16) + v).
+template <class FA, class FB>
+__device__ __forceinline__ dbl4 mfma_32(dbl4 c, FA a, FB b) {
+  const int l = threadIdx.x & 63, x = l & 15, kk = l >> 4;
+#pragma unroll
+  for (int p0 = 0; p0 < kChT; p0 += 4) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a(x, p0 + kk), b(p0 + kk, x), c, 0, 0, 0);
+  return c;
+}
+// the tile row / column of accumulator entry v of this lane (wave w: qi = w / 2, qj = w % 2)
+__device__ __forceinline__ int acc_row(int v) { return 16 * ((threadIdx.x >> 6) >> 1) + 4 * v + ((threadIdx.x & 63) >> 4); }
+__device__ __forceinline__ int acc_col() { return 16 * ((threadIdx.x >> 6) & 1) + (threadIdx.x & 15); }
+
+// 1 / sqrt(x) for a positive normal x: the hardware estimate refined by two
+// Newton steps (error within an ulp or so; the factor is a preconditioner's)
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  const double h = 0.5 * x;
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-h * y, y, 1.5);
+  return y * fma(-h * y, y, 1.5);
 }
 
 __device__ __forceinline__ double lane_bcast(double v, int lane) {
@@ -161,47 +195,88 @@ __device__ __forceinline__ double lane_bcast(double v, int lane) {
                           __builtin_amdgcn_readlane(__double2loint(v), lane));
 }
 
-// Cholesky of the tile in t (lower triangle read), by wave 0: lane r holds
-// row r (lanes 32..63 a copy of row r - 32, not stored); the factor goes to
-// dst (row-major, zero above the diagonal).  A pivot that is not positive
-// sets *info.
-__device__ void tile_potrf(const double (*t)[kChPad], double *dst, int *info) {
+// Cholesky of the 32 x 32 tile t (LDS, lower triangle read) by wave 0: lane r
+// holds row r (lanes 32 .. 63 a copy of row r - 32) and keeps its own
+// diagonal entry up to date, so the next pivot is read from its lane
+// (readlane) as soon as the column's multipliers are known; the multipliers
+// go round through LDS (bc, 32 doubles) for the rows' updates, and the next
+// pivot's square root is issued while those reads are in flight.  The factor
+// goes to dst (row-major, zero above the diagonal); a pivot that is not
+// positive sets *info.  The columns are unrolled by templates: as a loop, the
+// compiler leaves it partly rolled and moves the row to scratch memory.
+template <int J>
+__device__ __forceinline__ void potrf_col(double (&a)[kChT], double &dr, double &rs, double &sq, bool &bad, int r,
+                                          double *bc) {
+  const double l = r > J ? a[J] * rs : (r == J ? sq : 0.0);
+  a[J] = l;
+  if constexpr (J + 1 < kChT) {
+    dr = fma(-l, l, dr);  // = the update of a[r] below, the same operands
+    bc[r] = l;    // lanes r and r + 32 store the same value
+    __builtin_amdgcn_wave_barrier();
+    double lm[kChT - 1 - J];
+#pragma unroll
+    for (int m = J + 1; m < kChT; ++m) lm[m - J - 1] = bc[m];
+    const double piv = lane_bcast(dr, J + 1);
+    bad = bad || !(piv > 0.0);
+    rs = rsqrt_nr(piv);
+    sq = piv * rs;
+#pragma unroll
+    for (int m = J + 1; m < kChT; ++m) a[m] = fma(-l, lm[m - J - 1], a[m]);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+template <int... J>
+__device__ __forceinline__ void potrf_cols(double (&a)[kChT], double &dr, double &rs, double &sq, bool &bad, int r,
+                                           double *bc, std::integer_sequence<int, J...>) {
+  (potrf_col<J>(a, dr, rs, sq, bad, r, bc), ...);
+}
+__device__ __forceinline__ void tile_potrf(const double *t, double *dst, int *info, double *bc) {
   const int lane = threadIdx.x, r = lane & (kChT - 1);
   double a[kChT];
 #pragma unroll
-  for (int c = 0; c < kChT; ++c) a[c] = t[r][c];
-  bool bad = false;
+  for (int c = 0; c < kChT; ++c) a[c] = t[r * kChPad + c];
+  double dr = a[0];
 #pragma unroll
-  for (int j = 0; j < kChT; ++j) {
-    const double piv = lane_bcast(a[j], j);
-    bad = bad || !(piv > 0.0);
-    const double dg = sqrt(piv), rd = 1.0 / dg;
-    const double l = r > j ? a[j] * rd : (r == j ? dg : 0.0);
-    a[j] = l;
-#pragma unroll
-    for (int m = j + 1; m < kChT; ++m) a[m] -= l * lane_bcast(l, m);
-  }
+  for (int c = 1; c < kChT; ++c) dr = r == c ? a[c] : dr;
+  const double piv = t[0];
+  bool bad = !(piv > 0.0);
+  double rs = rsqrt_nr(piv), sq = piv * rs;
+  potrf_cols(a, dr, rs, sq, bad, r, bc, std::make_integer_sequence<int, kChT>{});
   if (lane < kChT)
 #pragma unroll
     for (int c = 0; c < kChT; ++c) dst[r * kChT + c] = c <= r ? a[c] : 0.0;
   if (bad && lane == 0) *info = 1;
 }
 
+// x = L^-1 x for the lower triangular L in LDS (rows of stride kChPad) with
+// dinv = 1 / diag(L), column by column (after x[c] is final, the later
+// entries take its term), so one multiply-add per column is on the chain
+__device__ __forceinline__ void trsv_lower(double (&x)[kChT], const double *L, const double *dinv) {
+#pragma unroll
+  for (int c = 0; c < kChT; ++c) {
+    x[c] *= dinv[c];
+#pragma unroll
+    for (int m = c + 1; m < kChT; ++m) x[m] = fma(-L[m * kChPad + c], x[c], x[m]);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_cs_chol(Band b, int k, double *Ab, double *Lb, double *Wp, int *info) {
-  __shared__ double sL[kChT][kChPad];     // L_kk
-  __shared__ double sX[2][kChT][kChPad];  // A_ik, A_jk -> L_ik, L_jk (or L_km, W_mj)
-  __shared__ double sF[kChT][kChPad];     // the updated tile (i, j) / the right-hand side of W_kj
-  __shared__ double sD[kChT];             // 1 / diag(L_kk)
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // LDS: L_kk | the tile for W's solve or the factor | 1 / diag(L_kk) | the
+  // factor's exchange | A_ik, A_jk (step) or the chunk's tiles (W row)
+  __shared__ double smem[(2 + 2 * kChChunk) * kChTile + 2 * kChT];
+  double *sL = smem, *sF = smem + kChTile, *sD = smem + 2 * kChTile, *bc = sD + kChT, *sX = bc + kChT;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int qi = wave >> 1, qj = wave & 1, ac = acc_col();
   if (k < 0) {  // factor tile (0, 0)
     tile_in(Ab, sF);
     __syncthreads();
-    if (wave == 0) tile_potrf(sF, Lb, info);
+    if (wave == 0) tile_potrf(sF, Lb, info, bc);
     return;
   }
   const int m = min(b.bt, b.nt - 1 - k), nch = m * (m + 1) / 2;
-  tile_in(Lb + band_slot(b, k, k) * kChTT, sL);
-  const int r0 = tid >> 5, cc = tid & (kChT - 1);  // this thread's outputs: (r0 + 8 q, cc), q = 0..3
+  const double *lkk = Lb + band_slot(b, k, k) * kChTT;
+  tile_in(lkk, sL);
+  if (tid < kChT) sD[tid] = 1.0 / lkk[tid * (kChT + 1)];
   if ((int)blockIdx.x < nch) {
     // ---- the right-looking step: tile (i, j) ----
     int di = 0, rest = blockIdx.x;
@@ -210,80 +285,93 @@ __global__ void __launch_bounds__(256) k_cs_chol(Band b, int k, double *Ab, doub
       ++di;
     }
     const int i = k + 1 + di, j = k + 1 + rest;
-    tile_in(Ab + band_slot(b, i, k) * kChTT, sX[0]);
-    if (i != j) tile_in(Ab + band_slot(b, j, k) * kChTT, sX[1]);
-    if (tid < kChT) sD[tid] = 1.0 / Lb[band_slot(b, k, k) * kChTT + tid * (kChT + 1)];
+    double *sXi = sX, *sXj = i != j ? sX + kChTile : sX;
+    tile_in(Ab + band_slot(b, i, k) * kChTT, sXi);
+    if (i != j) tile_in(Ab + band_slot(b, j, k) * kChTT, sXj);
     __syncthreads();
     // L_ik = A_ik L_kk^-T (lanes 0..31) and L_jk (lanes 32..63): row r solves
-    // L_kk x = a_r by forward substitution
+    // L_kk x = a_r
     if (wave == 0 && (lane < kChT || i != j)) {
-      const int tt = lane >> 5, r = lane & (kChT - 1);
+      double *t = lane < kChT ? sXi : sXj;
+      const int r = lane & (kChT - 1);
       double x[kChT];
 #pragma unroll
-      for (int c = 0; c < kChT; ++c) x[c] = sX[tt][r][c];
+      for (int c = 0; c < kChT; ++c) x[c] = t[r * kChPad + c];
+      trsv_lower(x, sL, sD);
 #pragma unroll
-      for (int c = 0; c < kChT; ++c) {
-        double s = x[c];
-#pragma unroll
-        for (int q = 0; q < c; ++q) s -= sL[c][q] * x[q];
-        x[c] = s * sD[c];
-      }
-#pragma unroll
-      for (int c = 0; c < kChT; ++c) sX[tt][r][c] = x[c];
+      for (int c = 0; c < kChT; ++c) t[r * kChPad + c] = x[c];
     }
     __syncthreads();
-    // A_ij -= L_ik L_jk^T
-    const int tj = i != j ? 1 : 0;
+    // A_ij -= L_ik L_jk^T, a quadrant per wave
     double *aij = Ab + band_slot(b, i, j) * kChTT;
+    dbl4 acc;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = r0 + 8 * q;
-      double s = aij[r * kChT + cc];
-#pragma unroll 8
-      for (int p = 0; p < kChT; ++p) s -= sX[0][r][p] * sX[tj][cc][p];
-      aij[r * kChT + cc] = s;
-      sF[r][cc] = s;
+    for (int v = 0; v < 4; ++v) acc[v] = aij[acc_row(v) * kChT + ac];
+    acc = mfma_32(
+        acc, [&](int x, int p) { return sXi[(16 * qi + x) * kChPad + p]; },
+        [&](int p, int x) { return -sXj[(16 * qj + x) * kChPad + p]; });
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      aij[acc_row(v) * kChT + ac] = acc[v];
+      sF[acc_row(v) * kChPad + ac] = acc[v];
     }
-    if (j == k + 1) tile_out(sX[0], Lb + band_slot(b, i, k) * kChTT);
+    if (j == k + 1) tile_out(sXi, Lb + band_slot(b, i, k) * kChTT);
     if (i == k + 1 && j == k + 1) {
       __syncthreads();
-      if (wave == 0) tile_potrf(sF, Lb + band_slot(b, k + 1, k + 1) * kChTT, info);
+      if (wave == 0) tile_potrf(sF, Lb + band_slot(b, k + 1, k + 1) * kChTT, info, bc);
     }
     return;
   }
   // ---- row k of W: tile (k, j) ----
   const int j = blockIdx.x - nch;
-  if (tid < kChT) sD[tid] = 1.0 / Lb[band_slot(b, k, k) * kChTT + tid * (kChT + 1)];
-  double acc[4];
+  dbl4 acc;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) acc[q] = (j == k && r0 + 8 * q == cc) ? 1.0 : 0.0;
-  for (int mm = max(j, k - b.bt); mm < k; ++mm) {
-    __syncthreads();  // the previous tiles are read
-    tile_in(Lb + band_slot(b, k, mm) * kChTT, sX[0]);
-    tile_in(Wp + tri_slot(mm, j) * kChTT, sX[1]);
+  for (int v = 0; v < 4; ++v) acc[v] = (j == k && acc_row(v) == ac) ? 1.0 : 0.0;
+  // chunks of kChChunk products: the chunk's tiles L_k,mm and W_mm,j load
+  // into registers (all loads issued before any store) while the previous
+  // chunk multiplies, then go to LDS
+  const int cbeg = max(j, k - b.bt);
+  double rg[2 * kChChunk][4];
+  auto load_chunk = [&](int c0) {
+#pragma unroll
+    for (int t = 0; t < 2 * kChChunk; ++t) {
+      const int mm = c0 + t / 2;
+      const double *src = mm >= k ? Lb : (t & 1) ? Wp + tri_slot(mm, j) * kChTT : Lb + band_slot(b, k, mm) * kChTT;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) rg[t][s] = src[tid + 256 * s];
+    }
+  };
+  if (cbeg < k) load_chunk(cbeg);
+  for (int c0 = cbeg; c0 < k; c0 += kChChunk) {
+    __syncthreads();  // the previous chunk's tiles are read
+#pragma unroll
+    for (int t = 0; t < 2 * kChChunk; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int e = tid + 256 * s;
+        sX[t * kChTile + (e >> 5) * kChPad + (e & 31)] = rg[t][s];
+      }
     __syncthreads();
+    if (c0 + kChChunk < k) load_chunk(c0 + kChChunk);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      double s = acc[q];
-#pragma unroll 8
-      for (int p = 0; p < kChT; ++p) s -= sX[0][r0 + 8 * q][p] * sX[1][p][cc];
-      acc[q] = s;
+    for (int t = 0; t < kChChunk; ++t) {
+      if (c0 + t >= k) break;
+      const double *lt = sX + 2 * t * kChTile, *wt = lt + kChTile;
+      acc = mfma_32(
+          acc, [&](int x, int p) { return lt[(16 * qi + x) * kChPad + p]; },
+          [&](int p, int x) { return -wt[p * kChPad + 16 * qj + x]; });
     }
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) sF[r0 + 8 * q][cc] = acc[q];
+  for (int v = 0; v < 4; ++v) sF[acc_row(v) * kChPad + ac] = acc[v];
   __syncthreads();
-  // W_kj = L_kk^-1 R: lane c solves column c by forward substitution
+  // W_kj = L_kk^-1 R: lane c solves column c
   if (wave == 0 && lane < kChT) {
     const int c = lane;
     double w[kChT];
 #pragma unroll
-    for (int r = 0; r < kChT; ++r) {
-      double s = sF[r][c];
-#pragma unroll
-      for (int q = 0; q < r; ++q) s -= sL[r][q] * w[q];
-      w[r] = s * sD[r];
-    }
+    for (int r = 0; r < kChT; ++r) w[r] = sF[r * kChPad + c];
+    trsv_lower(w, sL, sD);
     double *dst = Wp + tri_slot(k, j) * kChTT;
 #pragma unroll
     for (int r = 0; r < kChT; ++r) dst[r * kChT + c] = w[r];
@@ -291,45 +379,60 @@ __global__ void __launch_bounds__(256) k_cs_chol(Band b, int k, double *Ab, doub
 }
 
 // A_c^-1 = W^T W: block per lower tile (i, j) (tri_slot order), tile (i, j)
-// = sum over m = i .. nt - 1 of W_mi^T W_mj; the FP32 result goes to ainv
-// (n x n, symmetric) at (i, j) and, off the diagonal, transposed through LDS
-// at (j, i)
+// = sum over m = i .. nt - 1 of W_mi^T W_mj (FP64 MFMA, a quadrant per wave;
+// the next m's tiles load into registers while the current ones multiply, two
+// LDS buffers).  The FP32 result goes to ainv (n x n, symmetric) at (i, j)
+// and, through LDS, transposed at (j, i); a diagonal tile is written from its
+// lower triangle, so the inverse is exactly symmetric.
 __global__ void __launch_bounds__(256) k_cs_wtw(Band b, const double *Wp, float *ainv) {
-  __shared__ double sA[kChT][kChPad], sB[kChT][kChPad];
+  __shared__ double smem[4 * kChTile];
   int i = 0, rest = blockIdx.x;
   while (rest > i) {
     rest -= i + 1;
     ++i;
   }
-  const int j = rest, tid = threadIdx.x, r0 = tid >> 5, cc = tid & (kChT - 1);
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int m = i; m < b.nt; ++m) {
-    __syncthreads();
-    tile_in(Wp + tri_slot(m, i) * kChTT, sA);
-    tile_in(Wp + tri_slot(m, j) * kChTT, sB);
-    __syncthreads();
+  const int j = rest, tid = threadIdx.x, wave = tid >> 6, qi = wave >> 1, qj = wave & 1, ac = acc_col();
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  double rg[2][4];
+  auto load = [&](int m) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      double s = acc[q];
-#pragma unroll 8
-      for (int p = 0; p < kChT; ++p) s += sA[p][r0 + 8 * q] * sB[p][cc];
-      acc[q] = s;
+    for (int s = 0; s < 4; ++s) {
+      rg[0][s] = Wp[tri_slot(m, i) * kChTT + tid + 256 * s];
+      rg[1][s] = Wp[tri_slot(m, j) * kChTT + tid + 256 * s];
     }
-  }
-  const int n = b.n;
-  __syncthreads();
+  };
+  load(i);
+  for (int m = i; m < b.nt; ++m) {
+    double *wi = smem + (m & 1) * 2 * kChTile, *wj = wi + kChTile;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int row = i * kChT + r0 + 8 * q, col = j * kChT + cc;
-    if (row < n && col < n) ainv[(size_t)row * n + col] = (float)acc[q];
-    sA[r0 + 8 * q][cc] = acc[q];
+    for (int s = 0; s < 4; ++s) {
+      const int e = tid + 256 * s, o = (e >> 5) * kChPad + (e & 31);
+      wi[o] = rg[0][s];
+      wj[o] = rg[1][s];
+    }
+    __syncthreads();  // (the buffer's previous readers passed the last barrier)
+    if (m + 1 < b.nt) load(m + 1);
+    acc = mfma_32(
+        acc, [&](int x, int p) { return wi[p * kChPad + 16 * qi + x]; },
+        [&](int p, int x) { return wj[p * kChPad + 16 * qj + x]; });
+  }
+  __syncthreads();
+  double *o = smem;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) o[acc_row(v) * kChPad + ac] = acc[v];
+  __syncthreads();
+  const int n = b.n;
+  for (int e = tid; e < kChTT; e += 256) {
+    const int r = e >> 5, c = e & 31;
+    const int row = i * kChT + r, col = j * kChT + c;
+    const double v = i == j ? o[max(r, c) * kChPad + min(r, c)] : o[r * kChPad + c];
+    if (row < n && col < n) ainv[(size_t)row * n + col] = (float)v;
   }
   if (i == j) return;
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int row = j * kChT + r0 + 8 * q, col = i * kChT + cc;  // entry (cc, r0 + 8 q) of tile (i, j)
-    if (row < n && col < n) ainv[(size_t)row * n + col] = (float)sA[cc][r0 + 8 * q];
+  for (int e = tid; e < kChTT; e += 256) {
+    const int r = e >> 5, c = e & 31;  // entry (c, r) of tile (i, j) at (j T + r, i T + c)
+    const int row = j * kChT + r, col = i * kChT + c;
+    if (row < n && col < n) ainv[(size_t)row * n + col] = (float)o[c * kChPad + r];
   }
 }
 
@@ -444,15 +547,17 @@ int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipS
     float *ainv = nullptr;
     if ((rc = dalloc(&ainv, (size_t)nc * nc, false))) break;
     c->ainv = ainv;
-    if ((rc = dalloc(&c->t1, (size_t)3 * K * g.nz * g.ny * cg.mx))) break;
-    if ((rc = dalloc(&c->rc, (size_t)nc * K))) break;
-    if ((rc = dalloc(&c->ecp, (size_t)kCoarseSplit * nc * K))) break;
-    if ((rc = dalloc(&c->ec, (size_t)nc * K))) break;
-    if ((rc = dalloc(&c->z, mixed_vector_len(g, K)))) break;
-    if ((rc = dalloc(&c->prz, (size_t)3 * K * coarse_vec_blocks(g)))) break;
+    // zero fills on st (coarse_inverse synchronizes it)
+    if ((rc = dalloc(&c->t1, (size_t)3 * K * g.nz * g.ny * cg.mx, st))) break;
+    if ((rc = dalloc(&c->rc, (size_t)nc * K, st))) break;
+    if ((rc = dalloc(&c->ecp, (size_t)kCoarseSplit * nc * K, st))) break;
+    if ((rc = dalloc(&c->ec, (size_t)nc * K, st))) break;
+    if ((rc = dalloc(&c->z, mixed_vector_len(g, K), st))) break;
+    if ((rc = dalloc(&c->prz, (size_t)3 * K * coarse_vec_blocks(g), st))) break;
     rc = coarse_inverse(g, cg, et_d, ke_d, ainv, st, &ok);
   } while (0);
   if (rc || !ok) {
+    (void)hipStreamSynchronize(st);  // the zero fills of an early failure may still be queued
     coarse_free(c);
     return rc;
   }

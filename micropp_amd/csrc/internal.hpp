@@ -81,6 +81,19 @@ inline int dalloc(T **p, size_t n, bool zero = true) {
   return MPPD_OK;
 }
 
+// Device allocation of n elements, zero-filled on stream st: the fill is
+// ordered before st's later work, and the caller synchronizes st before the
+// buffer is used from another stream or by a blocking copy.
+template <typename T>
+inline int dalloc(T **p, size_t n, hipStream_t st) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  e = hipMemsetAsync(*p, 0, n * sizeof(T), st);
+  return e == hipSuccess ? MPPD_OK : hip_fail(e, "hipMemsetAsync");
+}
+
 inline void dfree(void *p) {
   if (p) (void)hipFree(p);
 }

@@ -312,18 +312,20 @@ int mppd_solver_create(mppd_solver **out, mppd_rve *r) {
       rc = hip_fail(e, "hipStreamCreate");
       break;
     }
+    // zero fills on s->st (synchronized below)
+    hipStream_t st = s->st;
     if ((rc = dalloc(&s->cols, np * kSlots, false))) break;
-    if ((rc = dalloc(&s->vals, np * kSlots))) break;
+    if ((rc = dalloc(&s->vals, np * kSlots, st))) break;
     if ((rc = dalloc(&s->ke, (s->phase_mode ? 3 : (size_t)g.nelem) * 576, false))) break;
     if ((rc = dalloc(&s->be, (size_t)g.nelem * 24, false))) break;
     double **vecs[] = {&s->x, &s->rr, &s->z, &s->p, &s->Ap, &s->kinv, &s->b, &s->du};
     for (double **v : vecs)
-      if ((rc = dalloc(v, np))) break;
+      if ((rc = dalloc(v, np, st))) break;
     if (rc) break;
     const size_t nb = std::max(round_up(np, 256) / 256, round_up((size_t)g.nelem, 256) / 256) + 1;
-    if ((rc = dalloc(&s->partials, nb * 8))) break;
-    if ((rc = dalloc(&s->cg, 2))) break;  // ping-pong pair
-    if ((rc = dalloc(&s->red, 1))) break;
+    if ((rc = dalloc(&s->partials, nb * 8, st))) break;
+    if ((rc = dalloc(&s->cg, 2, st))) break;  // ping-pong pair
+    if ((rc = dalloc(&s->red, 1, st))) break;
     e = hipHostMalloc((void **)&s->cg_h, sizeof(CgState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&s->red_h, sizeof(RedState), hipHostMallocDefault);
     if (e == hipSuccess) e = launch_ell_cols(g, s->cols, s->st);
@@ -745,20 +747,22 @@ int mppd_msolver_create(mppd_msolver **out, mppd_rve *r, int nrhs) {
       break;
     }
     const size_t nv = mixed_vector_len(g, nrhs);
-    if ((rc = dalloc(&m->ke, 3 * 576))) break;
-    if ((rc = dalloc(&m->mats_d, 3))) break;
+    // zero fills on m->st (synchronized below); mats_d is written by a blocking copy
+    hipStream_t st = m->st;
+    if ((rc = dalloc(&m->ke, 3 * 576, st))) break;
+    if ((rc = dalloc(&m->mats_d, 3, false))) break;
     if ((rc = dalloc(&m->A, mixed_matrix_floats(g), false))) break;
-    if ((rc = dalloc(&m->kinv, nv / nrhs))) break;
-    if ((rc = dalloc(&m->X, nv))) break;
-    if ((rc = dalloc(&m->R, nv))) break;
-    if ((rc = dalloc(&m->P, nv))) break;
-    if ((rc = dalloc(&m->AP, nv))) break;
-    if ((rc = dalloc(&m->U, nv))) break;
-    if ((rc = dalloc(&m->eps_d, 16 * 6))) break;
-    if ((rc = dalloc(&m->sig_d, 16 * 6))) break;
-    if ((rc = dalloc(&m->part, mixed_partials(g, nrhs)))) break;
-    if ((rc = dalloc((char **)&m->state, mixed_state_bytes()))) break;
-    if ((rc = dalloc(&m->act_d, kMaxRhsFields))) break;
+    if ((rc = dalloc(&m->kinv, nv / nrhs, st))) break;
+    if ((rc = dalloc(&m->X, nv, st))) break;
+    if ((rc = dalloc(&m->R, nv, st))) break;
+    if ((rc = dalloc(&m->P, nv, st))) break;
+    if ((rc = dalloc(&m->AP, nv, st))) break;
+    if ((rc = dalloc(&m->U, nv, st))) break;
+    if ((rc = dalloc(&m->eps_d, 16 * 6, st))) break;
+    if ((rc = dalloc(&m->sig_d, 16 * 6, st))) break;
+    if ((rc = dalloc(&m->part, mixed_partials(g, nrhs), st))) break;
+    if ((rc = dalloc((char **)&m->state, mixed_state_bytes(), st))) break;
+    if ((rc = dalloc(&m->act_d, kMaxRhsFields, st))) break;
     e = hipHostMalloc(&m->state_h, mixed_state_bytes(), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&m->act_h, kMaxRhsFields * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&m->eps_h, 16 * 6 * sizeof(double), hipHostMallocDefault);
