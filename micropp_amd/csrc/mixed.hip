@@ -1191,15 +1191,24 @@ __global__ void __launch_bounds__(256) k_m_accum(int NP, const float4 *X, double
 }
 
 // ---------------------------------------------------------------------------
-// calc_ave_stress (src/average.cpp:57-82) of the K fields, FP64: partials
-// part[(field * 6 + v) * gridDim.x + block], then one block per output.
+// calc_ave_stress (src/average.cpp:57-82) of the K fields, FP64, for linear
+// elastic phases (the only ones the multi-RHS solver takes): an element's
+// Gauss-weighted sum of D B_gp u_e over its 8 points is D (sum_gp wg B_gp) u_e,
+// so each element applies the weighted sum of the 8 B matrices once (8x fewer
+// products than point by point; the sums are rounded in another order).
+// Partials part[(field * 6 + v) * gridDim.x + block], then one block per
+// output.
 // ---------------------------------------------------------------------------
 template <int K>
 __global__ void __launch_bounds__(256) k_m_ave(Grid g, int NP, const double *bmat_g, const material_base *mats,
                                                const int *et, const double *U, double *part) {
-  __shared__ double bm[8 * 144];
+  __shared__ double bw[144];
   __shared__ double lds[4 * 6];
-  for (int t = threadIdx.x; t < 8 * 144; t += blockDim.x) bm[t] = bmat_g[t];
+  for (int t = threadIdx.x; t < 144; t += blockDim.x) {
+    double s = 0.0;
+    for (int gp = 0; gp < 8; ++gp) s += bmat_g[gp * 144 + t] * g.wg;
+    bw[t] = s;
+  }
   __syncthreads();
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   int ex = 0, ey = 0, ez = 0, ph = 0;
@@ -1214,21 +1223,21 @@ __global__ void __launch_bounds__(256) k_m_ave(Grid g, int NP, const double *bma
     double s6[6] = {0, 0, 0, 0, 0, 0};
     if (act) {
       double ue[24];
+#pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int node = ((ez + kHexPos[m][2]) * g.ny + (ey + kHexPos[m][1])) * g.nx + ex + kHexPos[m][0];
+#pragma unroll
         for (int d = 0; d < 3; ++d) ue[m * 3 + d] = U[(size_t)(d * K + r) * NP + node];
       }
-      const material_base m = mats[ph];
-      for (int gp = 0; gp < 8; ++gp) {
-        double eps[6], sig[6];
-        for (int v = 0; v < 6; ++v) {
-          double s = 0;
-          for (int q = 0; q < 24; ++q) s += bm[gp * 144 + v * 24 + q] * ue[q];
-          eps[v] = s;
-        }
-        mat_stress(m, eps, nullptr, sig);
-        for (int v = 0; v < 6; ++v) s6[v] += sig[v] * g.wg;
+      double eps[6];
+#pragma unroll
+      for (int v = 0; v < 6; ++v) {
+        double s = 0;
+#pragma unroll
+        for (int q = 0; q < 24; ++q) s += bw[v * 24 + q] * ue[q];
+        eps[v] = s;
       }
+      mat_stress(mats[ph], eps, nullptr, s6);
     }
     block_sum<6, 256>(s6, lds);
     if (threadIdx.x < 6) part[(size_t)(r * 6 + threadIdx.x) * gridDim.x + blockIdx.x] = s6[threadIdx.x];
