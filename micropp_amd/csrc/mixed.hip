@@ -87,21 +87,26 @@ __device__ __forceinline__ int xcd_block(int per) {
 // wave adds its lanes with DPP moves, and thread t < NR adds the waves' sums
 // of row t.  Fixed order throughout.  Thread t < NR returns the sum of row t
 // (others return 0).
+// The two halves of rows_sum, for callers that issue other loads in between
+// (vector-memory loads complete in issue order, so loads issued after the
+// partials' do not delay their sum): rows_load adds this thread's entries of
+// each row in index order, rows_finish combines the threads.
 template <int NR, int UNROLL = 1>
-__device__ __forceinline__ double rows_sum(const double *part, size_t stride, int n) {
+__device__ __forceinline__ void rows_load(const double *part, size_t stride, int n, double (&s)[NR]) {
+#pragma unroll
+  for (int q = 0; q < NR; ++q) s[q] = 0.0;
+  // UNROLL > 1 keeps several rounds of loads in flight (long rows)
+#pragma unroll UNROLL
+  for (int j = threadIdx.x; j < n; j += kRedThreads)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) s[q] += part[(size_t)q * stride + j];
+}
+template <int NR>
+__device__ __forceinline__ double rows_finish(double (&s)[NR]) {
   static_assert(NR <= kRedThreads, "one row per thread at the end");
   constexpr int kW = kRedThreads / 64;
   __shared__ double lds[kW][NR];
   const int t = threadIdx.x, w = t >> 6;
-  double s[NR];
-#pragma unroll
-  for (int q = 0; q < NR; ++q) s[q] = 0.0;
-  // UNROLL > 1 keeps several rounds of loads in flight (long rows); each s[q]
-  // still adds its entries in index order
-#pragma unroll UNROLL
-  for (int j = t; j < n; j += kRedThreads)
-#pragma unroll
-    for (int q = 0; q < NR; ++q) s[q] += part[(size_t)q * stride + j];
   __syncthreads();  // lds may still be read by a previous call
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
@@ -113,6 +118,12 @@ __device__ __forceinline__ double rows_sum(const double *part, size_t stride, in
   if (t < NR)
     for (int i = 0; i < kW; ++i) r += lds[i][t];
   return r;
+}
+template <int NR, int UNROLL = 1>
+__device__ __forceinline__ double rows_sum(const double *part, size_t stride, int n) {
+  double s[NR];
+  rows_load<NR, UNROLL>(part, stride, n, s);
+  return rows_finish<NR>(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -718,7 +729,23 @@ __global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const do
     nbeg = l0 * g.nx;
     nend = l1 * g.nx;
   }
-  const double pap = rows_sum<4, 8>(part1 + (size_t)r0 * nsl, nsl, nsl);  // one partial per slice
+  // The four fields' (p, Ap) partials (one per SpMV slice) are loaded first,
+  // then the block's vectors, which stay in flight while the partials are
+  // summed and alpha is formed.
+  double sp[4];
+  rows_load<4, 8>(part1 + (size_t)r0 * nsl, nsl, nsl, sp);
+  asm volatile("" ::: "memory");
+  float kvv[J];
+  float4 apv[J], rvv[J];
+#pragma unroll
+  for (int jn = 0; jn < J; ++jn) {
+    const int nl = min(nbeg + (int)threadIdx.x + jn * 256, nend - 1);  // clamped; the update skips n >= nend
+    const size_t o = (size_t)c * NP + nl;
+    kvv[jn] = kinv[(size_t)d * NP + nl];
+    apv[jn] = AP[o];
+    rvv[jn] = R[o];
+  }
+  const double pap = rows_finish<4>(sp);
   if (threadIdx.x < 4) {
     const int r = r0 + threadIdx.x;
     const double a = cur->done[r] ? 0.0 : cur->rz[r] / pap;
@@ -733,9 +760,9 @@ __global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const do
     const int n = nbeg + threadIdx.x + jn * 256;
     if (n >= nend) break;
     const size_t o = (size_t)c * NP + n;
-    const float kv = kinv[(size_t)d * NP + n];
-    const float4 ap = AP[o];
-    float4 r = R[o];
+    const float kv = kvv[jn];
+    const float4 ap = apv[jn];
+    float4 r = rvv[jn];
     r.x -= al.x * ap.x;
     r.y -= al.y * ap.y;
     r.z -= al.z * ap.z;
@@ -1085,10 +1112,17 @@ __global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kin
     }
     return;
   }
-  // This block's vectors are loaded first (rows clamped; the update skips
-  // n >= NP), so their latency overlaps the re-sums below.  (In k_m_upd the
-  // same order measured slower: its re-sum of one partial per SpMV slice is
-  // long enough that its loads queue behind the vector loads.)
+  const size_t row = (size_t)3 * gridDim.x, rowz = (size_t)3 * nbz;  // (r, z) / (z, z) partials per field
+  // The block's (z, z) / (r, z) partials are loaded first (block (0, 0) sums
+  // every field's after its vectors), then this block's vectors (rows
+  // clamped; the update skips n >= NP), which stay in flight while the
+  // partials are summed.
+  double szz[4] = {}, srz[4] = {};
+  if (!head) {
+    rows_load<4>(pz + r0 * rowz, rowz, (int)rowz, szz);
+    rows_load<4>(pr + r0 * row, row, (int)row, srz);
+  }
+  asm volatile("" ::: "memory");
   constexpr int J = kVecNodes / 256;
   float kvv[J];
   float4 rvv[J], pvv[J], xvv[J];
@@ -1107,7 +1141,6 @@ __global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kin
     pvv[jn] = P[o];
     xvv[jn] = X[o];
   }
-  const size_t row = (size_t)3 * gridDim.x, rowz = (size_t)3 * nbz;  // (r, z) / (z, z) partials per field
   const int its = cur->its + 1;
   if (head) {
     const double zz = rows_sum<K>(pz, rowz, (int)rowz);
@@ -1137,8 +1170,8 @@ __global__ void __launch_bounds__(kRedThreads) k_m_pupd(int NP, const float *kin
       nxt->all_done = all || its >= cur->max_its;
     }
   } else {
-    const double zz = rows_sum<4>(pz + r0 * rowz, rowz, (int)rowz);
-    const double rz = rows_sum<4>(pr + r0 * row, row, (int)row);
+    const double zz = rows_finish<4>(szz);
+    const double rz = rows_finish<4>(srz);
     if (threadIdx.x < 4) {
       const MStep o = m_step(cur, r0 + threadIdx.x, zz, rz);
       beta_s[threadIdx.x] = o.beta;
