@@ -329,7 +329,9 @@ __global__ void __launch_bounds__(256) k_m_adj_src(Grid g, int NP, int nadj, con
 // given), rounded into the FP32 CG start: R = r, X = 0, P = kinv R.
 // Partials: part[q * gridDim.x + block], q < K: ||kinv r||^2 (FP64) of field
 // q; K <= q < 2K: (R, P) of field q - K; 2K <= q: ||r||^2 (FP64) of field
-// q - 2K.
+// q - 2K.  Grid (node blocks, K/4): block (b, grp) takes fields 4 grp ..
+// 4 grp + 3 (each group rebuilds the node's stencil; four fields' sums keep
+// the registers at two waves per SIMD, twelve took all 256 VGPRs).
 // ---------------------------------------------------------------------------
 template <int K>
 __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const int *et, const double *ke,
@@ -337,14 +339,15 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const 
                                                  float4 *X, float4 *P, double *part) {
   constexpr int KG = K / 4;
   __shared__ double ke_s[3 * 576];
-  __shared__ double lds[4 * 3 * K];
+  __shared__ double lds[4 * 3 * 4];
+  const int grp = blockIdx.y, f0 = 4 * grp;
   for (int t = threadIdx.x; t < 3 * 576; t += blockDim.x) ke_s[t] = ke[t];
   __syncthreads();
   const int lb = xcd_block(per);
   const int n = lb * 256 + threadIdx.x;
-  double zz[K], rz[K], rr[K];
+  double zz[4], rz[4], rr[4];
 #pragma unroll
-  for (int r = 0; r < K; ++r) zz[r] = rz[r] = rr[r] = 0.0;
+  for (int r = 0; r < 4; ++r) zz[r] = rz[r] = rr[r] = 0.0;
   if (n < NP) {
     const int i = n % g.nx, j = (n / g.nx) % g.ny, k = n / (g.nx * g.ny);
     const bool interior = n < g.nn && !node_on_boundary(g, i, j, k);
@@ -352,11 +355,11 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const 
     // m = n + (dx, dy, dz) is the sum of the element blocks Ke(n, m) over the
     // (1, 2, 4 or 8) elements holding both, and is applied to every field
     // once, so each neighbour's u is loaded once per field.
-    double acc[3][K];
+    double acc[3][4];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int q = 0; q < K; ++q) acc[d][q] = 0.0;
+      for (int q = 0; q < 4; ++q) acc[d][q] = 0.0;
     if (interior) {
       const double *kbe[8];  // element (a, b, c) = (i - 1 + a, j - 1 + b, k - 1 + c): its phase block
 #pragma unroll
@@ -392,50 +395,49 @@ __global__ void __launch_bounds__(256) k_m_resid(Grid g, int NP, int per, const 
                 }
             const int m = n + (dz * g.ny + dy) * g.nx + dx;
 #pragma unroll
-            for (int q = 0; q < K; ++q)
+            for (int q = 0; q < 4; ++q)
 #pragma unroll
               for (int fj = 0; fj < 3; ++fj) {
-                const double u = U[(size_t)(fj * K + q) * NP + m];
+                const double u = U[(size_t)(fj * K + f0 + q) * NP + m];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) acc[d][q] += cf[d * 3 + fj] * u;
               }
           }
     }
 #pragma unroll
-    for (int kg = 0; kg < K; kg += 4) {
+    for (int d = 0; d < 3; ++d) {
+      const float kv = kinv[(size_t)d * NP + n];
+      float rf[4], pp[4];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const float kv = kinv[(size_t)d * NP + n];
-        float rf[4], pp[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const size_t idx = (size_t)(d * K + kg + q) * NP + n;
-          const double r64 = (F ? F[idx] : 0.0) - acc[d][kg + q];
-          rf[q] = (float)r64;
-          pp[q] = kv * rf[q];
-          const double z64 = (double)kv * r64;
-          zz[kg + q] += z64 * z64;
-          rz[kg + q] += (double)rf[q] * (double)pp[q];
-          rr[kg + q] += r64 * r64;
-        }
-        const size_t o = (size_t)(d * KG + kg / 4) * NP + n;
-        R[o] = make_float4(rf[0], rf[1], rf[2], rf[3]);
-        X[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        P[o] = make_float4(pp[0], pp[1], pp[2], pp[3]);
+      for (int q = 0; q < 4; ++q) {
+        const size_t idx = (size_t)(d * K + f0 + q) * NP + n;
+        const double r64 = (F ? F[idx] : 0.0) - acc[d][q];
+        rf[q] = (float)r64;
+        pp[q] = kv * rf[q];
+        const double z64 = (double)kv * r64;
+        zz[q] += z64 * z64;
+        rz[q] += (double)rf[q] * (double)pp[q];
+        rr[q] += r64 * r64;
       }
+      const size_t o = (size_t)(d * KG + grp) * NP + n;
+      R[o] = make_float4(rf[0], rf[1], rf[2], rf[3]);
+      X[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      P[o] = make_float4(pp[0], pp[1], pp[2], pp[3]);
     }
   }
-  double v[3 * K];
+  double v[12];
 #pragma unroll
-  for (int r = 0; r < K; ++r) {
+  for (int r = 0; r < 4; ++r) {
     v[r] = zz[r];
-    v[K + r] = rz[r];
-    v[2 * K + r] = rr[r];
+    v[4 + r] = rz[r];
+    v[8 + r] = rr[r];
   }
-  block_sum<3 * K, 256>(v, lds);
+  block_sum<12, 256>(v, lds);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < 3 * K; ++q) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) part[(size_t)(t * K + f0 + q) * gridDim.x + blockIdx.x] = v[t * 4 + q];
 }
 
 // one block: outer convergence test and the inner CG start state.  active
@@ -1652,7 +1654,7 @@ hipError_t launch_m_resid(const Grid &g, int K, const int *et, const double *ke,
   MState *ms = static_cast<MState *>(state);
   MOuter *mo = reinterpret_cast<MOuter *>(ms + 2);
   MPP_K_DISPATCH(K, {
-    hipLaunchKernelGGL(k_m_resid<KK>, dim3(s.nb), dim3(256), 0, st, g, s.NP, s.per, et, ke, U, F, kinv,
+    hipLaunchKernelGGL(k_m_resid<KK>, dim3(s.nb, KK / 4), dim3(256), 0, st, g, s.NP, s.per, et, ke, U, F, kinv,
                        reinterpret_cast<float4 *>(R), reinterpret_cast<float4 *>(X), reinterpret_cast<float4 *>(P),
                        part);
     hipLaunchKernelGGL(k_m_outer<KK>, dim3(1), dim3(kRedThreads), 0, st, s.nb, part, first, rel, abs_tol, tau_min,
