@@ -237,8 +237,10 @@ hipError_t launch_m_adj_src(const Grid &g, int K, int nadj, const double *bmat, 
 // forward fields): ctan[36] = d sigma_v / d eps_i (v * 6 + i), dmat[ns * 36] =
 // d sigma_v / d (E, nu) of phase p ((s * 6 + v) * 6 + p * 2 + {0, 1})
 size_t mixed_adj_partials(const Grid &g, int ns);
+// (ke: the per-phase FP64 element stiffness, 3 x 576)
 hipError_t launch_m_adj(const Grid &g, int K, const double *bmat, const material_base *mats_d, const int *et,
-                        const double *U, int s0, int ns, double *part, double *ctan, double *dmat, hipStream_t st);
+                        const double *ke, const double *U, int s0, int ns, double *part, double *ctan, double *dmat,
+                        hipStream_t st);
 // one inner CG iteration; R X P AP in the 4-field-group layout
 // Two-level preconditioner: coarse (may be null: Jacobi alone).  start:
 // after launch_m_resid, the inner CG's start direction P = z = D^-1 R +

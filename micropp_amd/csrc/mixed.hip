@@ -1322,62 +1322,110 @@ __device__ __forceinline__ void unit_bc(const Grid &g, int ui, int i, int j, int
   }
 }
 
-// grid (element blocks, 6): blockIdx.y = v; part[(v * 6 + i) * gridDim.x + block]
+// The elements with a node on the boundary (the outer layer), in a compact
+// order: layer ez = 0 whole, the rings of the middle layers, layer nez - 1
+// whole.  boundary_elems: how many; boundary_elem: element idx of them.
+__host__ __device__ inline int boundary_ring(const Grid &g) {
+  const int full = g.nex * g.ney;
+  return g.nex > 2 && g.ney > 2 ? full - (g.nex - 2) * (g.ney - 2) : full;
+}
+__host__ __device__ inline int boundary_elems(const Grid &g) {
+  const int full = g.nex * g.ney;
+  return g.nez == 1 ? full : 2 * full + (g.nez - 2) * boundary_ring(g);
+}
+__device__ __forceinline__ void boundary_elem(const Grid &g, int idx, int &ex, int &ey, int &ez) {
+  const int full = g.nex * g.ney, ring = boundary_ring(g), mid = g.nez >= 2 ? (g.nez - 2) * ring : 0;
+  if (idx < full || idx >= full + mid) {
+    const int rem = idx < full ? idx : idx - full - mid;
+    ex = rem % g.nex;
+    ey = rem / g.nex;
+    ez = idx < full ? 0 : g.nez - 1;
+    return;
+  }
+  const int rem = idx - full, r = rem % ring;
+  ez = 1 + rem / ring;
+  if (ring == full) {
+    ex = r % g.nex;
+    ey = r / g.nex;
+  } else if (r < g.nex) {
+    ex = r, ey = 0;
+  } else if (r < 2 * g.nex) {
+    ex = r - g.nex, ey = g.ney - 1;
+  } else if (r < 2 * g.nex + g.ney - 2) {
+    ex = 0, ey = 1 + r - 2 * g.nex;
+  } else {
+    ex = g.nex - 1, ey = 1 + r - 2 * g.nex - (g.ney - 2);
+  }
+}
+
+// d sigma_v / d eps_i: only elements with a boundary node see w_i.  With the
+// element's Gauss sums folded (elastic phases), its term is
+//   sum_gp wg ((C B w_i)_v / V - (B lambda_v)^T C B w_i)
+//     = (C Bw w_i)_v / V - lambda_v^T Ke w_i,   Bw = sum_gp wg B_gp,
+// Ke = sum_gp B^T (C B wg) the phase's element stiffness (ke, the same sums
+// as the multi-RHS matrix's).  Grid (boundary-element blocks, 6): blockIdx.y =
+// i; a thread forms Ke w_i and C Bw w_i once for the six v.
+// part[(v * 6 + i) * gridDim.x + block].
 template <int K>
 __global__ void __launch_bounds__(256) k_m_adj_ctan(Grid g, int NP, const double *bmat_g, const material_base *mats,
-                                                    const int *et, const double *U, double *part) {
-  __shared__ double bm[8 * 144];
+                                                    const int *et, const double *ke, const double *U, double *part) {
+  __shared__ double bw[144];
+  __shared__ double ke_s[3 * 576];
   __shared__ double lds[4 * 6];
-  for (int t = threadIdx.x; t < 8 * 144; t += blockDim.x) bm[t] = bmat_g[t];
+  for (int t = threadIdx.x; t < 144; t += blockDim.x) {
+    double s = 0.0;
+    for (int gp = 0; gp < 8; ++gp) s += bmat_g[gp * 144 + t] * g.wg;
+    bw[t] = s;
+  }
+  for (int t = threadIdx.x; t < 3 * 576; t += blockDim.x) ke_s[t] = ke[t];
   __syncthreads();
-  const int v = blockIdx.y;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ui = blockIdx.y;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  if (e < g.nelem) {
-    const int ex = e % g.nex, ey = (e / g.nex) % g.ney, ez = e / (g.nex * g.ney);
-    if (elem_touches_boundary(g, ex, ey, ez)) {
-      const material_base m = mats[et[e]];
-      double lam[24];
-      int bnd[8], ni[8], nj[8], nk[8];
-      for (int a = 0; a < 8; ++a) {
-        ni[a] = ex + kHexPos[a][0];
-        nj[a] = ey + kHexPos[a][1];
-        nk[a] = ez + kHexPos[a][2];
-        bnd[a] = node_on_boundary(g, ni[a], nj[a], nk[a]);
-        const int node = (nk[a] * g.ny + nj[a]) * g.nx + ni[a];
-        for (int d = 0; d < 3; ++d) lam[a * 3 + d] = U[(size_t)(d * K + v) * NP + node];
-      }
-      for (int ui = 0; ui < 6; ++ui) {
-        double we[24];
-        for (int a = 0; a < 8; ++a) {
-          if (bnd[a]) {
-            unit_bc(g, ui, ni[a], nj[a], nk[a], &we[a * 3]);
-          } else {
-            we[a * 3] = we[a * 3 + 1] = we[a * 3 + 2] = 0.0;
-          }
-        }
-        for (int gp = 0; gp < 8; ++gp) {
-          const double *B = &bm[gp * 144];
-          double ew[6], el[6], s[6];
-          for (int q = 0; q < 6; ++q) {
-            double a1 = 0.0, a2 = 0.0;
-            for (int c = 0; c < 24; ++c) {
-              a1 += B[q * 24 + c] * we[c];
-              a2 += B[q * 24 + c] * lam[c];
-            }
-            ew[q] = a1;
-            el[q] = a2;
-          }
-          lin_stress(m, ew, s);
-          double dl = 0.0;
-          for (int q = 0; q < 6; ++q) dl += el[q] * s[q];
-          acc[ui] += g.wg * (s[v] / g.vol_tot - dl);
-        }
-      }
+  if (idx < boundary_elems(g)) {
+    int ex, ey, ez;
+    boundary_elem(g, idx, ex, ey, ez);
+    const int e = (ez * g.ney + ey) * g.nex + ex, ph = et[e];
+    int node[8];
+    double we[24];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int ni = ex + kHexPos[a][0], nj = ey + kHexPos[a][1], nk = ez + kHexPos[a][2];
+      node[a] = (nk * g.ny + nj) * g.nx + ni;
+      double w3[3] = {0.0, 0.0, 0.0};  // (a small array of its own: unit_bc indexes it at run time)
+      if (node_on_boundary(g, ni, nj, nk)) unit_bc(g, ui, ni, nj, nk, w3);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) we[a * 3 + d] = w3[d];
+    }
+    double ew[6], sw[6], t[24];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      double a1 = 0.0;
+#pragma unroll
+      for (int c = 0; c < 24; ++c) a1 += bw[q * 24 + c] * we[c];
+      ew[q] = a1;
+    }
+    lin_stress(mats[ph], ew, sw);
+    const double *kp = ke_s + ph * 576;
+#pragma unroll
+    for (int r = 0; r < 24; ++r) {
+      double a1 = 0.0;
+#pragma unroll
+      for (int c = 0; c < 24; ++c) a1 += kp[r * 24 + c] * we[c];
+      t[r] = a1;
+    }
+#pragma unroll
+    for (int v = 0; v < 6; ++v) {
+      double dl = 0.0;
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dl += U[(size_t)(d * K + v) * NP + node[a]] * t[a * 3 + d];
+      acc[v] = sw[v] / g.vol_tot - dl;
     }
   }
   block_sum<6, 256>(acc, lds);
-  if (threadIdx.x < 6) part[(size_t)(v * 6 + threadIdx.x) * gridDim.x + blockIdx.x] = acc[threadIdx.x];
+  if (threadIdx.x < 6) part[(size_t)(threadIdx.x * 6 + ui) * gridDim.x + blockIdx.x] = acc[threadIdx.x];
 }
 
 // d lambda / d p and d mu / d p of an elastic phase (p = E: which 0, nu: 1),
@@ -1816,12 +1864,13 @@ hipError_t launch_m_adj_src(const Grid &g, int K, int nadj, const double *bmat, 
 }
 
 hipError_t launch_m_adj(const Grid &g, int K, const double *bmat, const material_base *mats_d, const int *et,
-                        const double *U, int s0, int ns, double *part, double *ctan, double *dmat, hipStream_t st) {
+                        const double *ke, const double *U, int s0, int ns, double *part, double *ctan, double *dmat,
+                        hipStream_t st) {
   const MLaunch s = mshape(g);
-  const unsigned nb = nblk(g.nelem, 256);
-  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_adj_ctan<KK>, dim3(nb, 6), dim3(256), 0, st, g, s.NP, bmat, mats_d, et,
-                                       U, part));
-  hipLaunchKernelGGL(k_m_rows, dim3(36), dim3(256), 0, st, part, (int)nb, 1.0, ctan);
+  const unsigned nb = nblk(g.nelem, 256), nbb = nblk(boundary_elems(g), 256);
+  MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_adj_ctan<KK>, dim3(nbb, 6), dim3(256), 0, st, g, s.NP, bmat, mats_d, et,
+                                       ke, U, part));
+  hipLaunchKernelGGL(k_m_rows, dim3(36), dim3(256), 0, st, part, (int)nbb, 1.0, ctan);
   if (ns > 0) {
     MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_adj_dmat<KK>, dim3(nb, 6 * ns), dim3(256), 0, st, g, s.NP, bmat,
                                          mats_d, et, U, s0, part));

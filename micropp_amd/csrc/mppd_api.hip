@@ -966,8 +966,8 @@ int mppd_msolver_adjoint(mppd_msolver *m, int n, const double *eps_h, const mppd
   std::vector<double> pn, pn0;
   int all_conv = 0;
   if ((rc = msolver_run(m, 6, n, eps_h, p, stats, pn, pn0, &all_conv))) return rc;
-  MPP_HIP(launch_m_adj(g, K, m->r->bmat, m->mats_d, m->r->et, m->U, 6, n, m->apart, m->adj_d, m->adj_d + 36,
-                       m->st));
+  MPP_HIP(launch_m_adj(g, K, m->r->bmat, m->mats_d, m->r->et, m->ke, m->U, 6, n, m->apart, m->adj_d,
+                       m->adj_d + 36, m->st));
   MPP_HIP(hipMemcpyAsync(m->adj_h, m->adj_d, 36 * (n + 1) * sizeof(double), hipMemcpyDeviceToHost, m->st));
   if ((rc = msolver_averages(m))) return rc;  // synchronises the stream
   for (int i = 0; i < 36; ++i) ctan_h[i] = m->adj_h[i];
