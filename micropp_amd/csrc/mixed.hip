@@ -1443,61 +1443,101 @@ __device__ __forceinline__ material_base dmat(const material_base &m, int which)
   return d;
 }
 
-// grid (element blocks, ns * 6): blockIdx.y = s * 6 + v; forward field s0 + s;
-// part[((s * 6 + v) * 6 + phase * 2 + {E: 0, nu: 1}) * gridDim.x + block]
+// d sigma_v / d theta for theta = E, nu of the element's phase: with the Gauss
+// sums folded (elastic phases) an element's term is
+//   (dC Bw u_s)_v / V - lambda_v^T dKe u_s,  dKe = sum_gp wg B^T dC B,
+// and since C = lambda N + mu M (N: ones on the normal 3x3 block, M =
+// diag(2, 2, 2, 1, 1, 1)), dKe = dlambda KN + dmu KM with KN, KM the element
+// matrices of N and M (built per block in LDS).  Grid (element blocks, ns):
+// blockIdx.y = s, forward field s0 + s; a thread forms dKe u_s once for the six
+// v.  part[((s * 6 + v) * 6 + phase * 2 + {E: 0, nu: 1}) * gridDim.x + block].
 template <int K>
 __global__ void __launch_bounds__(256) k_m_adj_dmat(Grid g, int NP, const double *bmat_g, const material_base *mats,
                                                     const int *et, const double *U, int s0, double *part) {
-  __shared__ double bm[8 * 144];
+  __shared__ double bw[144];
+  __shared__ double kn[576], km[576];
   __shared__ double lds[4 * 6];
-  for (int t = threadIdx.x; t < 8 * 144; t += blockDim.x) bm[t] = bmat_g[t];
-  __syncthreads();
-  const int s = blockIdx.y / 6, v = blockIdx.y % 6;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  if (e < g.nelem) {
-    const int ex = e % g.nex, ey = (e / g.nex) % g.ney, ez = e / (g.nex * g.ney);
-    const int p = et[e];
-    const material_base dE = dmat(mats[p], 0), dN = dmat(mats[p], 1);
-    double lam[24], ue[24];
-    for (int a = 0; a < 8; ++a) {
-      const int node = ((ez + kHexPos[a][2]) * g.ny + (ey + kHexPos[a][1])) * g.nx + ex + kHexPos[a][0];
-      for (int d = 0; d < 3; ++d) {
-        lam[a * 3 + d] = U[(size_t)(d * K + v) * NP + node];
-        ue[a * 3 + d] = U[(size_t)(d * K + s0 + s) * NP + node];
-      }
-    }
-    double cE = 0.0, cN = 0.0;
-    for (int gp = 0; gp < 8; ++gp) {
-      const double *B = &bm[gp * 144];
-      double eu[6], el[6], sE[6], sN[6];
-      for (int q = 0; q < 6; ++q) {
-        double a1 = 0.0, a2 = 0.0;
-        for (int c = 0; c < 24; ++c) {
-          a1 += B[q * 24 + c] * ue[c];
-          a2 += B[q * 24 + c] * lam[c];
-        }
-        eu[q] = a1;
-        el[q] = a2;
-      }
-      lin_stress(dE, eu, sE);
-      lin_stress(dN, eu, sN);
-      double dlE = 0.0, dlN = 0.0;
-      for (int q = 0; q < 6; ++q) {
-        dlE += el[q] * sE[q];
-        dlN += el[q] * sN[q];
-      }
-      cE += g.wg * (sE[v] / g.vol_tot - dlE);
-      cN += g.wg * (sN[v] / g.vol_tot - dlN);
-    }
-    for (int q = 0; q < 3; ++q)
-      if (p == q) {
-        acc[2 * q] = cE;
-        acc[2 * q + 1] = cN;
-      }
+  for (int t = threadIdx.x; t < 144; t += blockDim.x) {
+    double sum = 0.0;
+    for (int gp = 0; gp < 8; ++gp) sum += bmat_g[gp * 144 + t] * g.wg;
+    bw[t] = sum;
   }
-  block_sum<6, 256>(acc, lds);
-  if (threadIdx.x < 6) part[(size_t)(blockIdx.y * 6 + threadIdx.x) * gridDim.x + blockIdx.x] = acc[threadIdx.x];
+  for (int t = threadIdx.x; t < 576; t += blockDim.x) {
+    const int i = t / 24, j = t % 24;
+    double sn = 0.0, sm = 0.0;
+    for (int gp = 0; gp < 8; ++gp) {
+      const double *B = bmat_g + gp * 144;
+      const double ni = B[i] + B[24 + i] + B[48 + i], nj = B[j] + B[24 + j] + B[48 + j];
+      const double mm = 2.0 * (B[i] * B[j] + B[24 + i] * B[24 + j] + B[48 + i] * B[48 + j]) +
+                        B[72 + i] * B[72 + j] + B[96 + i] * B[96 + j] + B[120 + i] * B[120 + j];
+      sn += g.wg * (ni * nj);
+      sm += g.wg * mm;
+    }
+    kn[t] = sn;
+    km[t] = sm;
+  }
+  __syncthreads();
+  const int s = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = e < g.nelem;
+  int p = 0, node[8];
+  double tE[24], tN[24], sE[6], sN[6];
+  if (act) {
+    const int ex = e % g.nex, ey = (e / g.nex) % g.ney, ez = e / (g.nex * g.ney);
+    p = et[e];
+    const material_base dE = dmat(mats[p], 0), dN = dmat(mats[p], 1);
+    double ue[24];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      node[a] = ((ez + kHexPos[a][2]) * g.ny + (ey + kHexPos[a][1])) * g.nx + ex + kHexPos[a][0];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) ue[a * 3 + d] = U[(size_t)(d * K + s0 + s) * NP + node[a]];
+    }
+    double eu[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      double a1 = 0.0;
+#pragma unroll
+      for (int c = 0; c < 24; ++c) a1 += bw[q * 24 + c] * ue[c];
+      eu[q] = a1;
+    }
+    lin_stress(dE, eu, sE);
+    lin_stress(dN, eu, sN);
+#pragma unroll
+    for (int r = 0; r < 24; ++r) {
+      double an = 0.0, am = 0.0;
+#pragma unroll
+      for (int c = 0; c < 24; ++c) {
+        an += kn[r * 24 + c] * ue[c];
+        am += km[r * 24 + c] * ue[c];
+      }
+      tE[r] = dE.lambda * an + dE.mu * am;
+      tN[r] = dN.lambda * an + dN.mu * am;
+    }
+  }
+  for (int v = 0; v < 6; ++v) {
+    double c6[6] = {0, 0, 0, 0, 0, 0};
+    if (act) {
+      double dlE = 0.0, dlN = 0.0;
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double l = U[(size_t)(d * K + v) * NP + node[a]];
+          dlE += l * tE[a * 3 + d];
+          dlN += l * tN[a * 3 + d];
+        }
+      const double cE = sE[v] / g.vol_tot - dlE, cN = sN[v] / g.vol_tot - dlN;
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (p == q) {
+          c6[q * 2] = cE;
+          c6[q * 2 + 1] = cN;
+        }
+    }
+    block_sum<6, 256>(c6, lds);
+    if (threadIdx.x < 6) part[((size_t)(s * 6 + v) * 6 + threadIdx.x) * gridDim.x + blockIdx.x] = c6[threadIdx.x];
+  }
 }
 
 // one block per row: out[row] = (sum of part[row * n .. + n)) / div, fixed order
@@ -1872,7 +1912,7 @@ hipError_t launch_m_adj(const Grid &g, int K, const double *bmat, const material
                                        ke, U, part));
   hipLaunchKernelGGL(k_m_rows, dim3(36), dim3(256), 0, st, part, (int)nbb, 1.0, ctan);
   if (ns > 0) {
-    MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_adj_dmat<KK>, dim3(nb, 6 * ns), dim3(256), 0, st, g, s.NP, bmat,
+    MPP_K_DISPATCH(K, hipLaunchKernelGGL(k_m_adj_dmat<KK>, dim3(nb, ns), dim3(256), 0, st, g, s.NP, bmat,
                                          mats_d, et, U, s0, part));
     hipLaunchKernelGGL(k_m_rows, dim3(36 * ns), dim3(256), 0, st, part, (int)nb, 1.0, dmat);
   }
