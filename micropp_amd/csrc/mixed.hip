@@ -837,8 +837,8 @@ __device__ __forceinline__ MStep m_step(const MState *cur, int r, double zz, dou
 // Two-level preconditioner: z = D^-1 r + P_c A_c^-1 P_c^T r, with P_c the
 // trilinear hats of the interior nodes of a coarse grid aligned with the fine
 // one (cg.hx fine elements per coarse element along x, ...), A_c = P_c^T A
-// P_c (Galerkin; built on the host, inverted once, kept as a dense FP32
-// inverse).  Per application: restriction along x (k_c_rx), then along y and
+// P_c (Galerkin; assembled and inverted once on the device, coarse.hip, and
+// kept as a dense FP32 inverse).  Per application: restriction along x (k_c_rx), then along y and
 // z (k_c_ryz), the dense coarse solve in kCoarseSplit column splits
 // (k_c_gemv) and their sum (k_c_gemv_sum), then z with the prolongation
 // (k_c_zdot), which k_m_pupd reads.  Restriction and prolongation use the
