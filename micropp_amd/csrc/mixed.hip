@@ -870,18 +870,19 @@ __device__ __forceinline__ void axis_tables(const Grid &g, const CGrid &cg, Axis
   __syncthreads();
 }
 
-// P_c e at a fine node whose sides' table entries are ax, ay, az, DOF
-// direction d, fields 4 grp .. 4 grp + 3: the 8 coarse nodes around it, all
-// loaded at once (zero weights outside)
-template <int K>
-__device__ __forceinline__ float4 prolong(const CGrid &cg, const float4 *ec4, const AxisW &ax, const AxisW &ay,
-                                          const AxisW &az, int d, int grp) {
-  constexpr int KG = K / 4;
+// P_c e at a fine node whose sides' table entries are ax, ay, az, from the
+// block's staged coarse corrections el (coarse z-levels czlo .. czhi, every
+// y and x, of the block's direction and field group): the 8 coarse nodes
+// around the node, all read at once.  A corner outside the grid has weight 0
+// and index 0, which may lie below czlo: its level is clamped into the staged
+// ones.
+__device__ __forceinline__ float4 prolong(const CGrid &cg, const float4 *el, int czlo, int czhi, const AxisW &ax,
+                                          const AxisW &ay, const AxisW &az) {
   float4 e[8];
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
-    const int q = (az.q[t >> 2] * cg.my + ay.q[(t >> 1) & 1]) * cg.mx + ax.q[t & 1];
-    e[t] = ec4[(size_t)(q * 3 + d) * KG + grp];
+    const int cz = min(max(az.q[t >> 2], czlo), czhi) - czlo;
+    e[t] = el[(cz * cg.my + ay.q[(t >> 1) & 1]) * cg.mx + ax.q[t & 1]];
   }
   float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
@@ -1028,7 +1029,23 @@ __global__ void __launch_bounds__(kZdotThreads) k_c_zdot(Grid g, CGrid cg, int N
     if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole block
   }
   __shared__ AxisW tab[3][kCoarseMaxLine];
-  axis_tables(g, cg, tab);
+  __shared__ float4 el[kCoarseMaxDofs / 3 + 1];
+  // the coarse corrections of this block's nodes, staged: coarse z-levels
+  // k / hz - 1 and k / hz (0-based, where inside 0 .. mz - 1) of its planes
+  // (a block of the last plane alone, k = nz - 1, where every hat is zero,
+  // stages one level)
+  {
+    const int nxy = g.nx * g.ny, nlo = blockIdx.x * kVecNodes, nhi = min(nlo + kVecNodes, NP) - 1;
+    const int zhi = min(cg.mz - 1, min(nhi / nxy, g.nz - 1) / cg.hz);
+    const int zlo = min(zhi, max(0, min(nlo / nxy, g.nz - 1) / cg.hz - 1));
+    const int plane = cg.mx * cg.my;
+    for (int t = threadIdx.x; t < (zhi - zlo + 1) * plane; t += kZdotThreads)
+      el[t] = ec4[(size_t)((zlo * plane + t) * 3 + d) * KG + grp];
+  }
+  axis_tables(g, cg, tab);  // (its barrier also publishes el)
+  const int nxy0 = g.nx * g.ny, n0lo = blockIdx.x * kVecNodes, n0hi = min(n0lo + kVecNodes, NP) - 1;
+  const int czhi = min(cg.mz - 1, min(n0hi / nxy0, g.nz - 1) / cg.hz);
+  const int czlo = min(czhi, max(0, min(n0lo / nxy0, g.nz - 1) / cg.hz - 1));
   constexpr int J = kVecNodes / kZdotThreads;
   const int n0 = blockIdx.x * kVecNodes + threadIdx.x;
   // the node's (i, j, k), stepped by kZdotThreads nodes per round (nodes past
@@ -1043,7 +1060,7 @@ __global__ void __launch_bounds__(kZdotThreads) k_c_zdot(Grid g, CGrid cg, int N
     const size_t o = (size_t)c * NP + nl;
     const float kv = kinv[(size_t)d * NP + nl];
     const float4 r = R[o];
-    const float4 e = prolong<K>(cg, ec4, tab[0][i], tab[1][j], tab[2][min(k, g.nz - 1)], d, grp);
+    const float4 e = prolong(cg, el, czlo, czhi, tab[0][i], tab[1][j], tab[2][min(k, g.nz - 1)]);
     const bool real = n < g.nn;  // padding nodes have r = 0 and no coarse part
     i += di;
     int carry = dl;
