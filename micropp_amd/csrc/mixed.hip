@@ -929,31 +929,50 @@ __global__ void __launch_bounds__(256) k_c_rx(Grid g, CGrid cg, int NP, const fl
   T1[((size_t)(c * g.nz + k) * g.ny + j) * cg.mx + (cx - 1)] = acc;
 }
 
-// Restriction along y and z: one wave per (interior coarse node q, component
-// c); its lanes take the (2 hy - 1)(2 hz - 1) x-restricted lines under q's
-// hats in turn, then the wave adds them up.
+// Restriction along y and z: one block per (interior coarse y-z pair, component
+// c), for all mx coarse x nodes at once.  Thread t takes coarse x node t % mx
+// and the (j, k) lines slot, slot + nslot, ... of the pair's window
+// ((2 hy - 1)(2 hz - 1) lines, slot = t / mx, nslot = 256 / mx), so a line's
+// mx x-restricted values are read together; the slots' sums are added in LDS
+// in slot order.  (The window lines of neighbouring pairs overlap: each value
+// is read by up to 4 blocks.)
 template <int K>
 __global__ void __launch_bounds__(256) k_c_ryz(Grid g, CGrid cg, const float4 *T1, float4 *rc4, const MState *cur) {
   constexpr int KG = K / 4;
+  __shared__ float4 part[256];
   if (cur->all_done) return;
-  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (wave >= cg.nq * 3 * KG) return;  // whole wave
-  const int c = wave % (3 * KG), q = wave / (3 * KG), d = c / KG, grp = c % KG;
-  const int cx = q % cg.mx + 1, cy = (q / cg.mx) % cg.my + 1, cz = q / (cg.mx * cg.my) + 1;
+  const int c = blockIdx.y, d = c / KG, grp = c % KG;
+  const int cy = blockIdx.x % cg.my + 1, cz = blockIdx.x / cg.my + 1;
+  const int mx = cg.mx, nslot = 256 / mx, t = threadIdx.x;
   const int wy = 2 * cg.hy - 1, npair = wy * (2 * cg.hz - 1);
   const int j0 = (cy - 1) * cg.hy + 1, k0 = (cz - 1) * cg.hz + 1;
   float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  for (int s = lane; s < npair; s += 64) {
-    const int j = j0 + s % wy, k = k0 + s / wy;
-    const float w = hat(j, cy, cg.hy) * hat(k, cz, cg.hz);
-    const float4 v = T1[((size_t)(c * g.nz + k) * g.ny + j) * cg.mx + (cx - 1)];
-    acc.x += w * v.x;
-    acc.y += w * v.y;
-    acc.z += w * v.z;
-    acc.w += w * v.w;
+  if (t < nslot * mx) {
+    const int cx = t % mx;
+    for (int s = t / mx; s < npair; s += nslot) {
+      const int j = j0 + s % wy, k = k0 + s / wy;
+      const float w = hat(j, cy, cg.hy) * hat(k, cz, cg.hz);
+      const float4 v = T1[((size_t)(c * g.nz + k) * g.ny + j) * mx + cx];
+      acc.x += w * v.x;
+      acc.y += w * v.y;
+      acc.z += w * v.z;
+      acc.w += w * v.w;
+    }
   }
-  const float4 sum = make_float4(wave_sum_dpp(acc.x), wave_sum_dpp(acc.y), wave_sum_dpp(acc.z), wave_sum_dpp(acc.w));
-  if (lane == 0) rc4[(size_t)(q * 3 + d) * KG + grp] = sum;
+  part[t] = acc;
+  __syncthreads();
+  if (t < mx) {
+    float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int sl = 0; sl < nslot; ++sl) {
+      const float4 v = part[sl * mx + t];
+      sum.x += v.x;
+      sum.y += v.y;
+      sum.z += v.z;
+      sum.w += v.w;
+    }
+    const int q = ((cz - 1) * cg.my + (cy - 1)) * mx + t;
+    rc4[(size_t)(q * 3 + d) * KG + grp] = sum;
+  }
 }
 
 // The coarse solve e = A_c^-1 r_c (nc x nc times nc x K), split over the
@@ -1778,7 +1797,7 @@ static void coarse_solve(const Grid &g, const MLaunch &s, const CoarseDev &c, co
   if (rx)  // else T1 is already made (k_m_upd<K, true>)
     hipLaunchKernelGGL(k_c_rx<KK>, dim3(nblk(rx_threads, 256)), dim3(256), 0, st, g, cg, s.NP,
                        reinterpret_cast<const float4 *>(R), reinterpret_cast<float4 *>(c.t1), cur);
-  hipLaunchKernelGGL(k_c_ryz<KK>, dim3(nblk((size_t)cg.nq * 3 * KG, 4)), dim3(256), 0, st, g, cg,
+  hipLaunchKernelGGL(k_c_ryz<KK>, dim3(cg.my * cg.mz, 3 * KG), dim3(256), 0, st, g, cg,
                      reinterpret_cast<const float4 *>(c.t1), reinterpret_cast<float4 *>(c.rc), cur);
   hipLaunchKernelGGL(k_c_gemv<KK>, dim3(nblk(cg.nc, 64), kCoarseSplit), dim3(256), 0, st, cg, c.ainv, c.rc, c.ecp,
                      cur);
