@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(256) kb_m_outer(const BRve *rv, const int *act
     c.pn = c.pn0;
     c.min_err = 0.0;
     // aim a little below what the outer test needs, but not below what FP32 resolves
-    c.rel_err = pn > 0.0 ? fmax(tau_min, 0.3 * target / pn) : 1.0;
+    c.rel_err = pn > 0.0 ? fmax(tau_min, kInnerAim * target / pn) : 1.0;
     c.max_its = max_inner;
     c.its = 0;
     c.done = conv || cg_done(c);

@@ -158,6 +158,17 @@ void coarse_free(CoarseDev *c);
 // solve is taken closer to the exact Newton step.
 constexpr double kMixedLastStepTighten = 1e-2;
 
+// Mixed-precision defect correction: each FP32 inner solve stops at
+// max(inner_tol, kInnerAim * target / ||D^-1 r||), target being the outer
+// test's on the true FP64 residual.  The outer test alone decides
+// convergence, so this factor only trades inner iterations against the risk
+// of one more outer pass.  A second pass reduces its residual by about 1e-3,
+// where the FP32 recursive residual stays within a fraction of a percent of
+// the true one; on the 50^3 random-sphere RVE with 12 fields the final
+// ||D^-1 r|| came to 0.70 of the target at most (0.30 with a factor of 0.3),
+// with 73 instead of 76-77 inner iterations and still two outer passes.
+constexpr double kInnerAim = 0.7;
+
 // material_base with the derived constants of the reference's constructors
 // (mppd_api.hip); MPPD_ERR_ARG for an unknown law.
 int make_material(const material_base &in, material_base &m);

@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(kRedThreads) k_m_outer(int nparts, const doubl
     s0->pn[r] = pn;
     s0->pn0[r] = pn;
     // aim a little below what the outer test needs, but not below what FP32 can resolve
-    s0->tau[r] = (pn > 0.0) ? fmax(tau_min, 0.3 * target / pn) : 1.0;
+    s0->tau[r] = (pn > 0.0) ? fmax(tau_min, kInnerAim * target / pn) : 1.0;
     s0->alpha[r] = 0.0;
     s0->beta[r] = 0.0;
     s0->done[r] = conv;
