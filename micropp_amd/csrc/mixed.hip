@@ -719,10 +719,16 @@ __global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const do
   __shared__ double lds[4 * 8];
   __shared__ double al_s[4];
   __shared__ float4 rl[COARSE ? kVecNodes : 1];
+  __shared__ float wt[COARSE ? kCoarseMaxLine : 1];
   constexpr int KG = K / 4;
   const int c = blockIdx.y, grp = c % KG, d = c / KG, r0 = grp * 4;
   if (cur->all_done) return;
   if (cur->done[r0] && cur->done[r0 + 1] && cur->done[r0 + 2] && cur->done[r0 + 3]) return;  // whole block
+  // the x-hat weight at offset s of a coarse node's support, the same for
+  // every coarse node: hat(i0 + s, cx, hx) = hat(1 + s, 1, hx) (published by
+  // the barriers below)
+  if constexpr (COARSE)
+    for (int s2 = threadIdx.x; s2 < 2 * cg.hx - 1; s2 += kRedThreads) wt[s2] = hat(1 + s2, 1, cg.hx);
   constexpr int J = kVecNodes / 256;
   int nbeg = blockIdx.x * kVecNodes, nend = min(nbeg + kVecNodes, NP), l0 = 0, l1 = 0;
   if constexpr (COARSE) {
@@ -798,15 +804,16 @@ __global__ void __launch_bounds__(kRedThreads) k_m_upd(int NP, int nsl, const do
   if constexpr (COARSE) {
     // restriction along x of the block's lines (interior lines only)
     for (int idx = threadIdx.x; idx < (l1 - l0) * cg.mx; idx += kRedThreads) {
-      const int line = idx / cg.mx, cx = idx % cg.mx + 1, L = l0 + line;
-      const int j = L % g.ny, k = L / g.ny;
+      const int line = idx / cg.mx, cx = idx - line * cg.mx + 1, L = l0 + line;
+      const int k = L / g.ny, j = L - k * g.ny;
       if (j == 0 || k == 0 || j == g.ny - 1 || k == g.nz - 1) continue;
-      const float4 *rline = rl + line * g.nx;
-      const int i0 = (cx - 1) * cg.hx + 1, n = 2 * cg.hx - 1;
+      const float4 *rline = rl + line * g.nx + (cx - 1) * cg.hx + 1;
+      const int n = 2 * cg.hx - 1;
       float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 3
       for (int s2 = 0; s2 < n; ++s2) {
-        const float w = hat(i0 + s2, cx, cg.hx);
-        const float4 rv = rline[i0 + s2];
+        const float w = wt[s2];
+        const float4 rv = rline[s2];
         acc.x += w * rv.x;
         acc.y += w * rv.y;
         acc.z += w * rv.z;
