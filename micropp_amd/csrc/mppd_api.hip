@@ -726,6 +726,7 @@ struct mppd_msolver {
   // adjoint (allocated on first use): sources, per-phase source table,
   // partials, results (device and pinned)
   double *F = nullptr, *G = nullptr, *apart = nullptr, *adj_d = nullptr, *adj_h = nullptr;
+  void *arena = nullptr, *harena = nullptr;  // the device buffers above through act_d, and the pinned staging
 };
 
 extern "C" {
@@ -747,26 +748,58 @@ int mppd_msolver_create(mppd_msolver **out, mppd_rve *r, int nrhs) {
       break;
     }
     const size_t nv = mixed_vector_len(g, nrhs);
-    // zero fills on m->st (synchronized below); mats_d is written by a blocking copy
+    // The device buffers in one allocation, 256-byte aligned parts, the FP32
+    // matrix last: everything before it is zero-filled on m->st in one fill
+    // (ordered before the set-up kernels; mats_d is then written by a
+    // blocking copy), the matrix is written whole by the assembly.  The
+    // pinned staging in one host allocation.
     hipStream_t st = m->st;
-    if ((rc = dalloc(&m->ke, 3 * 576, st))) break;
-    if ((rc = dalloc(&m->mats_d, 3, false))) break;
-    if ((rc = dalloc(&m->A, mixed_matrix_floats(g), false))) break;
-    if ((rc = dalloc(&m->kinv, nv / nrhs, st))) break;
-    if ((rc = dalloc(&m->X, nv, st))) break;
-    if ((rc = dalloc(&m->R, nv, st))) break;
-    if ((rc = dalloc(&m->P, nv, st))) break;
-    if ((rc = dalloc(&m->AP, nv, st))) break;
-    if ((rc = dalloc(&m->U, nv, st))) break;
-    if ((rc = dalloc(&m->eps_d, 16 * 6, st))) break;
-    if ((rc = dalloc(&m->sig_d, 16 * 6, st))) break;
-    if ((rc = dalloc(&m->part, mixed_partials(g, nrhs), st))) break;
-    if ((rc = dalloc((char **)&m->state, mixed_state_bytes(), st))) break;
-    if ((rc = dalloc(&m->act_d, kMaxRhsFields, st))) break;
-    e = hipHostMalloc(&m->state_h, mixed_state_bytes(), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&m->act_h, kMaxRhsFields * sizeof(int), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&m->eps_h, 16 * 6 * sizeof(double), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&m->sig_h, 16 * 6 * sizeof(double), hipHostMallocDefault);
+    size_t off = 0;
+    auto carve = [&off](size_t bytes) {
+      const size_t o = off;
+      off += (bytes + 255) & ~(size_t)255;
+      return o;
+    };
+    const size_t o_ke = carve(3 * 576 * sizeof(double)), o_mats = carve(3 * sizeof(material_base));
+    const size_t o_kinv = carve(nv / nrhs * sizeof(float)), o_X = carve(nv * sizeof(float)),
+                 o_R = carve(nv * sizeof(float)), o_P = carve(nv * sizeof(float)), o_AP = carve(nv * sizeof(float));
+    const size_t o_U = carve(nv * sizeof(double)), o_eps = carve(16 * 6 * sizeof(double)),
+                 o_sig = carve(16 * 6 * sizeof(double)), o_part = carve(mixed_partials(g, nrhs) * sizeof(double));
+    const size_t o_state = carve(mixed_state_bytes()), o_act = carve(kMaxRhsFields * sizeof(int));
+    const size_t zeroed = off, o_A = carve(mixed_matrix_floats(g) * sizeof(float));
+    e = hipMalloc(&m->arena, off);
+    if (e == hipSuccess) e = hipMemsetAsync(m->arena, 0, zeroed, st);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "msolver buffers");
+      break;
+    }
+    char *base = static_cast<char *>(m->arena);
+    m->ke = reinterpret_cast<double *>(base + o_ke);
+    m->mats_d = reinterpret_cast<material_base *>(base + o_mats);
+    m->kinv = reinterpret_cast<float *>(base + o_kinv);
+    m->X = reinterpret_cast<float *>(base + o_X);
+    m->R = reinterpret_cast<float *>(base + o_R);
+    m->P = reinterpret_cast<float *>(base + o_P);
+    m->AP = reinterpret_cast<float *>(base + o_AP);
+    m->U = reinterpret_cast<double *>(base + o_U);
+    m->eps_d = reinterpret_cast<double *>(base + o_eps);
+    m->sig_d = reinterpret_cast<double *>(base + o_sig);
+    m->part = reinterpret_cast<double *>(base + o_part);
+    m->state = base + o_state;
+    m->act_d = reinterpret_cast<int *>(base + o_act);
+    m->A = reinterpret_cast<float *>(base + o_A);
+    off = 0;
+    const size_t h_state = carve(mixed_state_bytes()), h_act = carve(kMaxRhsFields * sizeof(int)),
+                 h_eps = carve(16 * 6 * sizeof(double)), h_sig = carve(16 * 6 * sizeof(double));
+    e = hipHostMalloc(&m->harena, off, hipHostMallocDefault);
+    if (e == hipSuccess) {
+      char *hb = static_cast<char *>(m->harena);
+      m->state_h = hb + h_state;
+      m->act_h = reinterpret_cast<int *>(hb + h_act);
+      m->eps_h = reinterpret_cast<double *>(hb + h_eps);
+      m->sig_h = reinterpret_cast<double *>(hb + h_sig);
+      e = hipStreamSynchronize(st);  // the fill, before the blocking copy into mats_d
+    }
     if (e == hipSuccess) e = hipMemcpy(m->mats_d, r->mats, 3 * sizeof(material_base), hipMemcpyHostToDevice);
     // element stiffness per phase (FP64, the plain path's kernel), then the FP32 node-block matrix
     if (e == hipSuccess) e = launch_elem_ke(g, r->bmat, r->mats_d, r->et, nullptr, nullptr, 1, m->ke, m->st);
@@ -790,11 +823,8 @@ int mppd_msolver_create(mppd_msolver **out, mppd_rve *r, int nrhs) {
 int mppd_msolver_destroy(mppd_msolver *m) {
   if (!m) return MPPD_OK;
   if (m->st) (void)hipStreamSynchronize(m->st);
-  for (void *p : {(void *)m->ke, (void *)m->mats_d, (void *)m->A, (void *)m->kinv, (void *)m->X, (void *)m->R,
-                  (void *)m->P, (void *)m->AP, (void *)m->U, (void *)m->eps_d, (void *)m->sig_d, (void *)m->part,
-                  m->state, (void *)m->act_d, (void *)m->F, (void *)m->G, (void *)m->apart, (void *)m->adj_d})
-    dfree(p);
-  for (void *p : {m->state_h, (void *)m->eps_h, (void *)m->sig_h, (void *)m->act_h, (void *)m->adj_h})
+  for (void *p : {m->arena, (void *)m->F, (void *)m->G, (void *)m->apart, (void *)m->adj_d}) dfree(p);
+  for (void *p : {m->harena, (void *)m->adj_h})
     if (p) (void)hipHostFree(p);
   coarse_free(m->coarse);
   if (m->st) (void)hipStreamDestroy(m->st);
