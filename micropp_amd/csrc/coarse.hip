@@ -440,8 +440,7 @@ __global__ void __launch_bounds__(256) k_cs_wtw(Band b, const double *Wp, float 
 
 void coarse_free(CoarseDev *c) {
   if (!c) return;
-  for (void *p : {(void *)c->ainv, (void *)c->t1, (void *)c->rc, (void *)c->ecp, (void *)c->ec, (void *)c->z, (void *)c->prz})
-    dfree(p);
+  dfree(c->arena);
   delete c;
 }
 
@@ -544,16 +543,35 @@ int coarse_setup(const Grid &g, const int *et_d, const double *ke_d, int K, hipS
   int rc = MPPD_OK;
   bool ok = false;
   do {
-    float *ainv = nullptr;
-    if ((rc = dalloc(&ainv, (size_t)nc * nc, false))) break;
+    // one allocation, 256-byte aligned parts: the inverse (written whole by
+    // coarse_inverse) last, everything before it zero-filled on st in one
+    // fill (coarse_inverse synchronizes st)
+    size_t off = 0;
+    auto carve = [&off](size_t bytes) {
+      const size_t o = off;
+      off += (bytes + 255) & ~(size_t)255;
+      return o;
+    };
+    const size_t o_t1 = carve((size_t)3 * K * g.nz * g.ny * cg.mx * sizeof(float));
+    const size_t o_rc = carve((size_t)nc * K * sizeof(float)), o_ecp = carve((size_t)kCoarseSplit * nc * K * sizeof(float));
+    const size_t o_ec = carve((size_t)nc * K * sizeof(float)), o_z = carve(mixed_vector_len(g, K) * sizeof(float));
+    const size_t o_prz = carve((size_t)3 * K * coarse_vec_blocks(g) * sizeof(double));
+    const size_t zeroed = off, o_ainv = carve((size_t)nc * nc * sizeof(float));
+    hipError_t e = hipMalloc(&c->arena, off);
+    if (e == hipSuccess) e = hipMemsetAsync(c->arena, 0, zeroed, st);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "coarse buffers");
+      break;
+    }
+    char *base = static_cast<char *>(c->arena);
+    c->t1 = reinterpret_cast<float *>(base + o_t1);
+    c->rc = reinterpret_cast<float *>(base + o_rc);
+    c->ecp = reinterpret_cast<float *>(base + o_ecp);
+    c->ec = reinterpret_cast<float *>(base + o_ec);
+    c->z = reinterpret_cast<float *>(base + o_z);
+    c->prz = reinterpret_cast<double *>(base + o_prz);
+    float *ainv = reinterpret_cast<float *>(base + o_ainv);
     c->ainv = ainv;
-    // zero fills on st (coarse_inverse synchronizes it)
-    if ((rc = dalloc(&c->t1, (size_t)3 * K * g.nz * g.ny * cg.mx, st))) break;
-    if ((rc = dalloc(&c->rc, (size_t)nc * K, st))) break;
-    if ((rc = dalloc(&c->ecp, (size_t)kCoarseSplit * nc * K, st))) break;
-    if ((rc = dalloc(&c->ec, (size_t)nc * K, st))) break;
-    if ((rc = dalloc(&c->z, mixed_vector_len(g, K), st))) break;
-    if ((rc = dalloc(&c->prz, (size_t)3 * K * coarse_vec_blocks(g), st))) break;
     rc = coarse_inverse(g, cg, et_d, ke_d, ainv, st, &ok);
   } while (0);
   if (rc || !ok) {

@@ -129,6 +129,7 @@ struct CoarseDev {
   float *ec;          // coarse correction, nc x K
   float *z;           // the preconditioned residual z, in R's layout
   double *prz;        // partials of (r, z) at the inner CG's start: 3 K x (vector blocks)
+  void *arena;        // the one device allocation holding all of the above
 };
 constexpr int kCoarseSplit = 16;    // column splits of the coarse solve (one partial each)
 constexpr int kCoarseMaxLine = 256;  // nodes per side a coarse level allows
